@@ -1,0 +1,278 @@
+"""MI355X (gfx950) explicit finite-volume residual-and-update path of FVMR.
+
+Host-side mirror of the reference solver surface over the C ABI of libfvmr.so:
+  Mesh          SU2MeshReader + FVM_2D + OldDataConverter (FVMR/src/input/SU2MeshReader.cpp,
+                FVMR/src/legacy/FVM_2D.cpp, FVMR/src/solvers/OldDataConverter.cpp)
+  field_init    FieldInitializer (FVMR/src/input/FieldInitializer.cpp:22-229)
+  GPUSolver2    GPU::GPUSolver2 + CIteration::iteration_device
+                (FVMR/src/solvers/GPUSolver2.h:32-89, FVMR/src/time/CIteration.cpp:89-161)
+  config        input.toml keys of TomlFileManager (FVMR/src/input/TomlFileManager.cpp:110-265)
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import *  # noqa: F401,F403  (codes)
+from ._abi import FvmrError, GenParams, MeshSoA, Params, StepInfo, check, lib
+
+__all__ = ["Mesh", "GPUSolver2", "field_init", "default_params", "ruvp_from_mach", "FvmrError",
+           "Params", "StepInfo", "MeshSoA", "lib"]
+
+
+def default_params(**kw):
+    """GlobalPara defaults (FVMR/src/global/GlobalPara.cpp:4-90) with overrides."""
+    p = Params()
+    check(lib().fvmr_params_default(C.byref(p)), "fvmr_params_default")
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(f"fvmr_params has no field {k}")
+        if k.endswith("_ruvp"):
+            arr = getattr(p, k)
+            for i in range(4):
+                arr[i] = float(v[i])
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def ruvp_from_mach(Ma, AoA, T0=288.16, p0=101325.0, gamma=1.4, R=287.06):
+    out = (C.c_double * 4)()
+    check(lib().fvmr_ruvp_from_mach(Ma, AoA, T0, p0, gamma, R, out), "fvmr_ruvp_from_mach")
+    return np.array(out[:])
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class Mesh:
+    """Host mesh in the reference SoA layout (GPU::NodeSoA/ElementSoA/EdgeSoA/BoundarySetMap)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+        self.soa = MeshSoA()
+        check(lib().fvmr_mesh_soa_view(self._h, C.byref(self.soa)), "fvmr_mesh_soa_view")
+
+    @classmethod
+    def read_su2(cls, path, equation=EQ_EULER, scale=1.0):
+        h = C.c_void_p()
+        check(lib().fvmr_mesh_read_su2(str(path).encode(), equation, scale, C.byref(h)), "fvmr_mesh_read_su2")
+        return cls(h.value)
+
+    @classmethod
+    def generate(cls, family, ni, nj, a=0.0, b=0.0, c=0.0, d=0.0, jitter=0.0, seed=0, naca_digits=12,
+                 equation=EQ_EULER, isothermal_wall=False):
+        g = GenParams(ni=ni, nj=nj, a=a, b=b, c=c, d=d, jitter=jitter, seed=seed, naca_digits=naca_digits,
+                      reserved=1 if isothermal_wall else 0)
+        h = C.c_void_p()
+        check(lib().fvmr_mesh_generate(family, C.byref(g), equation, C.byref(h)), "fvmr_mesh_generate")
+        return cls(h.value)
+
+    @classmethod
+    def from_lists(cls, x, y, node_ids, elem_types, elem_conn, markers, equation=EQ_EULER):
+        """markers: list of (name, [(n0, n1), ...]) with 0-based file node ids."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        ids = np.ascontiguousarray(node_ids, dtype=np.int32)
+        et = np.ascontiguousarray(elem_types, dtype=np.int32)
+        ec = np.full((len(et), 4), -1, dtype=np.int32)
+        conn = np.asarray(elem_conn)
+        ec[:, :conn.shape[1]] = conn
+        names = (C.c_char_p * max(1, len(markers)))(*[m[0].encode() for m in markers])
+        offs = np.zeros(len(markers) + 1, dtype=np.int32)
+        pairs = []
+        for k, (_, e) in enumerate(markers):
+            pairs.extend(e)
+            offs[k + 1] = len(pairs)
+        en = np.ascontiguousarray(np.array(pairs, dtype=np.int32).reshape(-1, 2)) if pairs else np.zeros((1, 2), np.int32)
+        h = C.c_void_p()
+        check(lib().fvmr_mesh_from_lists(len(x), _dptr(x), _dptr(y), _iptr(ids), len(et), _iptr(et), _iptr(ec),
+                                         len(markers), names, _iptr(offs), _iptr(en), equation, C.byref(h)),
+              "fvmr_mesh_from_lists")
+        return cls(h.value)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            try:
+                lib().fvmr_mesh_free(self._h)
+            except Exception:
+                pass
+            self._h = C.c_void_p()
+
+    @property
+    def num_element(self):
+        return self.soa.num_element
+
+    @property
+    def num_edge(self):
+        return self.soa.num_edge
+
+    @property
+    def num_node(self):
+        return self.soa.num_node
+
+    def write_su2(self, path):
+        check(lib().fvmr_mesh_write_su2(self._h, str(path).encode()), "fvmr_mesh_write_su2")
+
+    def arrays(self):
+        """numpy copies of every SoA array (keys follow the reference member names)."""
+        return soa_arrays(self.soa)
+
+    def element_node_ids(self):
+        p = C.POINTER(C.c_int)()
+        check(lib().fvmr_mesh_element_node_ids(self._h, C.byref(p)), "fvmr_mesh_element_node_ids")
+        return np.ctypeslib.as_array(p, shape=(self.num_element * 3,)).reshape(-1, 3).copy()
+
+    def boundary_sets(self):
+        out = []
+        for s in range(self.soa.num_boundary_set):
+            name = C.c_char_p()
+            typ = C.c_int()
+            edges = C.POINTER(C.c_int)()
+            n = C.c_int()
+            check(lib().fvmr_mesh_boundary_set(self._h, s, C.byref(name), C.byref(typ), C.byref(edges), C.byref(n)),
+                  "fvmr_mesh_boundary_set")
+            e = np.ctypeslib.as_array(edges, shape=(n.value,)).copy() if n.value else np.zeros(0, np.int32)
+            out.append((name.value.decode(), typ.value, e))
+        return out
+
+
+def soa_arrays(soa):
+    """numpy copies of the arrays of an fvmr_mesh_soa view."""
+    nv, ne, nf, nb = soa.num_node, soa.num_element, soa.num_edge, soa.num_boundary_set
+
+    def d(p, n):
+        return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+    return {
+        "node_xy": np.stack([d(soa.node_xy[k], nv) for k in range(2)]),
+        "element_nodes": np.stack([d(soa.element_nodes[k], ne) for k in range(4)]),
+        "element_edges": np.stack([d(soa.element_edges[k], ne) for k in range(4)]),
+        "element_neighbors": np.stack([d(soa.element_neighbors[k], ne) for k in range(4)]),
+        "element_xy": np.stack([d(soa.element_xy[k], ne) for k in range(2)]),
+        "element_volume": d(soa.element_volume, ne),
+        "edge_nodes": np.stack([d(soa.edge_nodes[k], nf) for k in range(2)]),
+        "edge_setID": d(soa.edge_setID, nf),
+        "edge_periodicPair": d(soa.edge_periodicPair, nf),
+        "edge_elementL": d(soa.edge_elementL, nf),
+        "edge_elementR": d(soa.edge_elementR, nf),
+        "edge_length": d(soa.edge_length, nf),
+        "edge_xy": np.stack([d(soa.edge_xy[k], nf) for k in range(2)]),
+        "edge_normal": np.stack([d(soa.edge_normal[k], nf) for k in range(2)]),
+        "boundary_type": d(soa.boundary_type, nb) if nb else np.zeros(0, np.int32),
+    }
+
+
+def _rows(U):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    return U, (C.POINTER(C.c_double) * 4)(*[_dptr(U[k]) for k in range(4)])
+
+
+def field_init(mesh, params, init_type=INIT_UNIFORM, args=()):
+    """FieldInitializer::setInitialAndBoundaryCondition: U (4 x num_element)."""
+    U = np.zeros((4, mesh.num_element))
+    a = np.zeros(max(4, len(args)))
+    a[:len(args)] = args
+    U, rows = _rows(U)
+    check(lib().fvmr_field_init(C.byref(mesh.soa), C.byref(params), init_type, _dptr(a), rows), "fvmr_field_init")
+    return U
+
+
+class GPUSolver2:
+    """Device solver: GPU::GPUSolver2 (allocate/initialize/host_to_device) + CIteration."""
+
+    PROF_STAGE, PROF_GRADIENT, PROF_DT, PROF_RESTRICT, PROF_RESIDUAL = range(5)
+
+    def __init__(self, mesh, params):
+        self.mesh = mesh  # keeps the SoA storage alive
+        self.params = params
+        self._h = C.c_void_p()
+        check(lib().fvmr_create(C.byref(mesh.soa), C.byref(params), C.byref(self._h)), "fvmr_create")
+        self.n = mesh.num_element
+
+    def close(self):
+        if self._h and self._h.value:
+            check(lib().fvmr_destroy(self._h), "fvmr_destroy")
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_state(self, U):
+        U, rows = _rows(U)
+        check(lib().fvmr_set_state(self._h, rows), "fvmr_set_state")
+
+    def get_state(self):
+        U = np.zeros((4, self.n))
+        U, rows = _rows(U)
+        check(lib().fvmr_get_state(self._h, rows), "fvmr_get_state")
+        return U
+
+    def iteration_device(self, t, T, residual_norm=0):
+        """CIteration::iteration_device: one step; returns (t, StepInfo)."""
+        tt = C.c_double(t)
+        info = StepInfo(residual_norm=residual_norm)
+        check(lib().fvmr_step(self._h, C.byref(tt), T, C.byref(info)), "fvmr_step")
+        return tt.value, info
+
+    def advance(self, nsteps, t, T, residual_norm=0):
+        tt = C.c_double(t)
+        info = StepInfo(residual_norm=residual_norm)
+        check(lib().fvmr_advance(self._h, nsteps, C.byref(tt), T, C.byref(info)), "fvmr_advance")
+        return tt.value, info
+
+    def stage_rhs(self):
+        R = np.zeros((4, self.n))
+        R, rows = _rows(R)
+        check(lib().fvmr_stage_rhs(self._h, rows), "fvmr_stage_rhs")
+        return R
+
+    def gradient(self):
+        gx, rx = _rows(np.zeros((4, self.n)))
+        gy, ry = _rows(np.zeros((4, self.n)))
+        check(lib().fvmr_gradient(self._h, rx, ry), "fvmr_gradient")
+        return gx, gy
+
+    def global_dt(self):
+        dt = C.c_double()
+        check(lib().fvmr_global_dt(self._h, C.byref(dt)), "fvmr_global_dt")
+        return dt.value
+
+    def updateResidualVector(self, norm=NORM_INF):
+        out = (C.c_double * 4)()
+        check(lib().fvmr_residual(self._h, norm, out), "fvmr_residual")
+        return np.array(out[:])
+
+    def permutation(self):
+        p = np.zeros(self.n, dtype=np.int32)
+        check(lib().fvmr_permutation(self._h, _iptr(p)), "fvmr_permutation")
+        return p
+
+    def profile(self, on=True):
+        check(lib().fvmr_profile_enable(self._h, 1 if on else 0), "fvmr_profile_enable")
+
+    def profile_query(self, kernel):
+        ms = C.c_double()
+        n = C.c_int()
+        check(lib().fvmr_profile_query(self._h, kernel, C.byref(ms), C.byref(n)), "fvmr_profile_query")
+        return ms.value, n.value
+
+    def profile_reset(self):
+        check(lib().fvmr_profile_reset(self._h), "fvmr_profile_reset")
+
+    def synchronize(self):
+        check(lib().fvmr_synchronize(self._h), "fvmr_synchronize")

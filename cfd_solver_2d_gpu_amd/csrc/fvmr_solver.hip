@@ -1,0 +1,1382 @@
+// Device path of the FVMR explicit residual-and-update step for MI355X (gfx950).
+//
+// One step (CIteration::iteration_device, FVMR/src/time/CIteration.cpp:89-161):
+//   dt kernel      per cell: spectral radius summed over the edges touching the cell in
+//                  the reference edge order, dt_c = CFL V / alpha; block min, last block
+//                  finishes the min, clamps to T - t and advances t on the device.
+//   grad kernel    per cell: Green-Gauss or least-squares gradient, Barth or
+//                  Venkatakrishnan limiter; writes the limited Ux, Uy.
+//   stage kernel   per cell: for each of its faces the canonical edge flux (linear
+//                  reconstruction, BC ghost state, Roe/LLF, viscous part) is evaluated
+//                  and summed with the reference's sign and slot order (cell-owned
+//                  gather, no atomics), divided by -V and folded into the RK update.
+//   restrict       single work-group, only when the stage kernel flagged cells:
+//                  repaired values from the neighbours (Jacobi reads), then written.
+// The state lives in three SoA buffers (U^n, stage in, stage out) that rotate.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/fvmr.h"
+#include "fvmr_internal.h"
+
+using fvmr::set_error;
+
+namespace {
+
+constexpr int kBlock = 256;      // 4 wave64 per work-group
+constexpr int kMaxDtSlots = 6;   // own 3 edges + pair edges of periodic faces
+constexpr int kNumProf = 5;
+
+#define HIP_OK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return set_error(FVMR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Device-side arguments (plain pointers, passed by value).
+// ---------------------------------------------------------------------------
+template <class G>
+struct Mesh {
+    int nc, nf;
+    const G *cx, *cy, *vol;          // centroid, area
+    const int *cface;                // [3][nc] edge of local slot j
+    const int *cnode;                // [3][nc]
+    const G *nx_, *ny_;              // node coordinates
+    const int2 *eLR;                 // L, R (R = pair's L for periodic, -1 boundary)
+    const G *enx, *eny, *elen;       // normal (L->R) and length
+    const G *emx, *emy;              // midpoint
+    const int *epair;                // periodic pair edge or -1
+    const int *etype;                // BC code of the edge's set, -1 interior
+    const int *dtedge;               // [kMaxDtSlots][nc] edges adding to alphaC, reference order
+};
+
+struct Phys {
+    double gamma, R, Pr, Pr_visc, mu0, sutherland_C1, CFL, wall_T, venkat_K;
+    double dsr_x, dsr_y, dsr_speed;
+    double inf[4], inlet[4], outlet[4];
+    int equation, scheme, reconstruct, gradient, limiter, mu_model, compat_inv;
+};
+
+// Device scalars of a handle: t, dt, T, t at the start of the step.
+struct Scalars {
+    double t, dt, T, t_rk;
+    unsigned int counter;       // last-block-done counter of the dt reduction
+    unsigned int norm_counter;  // same for the residual norms
+    int bad_count;              // cells flagged by the stage kernel
+    int n_restricted;           // cells repaired in the current step
+};
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous
+// range of logical blocks so that neighbouring cells share an L2.
+__device__ __forceinline__ int xcd_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
+template <class T> __device__ __forceinline__ T tmin(T a, T b) { return (a < b) ? a : b; }
+template <class T> __device__ __forceinline__ T tmax(T a, T b) { return (a > b) ? a : b; }
+
+// PhysicalKernelGPU.h:11-22
+template <class T>
+__device__ __forceinline__ void u2ruvp(const T U[4], T r[4], T gamma) {
+    r[0] = U[0];
+    r[1] = U[1] / U[0];
+    r[2] = U[2] / U[0];
+    r[3] = r[0] * (gamma - 1) * (U[3] / U[0] - (r[1] * r[1] + r[2] * r[2]) * T(0.5));
+}
+template <class T>
+__device__ __forceinline__ void ruvp2u(const T r[4], T U[4], T gamma) {
+    U[0] = r[0];
+    U[1] = r[0] * r[1];
+    U[2] = r[0] * r[2];
+    U[3] = r[3] / (gamma - 1) + T(0.5) * r[0] * (r[1] * r[1] + r[2] * r[2]);
+}
+// RestrictGPU.h:11-19
+template <class T>
+__device__ __forceinline__ bool out_of_range(const T r[4]) {
+    return r[0] < T(fvmr::kRHO_MIN) || r[0] > T(fvmr::kRHO_MAX) || r[3] < T(fvmr::kP_MIN) || r[3] > T(fvmr::kP_MAX);
+}
+__device__ __forceinline__ bool is_periodic(int t) { return t >= FVMR_BC_PERIODIC_0 && t <= FVMR_BC_PERIODIC_0 + 9; }
+
+// ---------------------------------------------------------------------------
+// Face physics (FluxGPU.cu:34-205, RoeGPU.cu, LocalLaxFriedrichsGPU.cu)
+// ---------------------------------------------------------------------------
+// get_Uvector_reconstruct_2_device (FluxGPU.cu:47-61)
+template <class C, class G>
+__device__ __forceinline__ void reconstruct(const Mesh<G> &m, const Phys &ph, const double *__restrict__ U,
+                                            const G *__restrict__ Ux, const G *__restrict__ Uy, int c, C x, C y,
+                                            C Ud[4]) {
+    const int nc = m.nc;
+    C Uc[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) Uc[i] = (C)U[i * nc + c];
+    if (ph.reconstruct == FVMR_REC_LINEAR) {
+        const C dx = x - (C)m.cx[c], dy = y - (C)m.cy[c];
+#pragma unroll
+        for (int i = 0; i < 4; i++) Ud[i] = Uc[i] + (C)Ux[i * nc + c] * dx + (C)Uy[i * nc + c] * dy;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) Ud[i] = Uc[i];
+    }
+    C r[4];
+    u2ruvp(Ud, r, (C)ph.gamma);
+    if (out_of_range(r)) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) Ud[i] = Uc[i];
+    }
+}
+
+// get_UR_wallNonViscous_device (FluxGPU.cu:63-77)
+template <class C>
+__device__ __forceinline__ void ghost_slip(const C UL[4], C UR[4], C nx, C ny) {
+    C uxL = UL[1] / UL[0], uyL = UL[2] / UL[0];
+    C unL = uxL * nx + uyL * ny;
+    C utL = uxL * ny - uyL * nx;
+    C unR = -unL, utR = utL;
+    C uxR = unR * nx + utR * ny;
+    C uyR = unR * ny - utR * nx;
+    UR[0] = UL[0];
+    UR[1] = UR[0] * uxR;
+    UR[2] = UR[0] * uyR;
+    UR[3] = UL[3];
+}
+
+// get_UR_farField_device (FluxGPU.cu:87-167)
+template <class C>
+__device__ __forceinline__ void ghost_farfield(const C UL[4], const double *ruvp_inf, C UR[4], C nx, C ny, C gamma) {
+    const C ga1 = gamma - 1;
+    const C rho_f = (C)ruvp_inf[0], u_f = (C)ruvp_inf[1], v_f = (C)ruvp_inf[2], p_f = (C)ruvp_inf[3];
+    C e[4];
+    u2ruvp(UL, e, gamma);
+    const C rho_e = e[0], u_e = e[1], v_e = e[2], p_e = e[3];
+    const C a2_f = gamma * p_f / rho_f;
+    const C af = sqrt(a2_f);
+    const C ae = sqrt(gamma * p_e / rho_e);
+    const C qnf = u_f * nx + v_f * ny;
+    const C qne = u_e * nx + v_e * ny;
+    const C rf = qnf - C(2.0) * af / ga1;
+    const C re = qne + C(2.0) * ae / ga1;
+    const C qn = C(0.5) * (re + rf);
+    const C as = ga1 * (re - rf) / C(4.0);
+    C r0[4];
+    if (qn <= -as) {
+        r0[0] = rho_f; r0[1] = u_f; r0[2] = v_f; r0[3] = p_f;
+    } else if (qn < 0) {
+        const C son = p_f / pow(rho_f, gamma);
+        const C qtx = u_f - qnf * nx, qty = v_f - qnf * ny;
+        r0[0] = pow((as * as / son / gamma), C(1.0) / ga1);
+        r0[1] = qtx + qn * nx; r0[2] = qty + qn * ny;
+        r0[3] = as * as * r0[0] / gamma;
+    } else if (qn < as) {
+        const C son = p_e / pow(rho_e, gamma);
+        const C qtx = u_e - qne * nx, qty = v_e - qne * ny;
+        r0[0] = pow((as * as / son / gamma), C(1.0) / ga1);
+        r0[1] = qtx + qn * nx; r0[2] = qty + qn * ny;
+        r0[3] = as * as * r0[0] / gamma;
+    } else {
+        r0[0] = rho_e; r0[1] = u_e; r0[2] = v_e; r0[3] = p_e;
+    }
+    ruvp2u(r0, UR, gamma);
+}
+
+// New: no-slip isothermal wall ghost (u_R = -u_L, p_R = p_L, T_R = 2 T_w - T_L, floored).
+template <class C>
+__device__ __forceinline__ void ghost_isothermal(const C UL[4], C UR[4], C Tw, C R, C gamma) {
+    C r[4];
+    u2ruvp(UL, r, gamma);
+    C TL = r[3] / (r[0] * R);
+    C TR = C(2.0) * Tw - TL;
+    if (TR < C(0.5) * Tw) TR = C(0.5) * Tw;
+    C g[4] = {r[3] / (R * TR), -r[1], -r[2], r[3]};
+    ruvp2u(g, UR, gamma);
+}
+
+// LocalLaxFriedrichs2d (LocalLaxFriedrichsGPU.cu:4-49)
+template <class C>
+__device__ __forceinline__ void flux_llf(const C UL[4], const C UR[4], C nx, C ny, C length, C flux[4], C gamma) {
+    C L[4], R[4];
+    u2ruvp(UL, L, gamma);
+    u2ruvp(UR, R, gamma);
+    const C rL = L[0], uL = L[1], vL = L[2], pL = L[3];
+    const C rR = R[0], uR = R[1], vR = R[2], pR = R[3];
+    const C unL = nx * uL + ny * vL, unR = nx * uR + ny * vR;
+    C FnL[4], FnR[4];
+    FnL[0] = rL * unL; FnL[1] = rL * unL * uL + nx * pL; FnL[2] = rL * unL * vL + ny * pL;
+    const C hl = pL * gamma / (rL * (gamma - C(1.))) + C(0.5) * (uL * uL + vL * vL);
+    FnL[3] = rL * unL * hl;
+    FnR[0] = rR * unR; FnR[1] = rR * unR * uR + nx * pR; FnR[2] = rR * unR * vR + ny * pR;
+    const C hr = pR * gamma / (rR * (gamma - C(1.))) + C(0.5) * (uR * uR + vR * vR);
+    FnR[3] = rR * unR * hr;
+    const C lam = tmax(fabs(unL) + sqrt(gamma * pL / rL), fabs(unR) + sqrt(gamma * pR / rR));
+#pragma unroll
+    for (int i = 0; i < 4; i++) flux[i] = C(0.5) * (FnL[i] + FnR[i] - lam * (UR[i] - UL[i])) * length;
+}
+
+// ConvectRoeCommon2d + RoeDissapationTerm2d (RoeGPU.cu:4-159): no entropy fix
+template <class C>
+__device__ __forceinline__ void flux_roe(const C UL[4], const C UR[4], C nx, C ny, C area, C F[4], C gamma) {
+    C L[4], R[4];
+    u2ruvp(UL, L, gamma);
+    u2ruvp(UR, R, gamma);
+    const C rL = L[0], uL = L[1], vL = L[2], pL = L[3];
+    const C rR = R[0], uR = R[1], vR = R[2], pR = R[3];
+    const C ul = nx * uL + ny * vL, ur = nx * uR + ny * vR;
+    const C rul = rL * ul, rur = rR * ur;
+    const C V2L = uL * uL + vL * vL, V2R = uR * uR + vR * vR;
+    const C ga1 = gamma - C(1.0);
+    const C HL = pL * gamma / (rL * ga1) + C(0.5) * V2L;
+    const C HR = pR * gamma / (rR * ga1) + C(0.5) * V2R;
+    const C psum = pL + pR;
+    F[0] = C(0.5) * (rul + rur) * area;
+    F[1] = C(0.5) * (rul * uL + rur * uR + nx * psum) * area;
+    F[2] = C(0.5) * (rul * vL + rur * vR + ny * psum) * area;
+    F[3] = C(0.5) * (rul * HL + rur * HR) * area;
+    const C rrorl = sqrt(rR / rL);
+    const C rrorlp1 = rrorl + C(1.0);
+    const C rm = sqrt(rR * rL);
+    const C um = (uL + uR * rrorl) / rrorlp1;
+    const C vm = (vL + vR * rrorl) / rrorlp1;
+    const C Hm = (HL + HR * rrorl) / rrorlp1;
+    const C Vm2 = um * um + vm * vm;
+    const C am2 = ga1 * (Hm - C(0.5) * Vm2);
+    const C am = sqrt(am2);
+    const C unm = um * nx + vm * ny;
+    const C eig1 = fabs(unm);
+    const C eig2 = fabs(unm + am);
+    const C eig3 = fabs(unm - am);
+    const C ER = HR - pR / rR, EL = HL - pL / rL;
+    const C Etm = Hm / gamma + ga1 / gamma * C(0.5) * Vm2;
+    const C dW0 = rR - rL;
+    const C dW1 = rm * (uR - uL) + dW0 * um;
+    const C dW2 = rm * (vR - vL) + dW0 * vm;
+    const C dW3 = rm * (ER - EL) + dW0 * Etm;
+    const C astar = (eig2 + eig3) / C(2.0);
+    const C Mstar = (eig2 - eig3) / C(2.0) / am;
+    const C dun = nx * (uR - uL) + ny * (vR - vL);
+    const C dUroe = Mstar * dun + (astar - eig1) * (pR - pL) / rm / am2;
+    const C dProe = Mstar * (pR - pL) + (astar - eig1) * rm * dun;
+    F[0] -= C(0.5) * (eig1 * dW0 + dUroe * rm) * area;
+    F[1] -= C(0.5) * (eig1 * dW1 + dUroe * rm * um + dProe * nx) * area;
+    F[2] -= C(0.5) * (eig1 * dW2 + dUroe * rm * vm + dProe * ny) * area;
+    F[3] -= C(0.5) * (eig1 * dW3 + dUroe * rm * Hm + dProe * unm) * area;
+}
+
+__device__ __forceinline__ double mu_of_T(const Phys &ph, double T) {
+    if (ph.mu_model == FVMR_MU_CONSTANT) return ph.mu0;
+    // get_mu_using_Sutherland_air_host_device (ViscousFluxGPU.h:30-32)
+    return ph.sutherland_C1 * pow(T, 1.5) / (T + 110.4);
+}
+
+// viscous_flux_device_kernel body (ViscousFluxGPU.cu:139-229); returns F_v (to be subtracted)
+template <class C, class G>
+__device__ __forceinline__ void flux_viscous(const Mesh<G> &m, const Phys &ph, const double *__restrict__ U,
+                                             const G *__restrict__ Ux, const G *__restrict__ Uy, int e, int L,
+                                             int R, C nx, C ny, C area, C F[4]) {
+    const int nc = m.nc;
+    const C ga1 = (C)ph.gamma - C(1.0);
+    const C Cv = (C)ph.R / ga1;
+    const C Cp = Cv * (C)ph.gamma;
+    const bool hasR = !(R < 0 || R >= nc);
+    C gx[4], gy[4], Um[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (hasR) {
+            gx[k] = C(0.5) * ((C)Ux[k * nc + L] + (C)Ux[k * nc + R]);
+            gy[k] = C(0.5) * ((C)Uy[k * nc + L] + (C)Uy[k * nc + R]);
+            Um[k] = C(0.5) * ((C)U[k * nc + L] + (C)U[k * nc + R]);
+        } else {
+            gx[k] = (C)Ux[k * nc + L];
+            gy[k] = (C)Uy[k * nc + L];
+            Um[k] = (C)U[k * nc + L];
+        }
+    }
+    const C one_on_rho = C(1.0) / Um[0];
+    const C u = Um[1] * one_on_rho, v = Um[2] * one_on_rho, E = Um[3] * one_on_rho;
+    const C dudx = one_on_rho * (gx[1] - u * gx[0]);
+    const C dvdx = one_on_rho * (gx[2] - v * gx[0]);
+    const C dudy = one_on_rho * (gy[1] - u * gy[0]);
+    const C dvdy = one_on_rho * (gy[2] - v * gy[0]);
+    const C dEdx = one_on_rho * (gx[3] - E * gx[0]);
+    const C dEdy = one_on_rho * (gy[3] - E * gy[0]);
+    const C one_on_Cv = C(1.0) / Cv;
+    const C V2 = u * u + v * v;
+    const C T = one_on_Cv * (E - C(0.5) * V2);
+    const C dTdx = (dEdx - u * dudx - v * dvdx) * one_on_Cv;
+    const C dTdy = (dEdy - u * dudy - v * dvdy) * one_on_Cv;
+    const C mu = (C)mu_of_T(ph, (double)T);
+    const C NMD = C(-2.0 / 3.0);
+    const C div = dudx + dvdy;
+    const C txx = mu * (NMD * div + C(2.0) * dudx);
+    const C tyy = mu * (NMD * div + C(2.0) * dvdy);
+    const C txy = mu * (dudy + dvdx);
+    const C akmu = Cp * (mu / (C)ph.Pr_visc);
+    const C Phix = akmu * dTdx + u * txx + v * txy;
+    const C Phiy = akmu * dTdy + u * txy + v * tyy;
+    F[0] = C(0.0);
+    F[1] = (txx * nx + txy * ny) * area;
+    F[2] = (txy * nx + tyy * ny) * area;
+    F[3] = (Phix * nx + Phiy * ny) * area;
+}
+
+// edge_convection_flux_device_kernel body (FluxGPU.cu:207-283) (+ viscous part for NS):
+// the canonical flux of edge e, oriented L -> R.
+template <class C, class G>
+__device__ __forceinline__ void edge_flux(const Mesh<G> &m, const Phys &ph, const double *__restrict__ U,
+                                          const G *__restrict__ Ux, const G *__restrict__ Uy, double t_rk, int e,
+                                          C F[4]) {
+    const int bType = m.etype[e];
+    const C nx = (C)m.enx[e], ny = (C)m.eny[e], len = (C)m.elen[e];
+    const C xe = (C)m.emx[e], ye = (C)m.emy[e];
+    const int2 lr = m.eLR[e];
+    const C gamma = (C)ph.gamma;
+    C UL[4], UR[4] = {C(0), C(0), C(0), C(0)};
+    reconstruct<C, G>(m, ph, U, Ux, Uy, lr.x, xe, ye, UL);
+    switch (bType) {
+    case FVMR_BC_SYMMETRY:
+    case FVMR_BC_WALL_NONVISCOUS: ghost_slip(UL, UR, nx, ny); break;
+    case FVMR_BC_WALL_ADIABAT:  // FluxGPU.cu:79-85 (momenta of a zeroed U_R negated)
+        UR[0] = UL[0]; UR[1] = -C(0); UR[2] = -C(0); UR[3] = UL[3];
+        break;
+    case FVMR_BC_WALL_ISOTHERMAL: ghost_isothermal(UL, UR, (C)ph.wall_T, (C)ph.R, gamma); break;
+    case FVMR_BC_INLET: ghost_farfield(UL, ph.inlet, UR, nx, ny, gamma); break;
+    case FVMR_BC_OUTLET: ghost_farfield(UL, ph.outlet, UR, nx, ny, gamma); break;
+    case FVMR_BC_INF: ghost_farfield(UL, ph.inf, UR, nx, ny, gamma); break;
+    case FVMR_BC_DSR: {
+        const double right = ph.dsr_x + ((double)m.emy[e] + ph.dsr_speed * t_rk) / sqrt(3.0);
+        if ((double)m.emx[e] < right) ghost_farfield(UL, ph.inlet, UR, nx, ny, gamma);
+        else ghost_farfield(UL, ph.outlet, UR, nx, ny, gamma);
+        break;
+    }
+    default:
+        if (lr.y != -1) {
+            const int pe = m.epair[e];
+            if (bType != -1) {  // periodic: R reconstructed at the pair's midpoint
+                if (pe >= 0 && pe < m.nf) reconstruct<C, G>(m, ph, U, Ux, Uy, lr.y, (C)m.emx[pe], (C)m.emy[pe], UR);
+            } else {
+                reconstruct<C, G>(m, ph, U, Ux, Uy, lr.y, xe, ye, UR);
+            }
+        }
+    }
+    if (ph.scheme == FVMR_SOL_LLF) flux_llf(UL, UR, nx, ny, len, F, gamma);
+    else flux_roe(UL, UR, nx, ny, len, F, gamma);
+    if (ph.equation == FVMR_EQ_NS) {
+        C Fv[4];
+        flux_viscous<C, G>(m, ph, U, Ux, Uy, e, lr.x, lr.y, nx, ny, len, Fv);
+#pragma unroll
+        for (int k = 0; k < 4; k++) F[k] -= Fv[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+// Gradient_2 (GradientGPU.cu:513-551): GG (GradientGreenGaussKernel :223-425) or LSQ
+// (Gradient.cpp:160-246, correct inverse unless compat), then Barth (:453-502) or Venkatakrishnan.
+template <class C, class G>
+__global__ __launch_bounds__(kBlock) void k_gradient(Mesh<G> m, Phys ph, const double *__restrict__ U,
+                                                     G *__restrict__ Ux, G *__restrict__ Uy) {
+    const int c = xcd_block() * kBlock + threadIdx.x;
+    const int nc = m.nc;
+    if (c >= nc) return;
+    C Uc[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) Uc[k] = (C)U[k * nc + c];
+    const C xc = (C)m.cx[c], yc = (C)m.cy[c];
+    int out[3], nbr[3];
+    C sgn[3], Sx[3], Sy[3], gC[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int e = m.cface[i * nc + c];
+        const int2 lr = m.eLR[e];
+        const int pe = m.epair[e];
+        if (c != lr.y) { sgn[i] = C(1); out[i] = lr.y; }
+        else { sgn[i] = C(-1); out[i] = lr.x; }
+        // element neighbours exclude periodic faces (ElementSoA.h:22)
+        nbr[i] = (pe >= 0) ? -1 : out[i];
+        const C len = (C)m.elen[e];
+        Sx[i] = (C)m.enx[e] * len;
+        Sy[i] = (C)m.eny[e] * len;
+        gC[i] = C(0.5);
+        if (ph.gradient == FVMR_GRA_GREENGAUSS && out[i] != -1) {
+            const C fx = (C)m.emx[e], fy = (C)m.emy[e];
+            C Fx = (C)m.cx[out[i]], Fy = (C)m.cy[out[i]];
+            if (m.etype[e] != -1 && pe >= 0) {
+                Fx += fx - (C)m.emx[pe];
+                Fy += fy - (C)m.emy[pe];
+            }
+            const C ax = fx - Fx, ay = fy - Fy, bx = xc - Fx, by = yc - Fy;
+            const C dFf = sqrt(ax * ax + ay * ay);
+            C dFC = sqrt(bx * bx + by * by);
+            if (dFC < C(fvmr::kEPS)) dFC = C(fvmr::kEPS);
+            gC[i] = dFf / dFC;
+        }
+    }
+    C Uo[3][4];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) Uo[i][k] = (out[i] != -1) ? (C)U[k * nc + out[i]] : Uc[k];
+    C gx[4], gy[4];
+    if (ph.gradient == FVMR_GRA_GREENGAUSS) {
+        const C inv_vol = C(1) / (C)m.vol[c];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            C s0 = C(0), s1 = C(0);
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                C phif;
+                if (out[i] != -1) phif = gC[i] * Uc[k] + (C(1.0) - gC[i]) * Uo[i][k];
+                else phif = Uc[k];
+                s0 += phif * Sx[i] * sgn[i];
+                s1 += phif * Sy[i] * sgn[i];
+            }
+            gx[k] = inv_vol * s0;
+            gy[k] = inv_vol * s1;
+        }
+    } else {
+        // least squares over the valid (non-periodic) neighbours in slot order; sums skip
+        // the invalid slots, so the order equals the reference's compacted neighbour list
+        C dX[3][2];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            dX[i][0] = (nbr[i] != -1) ? (C)m.cx[nbr[i]] - xc : C(0);
+            dX[i][1] = (nbr[i] != -1) ? (C)m.cy[nbr[i]] - yc : C(0);
+        }
+        C A[4];
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 2; y++) {
+                C acc = C(0);
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+                    if (nbr[i] != -1) acc += dX[i][x] * dX[i][y];
+                A[x * 2 + y] = acc;
+            }
+        C det = A[0] * A[3] - A[1] * A[2];
+        const C eps = C(fvmr::kEPS);
+        if (ph.compat_inv) {  // MatrixKernel.h:48-49 as written
+            if (det >= C(0) && det < eps) det = eps;
+            else if (det > -eps) det = -eps;
+        } else {
+            if (det >= C(0) && det < eps) det = eps;
+            else if (det < C(0) && det > -eps) det = -eps;
+        }
+        const C idet = C(1.0) / det;
+        const C i00 = A[3] * idet, i01 = -A[1] * idet, i10 = -A[2] * idet, i11 = A[0] * idet;
+        C B0[3], B1[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            B0[i] = i00 * dX[i][0] + i01 * dX[i][1];
+            B1[i] = i10 * dX[i][0] + i11 * dX[i][1];
+        }
+        bool any = false;
+#pragma unroll
+        for (int i = 0; i < 3; i++) any = any || (nbr[i] != -1);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            C a = C(0), b = C(0);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                if (nbr[i] != -1) a += B0[i] * (Uo[i][k] - Uc[k]);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                if (nbr[i] != -1) b += B1[i] * (Uo[i][k] - Uc[k]);
+            gx[k] = any ? a : C(0);
+            gy[k] = any ? b : C(0);
+        }
+    }
+    if (ph.limiter == FVMR_LIM_BARTH || ph.limiter == FVMR_LIM_VENKATAKRISHNAN) {
+        C ndx[3], ndy[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int nd = m.cnode[j * nc + c];
+            ndx[j] = (C)m.nx_[nd] - xc;
+            ndy[j] = (C)m.ny_[nd] - yc;
+        }
+        C eps2 = C(0);
+        if (ph.limiter == FVMR_LIM_VENKATAKRISHNAN) {
+            const C Kh = (C)ph.venkat_K * sqrt((C)m.vol[c]);
+            eps2 = Kh * Kh * Kh;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            C lo = Uc[k], hi = Uc[k];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                if (nbr[i] != -1) { lo = tmin(lo, Uo[i][k]); hi = tmax(hi, Uo[i][k]); }
+            C phi = C(1.0);
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                C pn = C(1.0);
+                if (ph.limiter == FVMR_LIM_BARTH) {
+                    const C Un = Uc[k] + gx[k] * ndx[j] + gy[k] * ndy[j];
+                    C rel = Un - Uc[k];
+                    const C hrel = hi - Uc[k], lrel = lo - Uc[k];
+                    if (rel > C(0)) { rel += C(fvmr::kEPS); pn = tmin(C(1), hrel / rel); }
+                    else if (rel < C(0)) { rel -= C(fvmr::kEPS); pn = tmin(C(1), lrel / rel); }
+                } else {
+                    const C d2 = gx[k] * ndx[j] + gy[k] * ndy[j];
+                    if (d2 != C(0)) {
+                        const C d1 = (d2 > C(0)) ? (hi - Uc[k]) : (lo - Uc[k]);
+                        const C num = (d1 * d1 + eps2) * d2 + C(2.0) * d2 * d2 * d1;
+                        const C den = d1 * d1 + C(2.0) * d2 * d2 + d1 * d2 + eps2;
+                        pn = num / (den * d2);
+                    }
+                }
+                phi = tmin(phi, pn);
+            }
+            gx[k] *= phi;
+            gy[k] *= phi;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        Ux[k * nc + c] = (G)gx[k];
+        Uy[k * nc + c] = (G)gy[k];
+    }
+}
+
+// One RK stage: rhs = -(1/V) sum_j s_j F_{e_j} (add_edge_flux_to_element_flux_kernel, FluxGPU.cu:285-323,
+// vector_weighted_divide_kernel, BasicAlgorithmGPU.cu:4-9), then
+//   stage 0:    Uout = Uin + dt rhs                      (vector_weighted_add_kernel)
+//   stage 1, 2: Uout = a Un + b (Uin + dt rhs)           (SSP-RK3)
+// Cells whose new state is NaN or out of range are appended to the restriction list.
+template <class C, class G>
+__global__ __launch_bounds__(kBlock) void k_stage(Mesh<G> m, Phys ph, const double *__restrict__ Uin,
+                                                  const double *__restrict__ Un, double *__restrict__ Uout,
+                                                  const G *__restrict__ Ux, const G *__restrict__ Uy,
+                                                  double *__restrict__ rhs_out, Scalars *__restrict__ sc,
+                                                  int *__restrict__ bad_list, double a, double b, int first) {
+    const int c = xcd_block() * kBlock + threadIdx.x;
+    const int nc = m.nc;
+    if (c >= nc) return;
+    const double dt = sc->dt;
+    const double t_rk = sc->t_rk;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int e = m.cface[i * nc + c];
+        C F[4];
+        edge_flux<C, G>(m, ph, Uin, Ux, Uy, t_rk, e, F);
+        const double s = (c != m.eLR[e].y) ? 1.0 : -1.0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc[k] += s * (double)F[k];
+    }
+    const double w = -1.0 * (double)m.vol[c];
+    double Unew[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double r = acc[k] / w;
+        if (rhs_out) rhs_out[k * nc + c] = r;
+        const double st = Uin[k * nc + c] + dt * r;
+        Unew[k] = first ? st : a * Un[k * nc + c] + b * st;
+        Uout[k * nc + c] = Unew[k];
+    }
+    double rv[4];
+    u2ruvp(Unew, rv, ph.gamma);
+    bool bad = out_of_range(rv);
+#pragma unroll
+    for (int k = 0; k < 4; k++) bad = bad || isnan(rv[k]);
+    if (bad) {
+        const int slot = atomicAdd(&sc->bad_count, 1);
+        if (slot < nc) bad_list[slot] = c;
+    }
+}
+
+// modifyElementFieldU2d_device_kernel (RestrictGPU.cu:6-84) with Jacobi reads: one work-group,
+// phase 1 computes all repairs from the unrepaired field, phase 2 writes them.
+__global__ __launch_bounds__(1024) void k_restrict(int nc, const int *__restrict__ cface, const int2 *__restrict__ eLR,
+                                                   const int *__restrict__ epair, const double *__restrict__ vol_d,
+                                                   const float *__restrict__ vol_f, double *__restrict__ U,
+                                                   Scalars *__restrict__ sc, const int *__restrict__ bad_list,
+                                                   double *__restrict__ fix) {
+    const int count = min(sc->bad_count, nc);
+    for (int i = threadIdx.x; i < count; i += blockDim.x) {
+        const int c = bad_list[i];
+        int nvalid = 0;
+        double Usum[4] = {0, 0, 0, 0}, vs = 0.0;
+        for (int j = 0; j < 3; j++) {
+            const int e = cface[j * nc + c];
+            const int2 lr = eLR[e];
+            int nb = (c != lr.y) ? lr.y : lr.x;
+            if (epair[e] >= 0) nb = -1;
+            if (nb == -1) continue;
+            nvalid++;
+            const double v = vol_d ? vol_d[nb] : (double)vol_f[nb];
+            for (int k = 0; k < 4; k++) {
+                Usum[k] += U[k * nc + nb] * v;
+                vs += v;
+            }
+        }
+        for (int k = 0; k < 4; k++) fix[4 * i + k] = (nvalid == 0) ? U[k * nc + c] : Usum[k] / vs;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < count; i += blockDim.x) {
+        const int c = bad_list[i];
+        for (int k = 0; k < 4; k++) U[k * nc + c] = fix[4 * i + k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        sc->n_restricted += count;
+        sc->bad_count = 0;
+    }
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// get_global_dt_device (CalculateDtGPU.cu:97-103) with update_alphaC in the sequential
+// order of CalculateDt.cpp:10-64, then the clamp and t update of CIteration.cpp:134-139.
+template <class G>
+__global__ __launch_bounds__(kBlock) void k_dt(Mesh<G> m, Phys ph, const double *__restrict__ U,
+                                               double *__restrict__ partial, Scalars *__restrict__ sc, int advance) {
+    const int nc = m.nc;
+    const int blk = xcd_block();
+    const int c = blk * kBlock + threadIdx.x;
+    double dtc = 1e10;
+    if (c < nc) {
+        const double gamma = ph.gamma;
+        const double value_1 = fmax(4.0 / 3.0, gamma / ph.Pr);
+        double alpha_c = 0.0;
+        for (int k = 0; k < kMaxDtSlots; k++) {
+            const int e = m.dtedge[k * nc + c];
+            if (e < 0) break;
+            const int2 lr = m.eLR[e];
+            double rL[4], UL[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) UL[q] = U[q * nc + lr.x];
+            u2ruvp(UL, rL, gamma);
+            double rho, u, v, p;
+            if (lr.y != -1) {
+                double rR[4], UR[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) UR[q] = U[q * nc + lr.y];
+                u2ruvp(UR, rR, gamma);
+                rho = 0.5 * (rL[0] + rR[0]);
+                u = 0.5 * (rL[1] + rR[1]);
+                v = 0.5 * (rL[2] + rR[2]);
+                p = 0.5 * (rL[3] + rR[3]);
+            } else {
+                rho = rL[0]; u = rL[1]; v = rL[2]; p = rL[3];
+            }
+            const double len = (double)m.elen[e];
+            const double dx = (double)m.enx[e] * len, dy = (double)m.eny[e] * len;
+            const double un = u * dx + v * dy;
+            alpha_c += fabs(un) + sqrt(gamma * p / rho) * sqrt(len);
+            if (ph.equation == FVMR_EQ_NS) {
+                const double T = p / ph.R / rho;
+                const double mu = mu_of_T(ph, T);
+                const double aV = 2.0 * len * value_1 * mu / rho;
+                alpha_c += aV / (double)m.vol[c];
+            }
+        }
+        dtc = ph.CFL * (double)m.vol[c] / alpha_c;
+    }
+    __shared__ double smin[kBlock / 64];
+    __shared__ bool last;
+    double v = wave_min(dtc);
+    if ((threadIdx.x & 63) == 0) smin[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = smin[0];
+        for (int i = 1; i < kBlock / 64; i++) b = fmin(b, smin[i]);
+        partial[blk] = b;
+        __threadfence();
+        const unsigned int done = atomicAdd(&sc->counter, 1u);
+        last = (done == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double b = 1e10;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) b = fmin(b, partial[i]);
+    b = wave_min(b);
+    if ((threadIdx.x & 63) == 0) smin[threadIdx.x >> 6] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double d = smin[0];
+        for (int i = 1; i < kBlock / 64; i++) d = fmin(d, smin[i]);
+        sc->counter = 0u;
+        if (advance) {
+            const double t = sc->t;
+            if (t + d > sc->T) d = sc->T - t;
+            sc->t_rk = t;
+            sc->t = t + d;
+            sc->n_restricted = 0;
+        }
+        sc->dt = d;
+    }
+}
+
+// get_residual_host (Residual.cpp:4-43) on the device: 1 = L1, 2 = L2, 3 = Linf, per variable.
+__global__ __launch_bounds__(kBlock) void k_norm(int nc, const double *__restrict__ rhs, int norm,
+                                                 double *__restrict__ partial, double *__restrict__ out,
+                                                 Scalars *__restrict__ sc) {
+    const int blk = blockIdx.x;
+    const int c = blk * kBlock + threadIdx.x;
+    __shared__ double s[4][kBlock / 64];
+    __shared__ bool last;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double f = (c < nc) ? fabs(rhs[k * nc + c]) : 0.0;
+        double v = (norm == FVMR_NORM_INF) ? wave_max(f) : wave_sum(norm == FVMR_NORM_2 ? f * f : f);
+        if ((threadIdx.x & 63) == 0) s[k][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 4; k++) {
+            double b = s[k][0];
+            for (int i = 1; i < kBlock / 64; i++) b = (norm == FVMR_NORM_INF) ? fmax(b, s[k][i]) : b + s[k][i];
+            partial[4 * blk + k] = b;
+        }
+        __threadfence();
+        last = (atomicAdd(&sc->norm_counter, 1u) == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x < 4) {
+        const int k = threadIdx.x;
+        double b = 0.0;
+        for (int i = 0; i < (int)gridDim.x; i++) {
+            const double v = partial[4 * i + k];
+            b = (norm == FVMR_NORM_INF) ? fmax(b, v) : b + v;
+        }
+        out[k] = (norm == FVMR_NORM_2) ? sqrt(b) : b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) sc->norm_counter = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+template <class T>
+int upload(T **dst, const std::vector<T> &src, hipStream_t s) {
+    HIP_OK(hipMalloc((void **)dst, std::max<size_t>(1, src.size()) * sizeof(T)));
+    if (!src.empty()) HIP_OK(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return FVMR_OK;
+}
+
+// Reverse Cuthill-McKee over the cell adjacency (faces, periodic faces included),
+// starting each component from a pseudo-peripheral cell; ties by lower index.
+std::vector<int> rcm_order(int nc, const std::vector<int> &adj_off, const std::vector<int> &adj) {
+    std::vector<int> order;
+    order.reserve(nc);
+    std::vector<char> seen(nc, 0);
+    auto deg = [&](int c) { return adj_off[c + 1] - adj_off[c]; };
+    std::vector<int> level(nc, -1);
+    auto bfs_last = [&](int s, int &ecc) {
+        std::vector<int> q{s};
+        std::vector<int> touched{s};
+        level[s] = 0;
+        size_t h = 0;
+        int last = s;
+        while (h < q.size()) {
+            int c = q[h++];
+            last = c;
+            for (int k = adj_off[c]; k < adj_off[c + 1]; k++) {
+                int d = adj[k];
+                if (level[d] < 0) { level[d] = level[c] + 1; q.push_back(d); touched.push_back(d); }
+            }
+        }
+        // among the last level pick the minimum degree (then lowest index)
+        ecc = level[last];
+        int best = last;
+        for (int c : q) if (level[c] == ecc && (deg(c) < deg(best) || (deg(c) == deg(best) && c < best))) best = c;
+        for (int c : touched) level[c] = -1;
+        return best;
+    };
+    for (int s0 = 0; s0 < nc; s0++) {
+        if (seen[s0]) continue;
+        int start = s0, ecc = 0;
+        for (int it = 0; it < 4; it++) {
+            int e2;
+            int far = bfs_last(start, e2);
+            if (it > 0 && e2 <= ecc) break;
+            ecc = e2;
+            start = far;
+        }
+        size_t h = order.size();
+        order.push_back(start);
+        seen[start] = 1;
+        std::vector<int> nb;
+        while (h < order.size()) {
+            int c = order[h++];
+            nb.clear();
+            for (int k = adj_off[c]; k < adj_off[c + 1]; k++) if (!seen[adj[k]]) nb.push_back(adj[k]);
+            std::sort(nb.begin(), nb.end(), [&](int a, int b) { return deg(a) != deg(b) ? deg(a) < deg(b) : a < b; });
+            for (int d : nb) { seen[d] = 1; order.push_back(d); }
+        }
+    }
+    std::reverse(order.begin(), order.end());
+    return order;
+}
+
+struct Prof {
+    std::vector<hipEvent_t> ev;  // pairs
+    double total_ms = 0;
+    int count = 0;
+};
+
+}  // namespace
+
+struct fvmr_solver {
+    int nc = 0, nf = 0, nv = 0, nb = 0;
+    fvmr_params p{};
+    Phys ph{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    bool mixed = false;
+    std::vector<int> perm, iperm;   // perm[new] = reference cell; iperm[reference] = new
+    // device storage
+    double *Ubuf[3] = {nullptr, nullptr, nullptr};
+    int cur = 0;                    // index of the buffer holding the current state
+    void *Ux = nullptr, *Uy = nullptr;
+    double *rhs = nullptr;
+    void *geo = nullptr;            // all floating geometry of type G in one block
+    int *cface = nullptr, *cnode = nullptr, *epair = nullptr, *etype = nullptr, *dtedge = nullptr;
+    int2 *eLR = nullptr;
+    double *partial = nullptr, *norm_out = nullptr, *fix = nullptr, *vol_d = nullptr;
+    int *bad_list = nullptr;
+    Scalars *sc = nullptr;
+    // geometry pointers into geo
+    void *g_cx, *g_cy, *g_vol, *g_nx, *g_ny, *g_enx, *g_eny, *g_elen, *g_emx, *g_emy;
+    bool rhs_valid = false;
+    bool grad_valid = false;
+    bool profiling = false;
+    Prof prof[kNumProf];
+    std::vector<hipEvent_t> pending[kNumProf];
+};
+
+namespace {
+
+template <class G>
+Mesh<G> mesh_args(const fvmr_solver *h) {
+    Mesh<G> m;
+    m.nc = h->nc; m.nf = h->nf;
+    m.cx = (const G *)h->g_cx; m.cy = (const G *)h->g_cy; m.vol = (const G *)h->g_vol;
+    m.cface = h->cface; m.cnode = h->cnode;
+    m.nx_ = (const G *)h->g_nx; m.ny_ = (const G *)h->g_ny;
+    m.eLR = h->eLR;
+    m.enx = (const G *)h->g_enx; m.eny = (const G *)h->g_eny; m.elen = (const G *)h->g_elen;
+    m.emx = (const G *)h->g_emx; m.emy = (const G *)h->g_emy;
+    m.epair = h->epair; m.etype = h->etype; m.dtedge = h->dtedge;
+    return m;
+}
+
+int prof_begin(fvmr_solver *h, int k, hipEvent_t *out) {
+    if (!h->profiling) return FVMR_OK;
+    HIP_OK(hipEventCreate(out));
+    HIP_OK(hipEventRecord(*out, h->stream));
+    return FVMR_OK;
+}
+int prof_end(fvmr_solver *h, int k, hipEvent_t start) {
+    if (!h->profiling) return FVMR_OK;
+    hipEvent_t e;
+    HIP_OK(hipEventCreate(&e));
+    HIP_OK(hipEventRecord(e, h->stream));
+    h->pending[k].push_back(start);
+    h->pending[k].push_back(e);
+    return FVMR_OK;
+}
+int prof_collect(fvmr_solver *h) {
+    for (int k = 0; k < kNumProf; k++) {
+        auto &v = h->pending[k];
+        for (size_t i = 0; i + 1 < v.size(); i += 2) {
+            HIP_OK(hipEventSynchronize(v[i + 1]));
+            float ms = 0;
+            HIP_OK(hipEventElapsedTime(&ms, v[i], v[i + 1]));
+            h->prof[k].total_ms += ms;
+            h->prof[k].count++;
+            HIP_OK(hipEventDestroy(v[i]));
+            HIP_OK(hipEventDestroy(v[i + 1]));
+        }
+        v.clear();
+    }
+    return FVMR_OK;
+}
+
+int grid_of(int n) { return (n + kBlock - 1) / kBlock; }
+
+template <class C, class G>
+int launch_gradient(fvmr_solver *h, const double *U) {
+    hipEvent_t e0{};
+    int rc = prof_begin(h, 1, &e0);
+    if (rc) return rc;
+    k_gradient<C, G><<<grid_of(h->nc), kBlock, 0, h->stream>>>(mesh_args<G>(h), h->ph, U, (G *)h->Ux, (G *)h->Uy);
+    HIP_OK(hipGetLastError());
+    return prof_end(h, 1, e0);
+}
+
+template <class C, class G>
+int launch_stage(fvmr_solver *h, const double *Uin, const double *Un, double *Uout, double a, double b, int first,
+                 double *rhs_out) {
+    hipEvent_t e0{};
+    int rc = prof_begin(h, 0, &e0);
+    if (rc) return rc;
+    k_stage<C, G><<<grid_of(h->nc), kBlock, 0, h->stream>>>(mesh_args<G>(h), h->ph, Uin, Un, Uout, (const G *)h->Ux,
+                                                            (const G *)h->Uy, rhs_out, h->sc, h->bad_list, a, b, first);
+    HIP_OK(hipGetLastError());
+    if ((rc = prof_end(h, 0, e0))) return rc;
+    if ((rc = prof_begin(h, 3, &e0))) return rc;
+    k_restrict<<<1, 1024, 0, h->stream>>>(h->nc, h->cface, h->eLR, h->epair, h->mixed ? nullptr : (const double *)h->g_vol,
+                                          h->mixed ? (const float *)h->g_vol : nullptr, Uout, h->sc, h->bad_list, h->fix);
+    HIP_OK(hipGetLastError());
+    return prof_end(h, 3, e0);
+}
+
+template <class G>
+int launch_dt(fvmr_solver *h, const double *U, int advance) {
+    hipEvent_t e0{};
+    int rc = prof_begin(h, 2, &e0);
+    if (rc) return rc;
+    k_dt<G><<<grid_of(h->nc), kBlock, 0, h->stream>>>(mesh_args<G>(h), h->ph, U, h->partial, h->sc, advance);
+    HIP_OK(hipGetLastError());
+    return prof_end(h, 2, e0);
+}
+
+// One step on the device (no host synchronisation).  store_rhs: keep stage-1 rhs.
+template <class C, class G>
+int device_step(fvmr_solver *h, bool store_rhs) {
+    int rc;
+    const int a = h->cur, b = (a + 1) % 3, c = (a + 2) % 3;
+    double *Ua = h->Ubuf[a], *Ub = h->Ubuf[b], *Uc = h->Ubuf[c];
+    if ((rc = launch_dt<G>(h, Ua, 1))) return rc;
+    if (h->p.time_advance == FVMR_EVO_EXPLICIT) {
+        if ((rc = launch_gradient<C, G>(h, Ua))) return rc;
+        if ((rc = launch_stage<C, G>(h, Ua, Ua, Ub, 0.0, 1.0, 1, store_rhs ? h->rhs : nullptr))) return rc;
+        h->cur = b;
+    } else {
+        // SSP-RK3 (Shu-Osher): U1 = U + dt R(U); U2 = 3/4 U + 1/4 (U1 + dt R(U1));
+        // U^{n+1} = 1/3 U + 2/3 (U2 + dt R(U2))
+        if ((rc = launch_gradient<C, G>(h, Ua))) return rc;
+        if ((rc = launch_stage<C, G>(h, Ua, Ua, Ub, 0.0, 1.0, 1, store_rhs ? h->rhs : nullptr))) return rc;
+        if ((rc = launch_gradient<C, G>(h, Ub))) return rc;
+        if ((rc = launch_stage<C, G>(h, Ub, Ua, Uc, 0.75, 0.25, 0, nullptr))) return rc;
+        if ((rc = launch_gradient<C, G>(h, Uc))) return rc;
+        if ((rc = launch_stage<C, G>(h, Uc, Ua, Ub, 1.0 / 3.0, 2.0 / 3.0, 0, nullptr))) return rc;
+        h->cur = b;
+    }
+    h->grad_valid = false;
+    if (store_rhs) h->rhs_valid = true;
+    return FVMR_OK;
+}
+
+int run_step(fvmr_solver *h, bool store_rhs) {
+    if (h->mixed) return device_step<float, float>(h, store_rhs);
+    return device_step<double, double>(h, store_rhs);
+}
+
+template <class G>
+int build_geometry(fvmr_solver *h, const fvmr_mesh_soa *m, const std::vector<int> &eperm) {
+    // eperm[new edge] = reference edge
+    const int nc = h->nc, nf = h->nf, nv = h->nv;
+    const size_t total = (size_t)3 * nc + 2 * nv + 5 * (size_t)nf;
+    std::vector<G> g(total);
+    G *cx = g.data(), *cy = cx + nc, *vol = cy + nc, *nx = vol + nc, *ny = nx + nv;
+    G *enx = ny + nv, *eny = enx + nf, *elen = eny + nf, *emx = elen + nf, *emy = emx + nf;
+    for (int i = 0; i < nc; i++) {
+        const int r = h->perm[i];
+        cx[i] = (G)m->element_xy[0][r]; cy[i] = (G)m->element_xy[1][r]; vol[i] = (G)m->element_volume[r];
+    }
+    for (int v = 0; v < nv; v++) { nx[v] = (G)m->node_xy[0][v]; ny[v] = (G)m->node_xy[1][v]; }
+    for (int e = 0; e < nf; e++) {
+        const int r = eperm[e];
+        enx[e] = (G)m->edge_normal[0][r]; eny[e] = (G)m->edge_normal[1][r]; elen[e] = (G)m->edge_length[r];
+        emx[e] = (G)m->edge_xy[0][r]; emy[e] = (G)m->edge_xy[1][r];
+    }
+    G *d = nullptr;
+    HIP_OK(hipMalloc((void **)&d, total * sizeof(G)));
+    HIP_OK(hipMemcpyAsync(d, g.data(), total * sizeof(G), hipMemcpyHostToDevice, h->stream));
+    h->geo = d;
+    h->g_cx = d; h->g_cy = d + nc; h->g_vol = d + 2 * (size_t)nc; h->g_nx = d + 3 * (size_t)nc; h->g_ny = d + 3 * (size_t)nc + nv;
+    G *eb = d + 3 * (size_t)nc + 2 * (size_t)nv;
+    h->g_enx = eb; h->g_eny = eb + nf; h->g_elen = eb + 2 * (size_t)nf; h->g_emx = eb + 3 * (size_t)nf; h->g_emy = eb + 4 * (size_t)nf;
+    return FVMR_OK;
+}
+
+void free_all(fvmr_solver *h) {
+    for (auto &b : h->Ubuf) if (b) (void)hipFree(b);
+    void *ptrs[] = {h->Ux, h->Uy, h->rhs, h->geo, h->cface, h->cnode, h->epair, h->etype, h->dtedge, h->eLR,
+                    h->partial, h->norm_out, h->fix, h->bad_list, h->sc, h->vol_d};
+    for (void *q : ptrs) if (q) (void)hipFree(q);
+    for (int k = 0; k < kNumProf; k++)
+        for (auto e : h->pending[k]) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+int validate_params(const fvmr_params *p) {
+    if (p->equation != FVMR_EQ_EULER && p->equation != FVMR_EQ_NS) return set_error(FVMR_ERR_INVALID_ARG, "invalid equation");
+    if (p->flux_scheme != FVMR_SOL_LLF && p->flux_scheme != FVMR_SOL_ROE) return set_error(FVMR_ERR_INVALID_ARG, "invalid scheme @get_flux_RiemannSolve");
+    if (p->reconstruct != FVMR_REC_CONSTANT && p->reconstruct != FVMR_REC_LINEAR && p->reconstruct != FVMR_REC_MUSCL)
+        return set_error(FVMR_ERR_INVALID_ARG, "invalid flag_reconstruct");
+    if (p->gradient != FVMR_GRA_GREENGAUSS && p->gradient != FVMR_GRA_LEASTSQUARE) return set_error(FVMR_ERR_INVALID_ARG, "invalid graident type");
+    if (p->limiter != FVMR_LIM_NONE && p->limiter != FVMR_LIM_BARTH && p->limiter != FVMR_LIM_VENKATAKRISHNAN)
+        return set_error(FVMR_ERR_INVALID_ARG, "invalid limiter type");
+    if (p->time_advance != FVMR_EVO_EXPLICIT && p->time_advance != FVMR_EVO_RK3)
+        return set_error(FVMR_ERR_UNSUPPORTED, "invalid temporal_choice");
+    if (p->precision != FVMR_PREC_FP64 && p->precision != FVMR_PREC_MIXED) return set_error(FVMR_ERR_INVALID_ARG, "invalid precision");
+    if (!(p->gamma > 1.0) || !(p->CFL > 0.0)) return set_error(FVMR_ERR_INVALID_ARG, "invalid gamma or CFL");
+    return FVMR_OK;
+}
+
+int validate_mesh(const fvmr_mesh_soa *m) {
+    if (!m || m->num_element <= 0 || m->num_edge <= 0 || m->num_node <= 0) return set_error(FVMR_ERR_INVALID_ARG, "empty mesh");
+    const int nc = m->num_element, nf = m->num_edge, nv = m->num_node;
+    for (int c = 0; c < nc; c++) {
+        for (int j = 0; j < 3; j++) {
+            const int e = m->element_edges[j][c], v = m->element_nodes[j][c];
+            if (e < 0 || e >= nf) return set_error(FVMR_ERR_MESH, "element edge index out of range (triangles only)");
+            if (v < 0 || v >= nv) return set_error(FVMR_ERR_MESH, "element node index out of range");
+            const int L = m->edge_elementL[e], R = m->edge_elementR[e];
+            if (L != c && R != c) return set_error(FVMR_ERR_MESH, "element edge does not own the element");
+            const int out = (c != R) ? R : L;
+            const int expect = (m->edge_periodicPair[e] >= 0) ? -1 : out;
+            if (m->element_neighbors[j][c] != expect)
+                return set_error(FVMR_ERR_MESH, "element neighbours disagree with the edge owners");
+        }
+    }
+    for (int e = 0; e < nf; e++) {
+        const int L = m->edge_elementL[e], R = m->edge_elementR[e], s = m->edge_setID[e], pe = m->edge_periodicPair[e];
+        if (L < 0 || L >= nc || R < -1 || R >= nc) return set_error(FVMR_ERR_MESH, "edge owner out of range");
+        if (s != -1 && (s < 1 || s > m->num_boundary_set)) return set_error(FVMR_ERR_MESH, "edge setID out of range");
+        if (pe < -1 || pe >= nf) return set_error(FVMR_ERR_MESH, "periodic pair out of range");
+        if (R == -1 && s == -1) return set_error(FVMR_ERR_MESH, "boundary edge without a boundary set");
+    }
+    return FVMR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out) {
+    if (!m || !p || !out) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    int rc = validate_params(p);
+    if (rc) return rc;
+    if ((rc = validate_mesh(m))) return rc;
+    int ndev = 0;
+    HIP_OK(hipGetDeviceCount(&ndev));
+    if (ndev <= 0) return set_error(FVMR_ERR_HIP, "no HIP device");
+    fvmr_solver *h = new fvmr_solver();
+    h->nc = m->num_element; h->nf = m->num_edge; h->nv = m->num_node; h->nb = m->num_boundary_set;
+    h->p = *p;
+    h->mixed = (p->precision == FVMR_PREC_MIXED);
+    (void)hipGetDevice(&h->device);
+    auto bail = [&](int code) { free_all(h); delete h; return code; };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return bail(set_error(FVMR_ERR_HIP, "hipStreamCreate failed"));
+    const int nc = h->nc, nf = h->nf;
+    // physics constants
+    Phys &ph = h->ph;
+    ph.gamma = p->gamma; ph.R = p->R; ph.Pr = p->Pr; ph.Pr_visc = p->Pr_viscous; ph.mu0 = p->mu0;
+    ph.sutherland_C1 = p->mu0 / std::pow(p->T0, 1.5) * (p->T0 + 110.4);  // ViscousFluxGPU.h:26-28
+    ph.CFL = p->CFL; ph.wall_T = p->wall_temperature; ph.venkat_K = p->venkat_K;
+    ph.dsr_x = p->dsr_shock_x; ph.dsr_y = p->dsr_shock_y; ph.dsr_speed = p->dsr_shock_speed;
+    for (int k = 0; k < 4; k++) { ph.inf[k] = p->inf_ruvp[k]; ph.inlet[k] = p->inlet_ruvp[k]; ph.outlet[k] = p->outlet_ruvp[k]; }
+    ph.equation = p->equation; ph.scheme = p->flux_scheme;
+    ph.reconstruct = (p->reconstruct == FVMR_REC_LINEAR) ? FVMR_REC_LINEAR : FVMR_REC_CONSTANT;
+    ph.gradient = p->gradient; ph.limiter = p->limiter; ph.mu_model = p->viscosity_model; ph.compat_inv = p->compat_inv2x2;
+    // cell permutation
+    h->perm.resize(nc);
+    if (p->renumber) {
+        std::vector<int> off(nc + 1, 0), adj;
+        adj.reserve(3 * (size_t)nc);
+        for (int c = 0; c < nc; c++) {
+            for (int j = 0; j < 3; j++) {
+                const int e = m->element_edges[j][c];
+                const int L = m->edge_elementL[e], R = m->edge_elementR[e];
+                const int o = (c != R) ? R : L;
+                if (o >= 0 && o != c) adj.push_back(o);
+            }
+            off[c + 1] = (int)adj.size();
+        }
+        h->perm = rcm_order(nc, off, adj);
+    } else {
+        std::iota(h->perm.begin(), h->perm.end(), 0);
+    }
+    h->iperm.assign(nc, -1);
+    for (int i = 0; i < nc; i++) h->iperm[h->perm[i]] = i;
+    // edge order: first appearance over the new cell order
+    std::vector<int> enew(nf, -1), eperm;
+    eperm.reserve(nf);
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < 3; j++) {
+            const int e = m->element_edges[j][h->perm[i]];
+            if (enew[e] < 0) { enew[e] = (int)eperm.size(); eperm.push_back(e); }
+        }
+    for (int e = 0; e < nf; e++) if (enew[e] < 0) { enew[e] = (int)eperm.size(); eperm.push_back(e); }
+    // connectivity in the new numbering
+    std::vector<int> cface(3 * (size_t)nc), cnode(3 * (size_t)nc), epair(nf), etype(nf), dtedge((size_t)kMaxDtSlots * nc, -1);
+    std::vector<int2> eLR(nf);
+    for (int i = 0; i < nc; i++) {
+        const int r = h->perm[i];
+        for (int j = 0; j < 3; j++) {
+            cface[(size_t)j * nc + i] = enew[m->element_edges[j][r]];
+            cnode[(size_t)j * nc + i] = m->element_nodes[j][r];
+        }
+    }
+    for (int e = 0; e < nf; e++) {
+        const int r = eperm[e];
+        const int L = m->edge_elementL[r], R = m->edge_elementR[r];
+        eLR[e] = make_int2(h->iperm[L], R >= 0 ? h->iperm[R] : -1);
+        epair[e] = m->edge_periodicPair[r] >= 0 ? enew[m->edge_periodicPair[r]] : -1;
+        const int s = m->edge_setID[r];
+        etype[e] = (s == -1) ? -1 : m->boundary_type[s - 1];
+    }
+    // alphaC contributions in the reference edge order (CalculateDt.cpp:22-63)
+    {
+        std::vector<int> cnt(nc, 0);
+        for (int r = 0; r < nf; r++) {
+            const int L = m->edge_elementL[r], R = m->edge_elementR[r];
+            const int cells[2] = {L, R};
+            for (int q = 0; q < 2; q++) {
+                if (cells[q] < 0) continue;
+                const int ci = h->iperm[cells[q]];
+                if (cnt[ci] >= kMaxDtSlots) return bail(set_error(FVMR_ERR_MESH, "too many edges touch one cell"));
+                dtedge[(size_t)cnt[ci]++ * nc + ci] = enew[r];
+            }
+        }
+    }
+    // device memory
+    const size_t n4 = 4 * (size_t)nc;
+    const size_t gsz = h->mixed ? sizeof(float) : sizeof(double);
+    for (int k = 0; k < 3; k++) if (hipMalloc((void **)&h->Ubuf[k], n4 * sizeof(double)) != hipSuccess) return bail(set_error(FVMR_ERR_HIP, "hipMalloc U"));
+    if (hipMalloc(&h->Ux, n4 * gsz) != hipSuccess || hipMalloc(&h->Uy, n4 * gsz) != hipSuccess ||
+        hipMalloc((void **)&h->rhs, n4 * sizeof(double)) != hipSuccess || hipMalloc((void **)&h->fix, n4 * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&h->bad_list, (size_t)nc * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&h->partial, (size_t)4 * grid_of(nc) * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&h->norm_out, 4 * sizeof(double)) != hipSuccess || hipMalloc((void **)&h->sc, sizeof(Scalars)) != hipSuccess)
+        return bail(set_error(FVMR_ERR_HIP, "hipMalloc failed"));
+    if ((rc = upload(&h->cface, cface, h->stream)) || (rc = upload(&h->cnode, cnode, h->stream)) ||
+        (rc = upload(&h->epair, epair, h->stream)) || (rc = upload(&h->etype, etype, h->stream)) ||
+        (rc = upload(&h->dtedge, dtedge, h->stream)) || (rc = upload(&h->eLR, eLR, h->stream)))
+        return bail(rc);
+    rc = h->mixed ? build_geometry<float>(h, m, eperm) : build_geometry<double>(h, m, eperm);
+    if (rc) return bail(rc);
+    Scalars s0{};
+    if (hipMemsetAsync(h->Ubuf[0], 0, n4 * sizeof(double), h->stream) != hipSuccess ||
+        hipMemcpyAsync(h->sc, &s0, sizeof(Scalars), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return bail(set_error(FVMR_ERR_HIP, "initial copy failed"));
+    *out = h;
+    return FVMR_OK;
+}
+
+int fvmr_destroy(fvmr_handle *h) {
+    if (!h) return FVMR_OK;
+    (void)hipStreamSynchronize(h->stream);
+    free_all(h);
+    delete h;
+    return FVMR_OK;
+}
+
+int fvmr_set_state(fvmr_handle *h, const double *const U[4]) {
+    if (!h || !U) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
+    const int nc = h->nc;
+    std::vector<double> buf(4 * (size_t)nc);
+    for (int k = 0; k < 4; k++) {
+        if (!U[k]) return set_error(FVMR_ERR_INVALID_ARG, "null state component");
+        for (int i = 0; i < nc; i++) buf[(size_t)k * nc + i] = U[k][h->perm[i]];
+    }
+    HIP_OK(hipMemcpyAsync(h->Ubuf[h->cur], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    h->rhs_valid = false;
+    h->grad_valid = false;
+    return FVMR_OK;
+}
+
+int fvmr_get_state(fvmr_handle *h, double *const U[4]) {
+    if (!h || !U) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
+    const int nc = h->nc;
+    std::vector<double> buf(4 * (size_t)nc);
+    HIP_OK(hipMemcpyAsync(buf.data(), h->Ubuf[h->cur], buf.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < 4; k++) {
+        if (!U[k]) return set_error(FVMR_ERR_INVALID_ARG, "null state component");
+        for (int i = 0; i < nc; i++) U[k][h->perm[i]] = buf[(size_t)k * nc + i];
+    }
+    return FVMR_OK;
+}
+
+static int read_back(fvmr_handle *h, const double *dev, double *const out[4]) {
+    const int nc = h->nc;
+    std::vector<double> buf(4 * (size_t)nc);
+    HIP_OK(hipMemcpyAsync(buf.data(), dev, buf.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < 4; k++)
+        for (int i = 0; i < nc; i++) out[k][h->perm[i]] = buf[(size_t)k * nc + i];
+    return FVMR_OK;
+}
+
+static int finish_info(fvmr_handle *h, fvmr_step_info *info, int steps, Scalars &s) {
+    HIP_OK(hipMemcpyAsync(&s, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    if (info) {
+        info->t = s.t;
+        info->dt = s.dt;
+        info->n_restricted = s.n_restricted;
+        info->steps_done = steps;
+        if (info->residual_norm) {
+            int rc = fvmr_residual(h, info->residual_norm, info->residual);
+            if (rc) return rc;
+        }
+    }
+    return FVMR_OK;
+}
+
+int fvmr_advance(fvmr_handle *h, int nsteps, double *t, double T, fvmr_step_info *info) {
+    if (!h || !t || nsteps < 0) return set_error(FVMR_ERR_INVALID_ARG, "invalid argument");
+    Scalars s{};
+    HIP_OK(hipMemcpyAsync(&s, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    s.t = *t;
+    s.T = T;
+    s.counter = 0; s.norm_counter = 0; s.bad_count = 0; s.n_restricted = 0;
+    HIP_OK(hipMemcpyAsync(h->sc, &s, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
+    for (int i = 0; i < nsteps; i++) {
+        int rc = run_step(h, i == nsteps - 1);
+        if (rc) return rc;
+    }
+    Scalars s2;
+    int rc = finish_info(h, info, nsteps, s2);
+    if (rc) return rc;
+    if (h->profiling && (rc = prof_collect(h))) return rc;
+    *t = s2.t;
+    return FVMR_OK;
+}
+
+int fvmr_step(fvmr_handle *h, double *t, double T, fvmr_step_info *info) { return fvmr_advance(h, 1, t, T, info); }
+
+int fvmr_stage_rhs(fvmr_handle *h, double *const rhs[4]) {
+    if (!h || !rhs) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
+    // one forward-Euler stage with dt = 0 into a scratch buffer: rhs of the current state
+    Scalars s{};
+    HIP_OK(hipMemcpyAsync(&s, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    Scalars z = s;
+    z.dt = 0.0;
+    z.t_rk = s.t;
+    HIP_OK(hipMemcpyAsync(h->sc, &z, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
+    const int a = h->cur, b = (a + 1) % 3;
+    int rc;
+    if (h->mixed) {
+        if ((rc = launch_gradient<float, float>(h, h->Ubuf[a])) || (rc = launch_stage<float, float>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs))) return rc;
+    } else {
+        if ((rc = launch_gradient<double, double>(h, h->Ubuf[a])) || (rc = launch_stage<double, double>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs))) return rc;
+    }
+    HIP_OK(hipMemcpyAsync(h->sc, &s, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
+    h->rhs_valid = true;
+    h->grad_valid = true;
+    return read_back(h, h->rhs, rhs);
+}
+
+int fvmr_gradient(fvmr_handle *h, double *const Ux[4], double *const Uy[4]) {
+    if (!h || !Ux || !Uy) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
+    int rc = h->mixed ? launch_gradient<float, float>(h, h->Ubuf[h->cur]) : launch_gradient<double, double>(h, h->Ubuf[h->cur]);
+    if (rc) return rc;
+    h->grad_valid = true;
+    if (!h->mixed) {
+        if ((rc = read_back(h, (const double *)h->Ux, Ux))) return rc;
+        return read_back(h, (const double *)h->Uy, Uy);
+    }
+    const int nc = h->nc;
+    std::vector<float> bx(4 * (size_t)nc), by(4 * (size_t)nc);
+    HIP_OK(hipMemcpyAsync(bx.data(), h->Ux, bx.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(by.data(), h->Uy, by.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < 4; k++)
+        for (int i = 0; i < nc; i++) {
+            Ux[k][h->perm[i]] = bx[(size_t)k * nc + i];
+            Uy[k][h->perm[i]] = by[(size_t)k * nc + i];
+        }
+    return FVMR_OK;
+}
+
+int fvmr_global_dt(fvmr_handle *h, double *dt) {
+    if (!h || !dt) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
+    int rc = h->mixed ? launch_dt<float>(h, h->Ubuf[h->cur], 0) : launch_dt<double>(h, h->Ubuf[h->cur], 0);
+    if (rc) return rc;
+    Scalars s;
+    HIP_OK(hipMemcpyAsync(&s, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    *dt = s.dt;
+    return FVMR_OK;
+}
+
+int fvmr_residual(fvmr_handle *h, int norm, double out[4]) {
+    if (!h || !out) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
+    if (norm != FVMR_NORM_1 && norm != FVMR_NORM_2 && norm != FVMR_NORM_INF) return set_error(FVMR_ERR_INVALID_ARG, "invalid norm type");
+    if (!h->rhs_valid) return set_error(FVMR_ERR_INVALID_ARG, "no residual available: run a step first");
+    hipEvent_t e0{};
+    int rc = prof_begin(h, 4, &e0);
+    if (rc) return rc;
+    k_norm<<<grid_of(h->nc), kBlock, 0, h->stream>>>(h->nc, h->rhs, norm, h->partial, h->norm_out, h->sc);
+    HIP_OK(hipGetLastError());
+    if ((rc = prof_end(h, 4, e0))) return rc;
+    HIP_OK(hipMemcpyAsync(out, h->norm_out, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return FVMR_OK;
+}
+
+int fvmr_permutation(fvmr_handle *h, int *perm) {
+    if (!h || !perm) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
+    std::copy(h->perm.begin(), h->perm.end(), perm);
+    return FVMR_OK;
+}
+
+int fvmr_profile_enable(fvmr_handle *h, int on) {
+    if (!h) return set_error(FVMR_ERR_INVALID_ARG, "null handle");
+    h->profiling = (on != 0);
+    return FVMR_OK;
+}
+
+int fvmr_profile_query(fvmr_handle *h, int kernel, double *avg_ms, int *count) {
+    if (!h || kernel < 0 || kernel >= kNumProf) return set_error(FVMR_ERR_INVALID_ARG, "invalid kernel class");
+    int rc = prof_collect(h);
+    if (rc) return rc;
+    if (avg_ms) *avg_ms = h->prof[kernel].count ? h->prof[kernel].total_ms / h->prof[kernel].count : 0.0;
+    if (count) *count = h->prof[kernel].count;
+    return FVMR_OK;
+}
+
+int fvmr_profile_reset(fvmr_handle *h) {
+    if (!h) return set_error(FVMR_ERR_INVALID_ARG, "null handle");
+    int rc = prof_collect(h);
+    if (rc) return rc;
+    for (int k = 0; k < kNumProf; k++) { h->prof[k].total_ms = 0; h->prof[k].count = 0; }
+    return FVMR_OK;
+}
+
+int fvmr_synchronize(fvmr_handle *h) {
+    if (!h) return set_error(FVMR_ERR_INVALID_ARG, "null handle");
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return FVMR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,258 @@
+/*
+ * fvmr.h -- C ABI of the MI355X (gfx950) explicit finite-volume
+ * residual-and-update path of FVMR (OliveTIen/CFD_Solver_2D_GPU).
+ *
+ * The ABI replaces the reference's GPU solver surface:
+ *   - the solver container  GPU::GPUSolver2      (FVMR/src/solvers/GPUSolver2.h:32-89)
+ *   - the per-step driver    CIteration           (FVMR/src/time/CIteration.h:8-9)
+ *   - the mesh preprocessing SU2MeshReader/FVM_2D  (FVMR/src/input/SU2MeshReader.cpp:10-251,
+ *                                                   FVMR/src/legacy/FVM_2D.cpp:25-198)
+ *   - the AoS->SoA converter OldDataConverter     (FVMR/src/solvers/OldDataConverter.cpp:3-195)
+ *   - the initial fields     FieldInitializer      (FVMR/src/input/FieldInitializer.cpp:22-177)
+ *   - the residual norms     get_residual_host     (FVMR/src/output/residual/Residual.cpp:4-43)
+ *
+ * Conventions are the reference's: 0-based SoA indices, -1 sentinels, 1-based
+ * edge setID (-1 = interior), element slot 3 = -1 (triangles), BC/scheme codes
+ * of FVMR/src/global/Constexpr.h.  Arrays are SoA (one pointer per component).
+ * Every function returns an int status (FVMR_OK == 0) and never calls exit();
+ * the message of the last failure on the calling thread is fvmr_last_error().
+ * Host buffers are caller-owned; device buffers are library-owned.  One solver
+ * handle owns one HIP stream; a handle is thread-compatible, not thread-safe.
+ */
+#ifndef FVMR_H
+#define FVMR_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define FVMR_OK               0
+#define FVMR_ERR_INVALID_ARG  1
+#define FVMR_ERR_HIP          2
+#define FVMR_ERR_MESH         3
+#define FVMR_ERR_UNSUPPORTED  4
+#define FVMR_ERR_IO           5
+#define FVMR_ERR_NUMERIC      6
+
+/* ---- codes of FVMR/src/global/Constexpr.h:13-57 ------------------------- */
+#define FVMR_EQ_EULER            1
+#define FVMR_EQ_NS               2
+#define FVMR_SOL_LLF          2001
+#define FVMR_SOL_ROE          2002
+#define FVMR_REC_CONSTANT     3001
+#define FVMR_REC_LINEAR       3002
+#define FVMR_REC_MUSCL        3003   /* reference falls back to constant */
+#define FVMR_GRA_LEASTSQUARE     1
+#define FVMR_GRA_GREENGAUSS      2
+#define FVMR_EVO_EXPLICIT     3101   /* forward Euler, the reference GPU path */
+#define FVMR_EVO_RK3          3103   /* SSP-RK3 (Shu-Osher) in this library   */
+#define FVMR_LIM_NONE            0
+#define FVMR_LIM_BARTH           1
+#define FVMR_LIM_VENKATAKRISHNAN 4   /* new: not in the reference */
+#define FVMR_BC_WALL_NONVISCOUS 6001
+#define FVMR_BC_WALL_ADIABAT    6002
+#define FVMR_BC_WALL_ISOTHERMAL 6003 /* declared by the reference, produced here */
+#define FVMR_BC_INLET           6011
+#define FVMR_BC_OUTLET          6012
+#define FVMR_BC_INF             6013
+#define FVMR_BC_DSR             6014
+#define FVMR_BC_SYMMETRY        6021
+#define FVMR_BC_PERIODIC_0      6100 /* 6100..6109 */
+
+/* precision of the device path */
+#define FVMR_PREC_FP64        0     /* plain: every array and operation FP64 */
+#define FVMR_PREC_MIXED       1     /* FP32 flux/reconstruction/limiter, FP64
+                                       state, residual accumulation, RK update
+                                       and dt reduction */
+/* viscosity law */
+#define FVMR_MU_SUTHERLAND    0     /* reference (always on there) */
+#define FVMR_MU_CONSTANT      1     /* new: mu = mu0 */
+
+/* residual norm types, FVMR/src/output/residual/Residual.h */
+#define FVMR_NORM_1           1
+#define FVMR_NORM_2           2
+#define FVMR_NORM_INF         3
+
+/* field initialiser types, FVMR/src/input/FieldInitializer.h:8-13 */
+#define FVMR_INIT_UNIFORM        1
+#define FVMR_INIT_ISENTROPIC_VORTEX 2
+#define FVMR_INIT_SHOCK_TUBE     3
+#define FVMR_INIT_DSR            4
+
+/* synthetic mesh families (fvmr_mesh_generate) */
+#define FVMR_GEN_PERIODIC_SQUARE 1  /* [x0,x0+L]^2, periodic_0 bottom/top, periodic_1 left/right */
+#define FVMR_GEN_WEDGE_CHANNEL   2  /* [0,Lx]x[0,Ly] channel with a ramp on the bottom wall */
+#define FVMR_GEN_CYLINDER_OGRID  3  /* O-grid around a cylinder, wall + farfield */
+#define FVMR_GEN_NACA_OGRID      4  /* O-grid around a NACA 4-digit airfoil */
+#define FVMR_GEN_RECT_CHANNEL    5  /* jittered channel, inlet/outlet/slip walls */
+
+/*
+ * Mesh in the reference SoA layout: GPU::NodeSoA (FVMR/src/gpu/datatype/NodeSoA.h:9-52),
+ * GPU::ElementSoA (ElementSoA.h:15-105), GPU::EdgeSoA (EdgeSoA.h:9-100) and
+ * GPU::BoundarySetMap (BoundarySetMap.h:10-70).  The IDs of all three are the
+ * array indices, so the ID arrays are omitted.
+ */
+typedef struct fvmr_mesh_soa {
+    int num_node;
+    int num_element;
+    int num_edge;
+    int num_boundary_set;
+    const double *node_xy[2];
+    const int    *element_nodes[4];      /* slot 3 = -1 */
+    const int    *element_edges[4];      /* slot 3 = -1 */
+    const int    *element_neighbors[4];  /* local-edge order; -1 = none (periodic too) */
+    const double *element_xy[2];         /* centroid */
+    const double *element_volume;        /* area */
+    const int    *edge_nodes[2];
+    const int    *edge_setID;            /* 1-based boundary set, -1 interior */
+    const int    *edge_periodicPair;     /* pair edge index or -1 */
+    const int    *edge_elementL;
+    const int    *edge_elementR;         /* -1 boundary; periodic -> pair's L */
+    const double *edge_length;
+    const double *edge_xy[2];            /* midpoint */
+    const double *edge_normal[2];        /* (dy,-dx)/length, L -> R */
+    const int    *boundary_type;         /* type of set s is boundary_type[s-1] */
+} fvmr_mesh_soa;
+
+/* Inputs the reference reads from GlobalPara (FVMR/src/global/GlobalPara.h:10-98). */
+typedef struct fvmr_params {
+    double gamma;            /* constant.gamma */
+    double R;                /* constant R (287.06), "rcpcv" in the kernels */
+    double Pr;               /* constant.Pr: used by the viscous dt term */
+    double Pr_viscous;       /* Prandtl number of the viscous flux (reference: 0.72) */
+    double T0;               /* constant.T0: Sutherland reference temperature */
+    double mu0;              /* constant.mu0 (or rho*U*L/Re) */
+    double CFL;              /* time.CFL */
+    int    equation;         /* FVMR_EQ_* */
+    int    flux_scheme;      /* FVMR_SOL_* */
+    int    reconstruct;      /* FVMR_REC_* */
+    int    gradient;         /* FVMR_GRA_* */
+    int    limiter;          /* FVMR_LIM_* */
+    int    time_advance;     /* FVMR_EVO_* */
+    int    viscosity_model;  /* FVMR_MU_* */
+    int    precision;        /* FVMR_PREC_* */
+    int    renumber;         /* 1: reverse Cuthill-McKee cell order on the device */
+    int    compat_inv2x2;    /* 1: reproduce the reference inv_2x2 determinant bug (LSQ) */
+    double inf_ruvp[4];      /* boundaryCondition.2D.inf    (rho,u,v,p) */
+    double inlet_ruvp[4];    /* boundaryCondition.2D.inlet  */
+    double outlet_ruvp[4];   /* boundaryCondition.2D.outlet */
+    double wall_temperature; /* isothermal wall temperature (FVMR_BC_WALL_ISOTHERMAL) */
+    double venkat_K;         /* Venkatakrishnan constant K */
+    double dsr_shock_x;      /* double-Mach-reflection shock foot */
+    double dsr_shock_y;
+    double dsr_shock_speed;  /* reference: 20.0 */
+} fvmr_params;
+
+/* Per-step report of fvmr_step/fvmr_advance. */
+typedef struct fvmr_step_info {
+    double t;                /* physical time after the step(s) */
+    double dt;               /* dt of the last step (after the T-t clamp) */
+    double residual[4];      /* norm of dU/dt of the last step (stage 1), if requested */
+    int    residual_norm;    /* in: 0 = do not compute, else FVMR_NORM_* */
+    int    n_restricted;     /* cells repaired by the restriction in the last step */
+    int    steps_done;
+    int    reserved;
+} fvmr_step_info;
+
+/* Parameters of the synthetic mesh generators (fvmr_mesh_generate). */
+typedef struct fvmr_gen_params {
+    int    ni, nj;           /* cells (quads) along the two index directions */
+    double a, b, c, d;       /* family-specific lengths, see fvmr_mesh_generate */
+    double jitter;           /* interior node jitter, fraction of local spacing */
+    unsigned long long seed; /* splitmix64 seed for the jitter */
+    int    naca_digits;      /* e.g. 12 for NACA0012 */
+    int    reserved;
+} fvmr_gen_params;
+
+typedef struct fvmr_mesh   fvmr_mesh;    /* host mesh (SU2 ingest / generator) */
+typedef struct fvmr_solver fvmr_handle;  /* device solver */
+
+/* ---------------------------------------------------------------------- */
+/* library                                                                 */
+/* ---------------------------------------------------------------------- */
+const char *fvmr_last_error(void);
+/* Defaults of FVMR/src/global/GlobalPara.cpp:4-90 (Roe, GG, Barth, linear, Euler, explicit). */
+int fvmr_params_default(fvmr_params *p);
+/* freestream from Mach/AoA: AirParameterConverter (FVMR/src/input/AirParameterConverter.cpp:3-24) */
+int fvmr_ruvp_from_mach(double Ma, double AoA_deg, double T0, double p0, double gamma, double R,
+                        double ruvp[4]);
+
+/* ---------------------------------------------------------------------- */
+/* host mesh: SU2MeshReader::read_mesh_and_process + FVM_2D::ini* +        */
+/* OldDataConverter::Convert_FVM2D_to_HostData, with a hashed edge builder */
+/* that reproduces the reference's first-appearance edge order.           */
+/* ---------------------------------------------------------------------- */
+/* SU2MeshReader::read_mesh_and_process (FVMR/src/input/SU2MeshReader.cpp:10-19).
+ * equation selects the wall type of "wall"-named markers (BoundaryManager.cpp:21-24). */
+int fvmr_mesh_read_su2(const char *path, int equation, double scale, fvmr_mesh **out);
+/* Same pipeline from in-memory lists (SU2 semantics: node/element IDs are the
+ * file's 0-based ids; each marker is a list of node-id pairs).
+ * elem_types: 5 triangle, 9 quad (split (0,1,3),(3,1,2) as SU2MeshReader.cpp:103-119);
+ * elem_conn holds 4 ids per element (ignored 4th for triangles). */
+int fvmr_mesh_from_lists(int num_node, const double *x, const double *y, const int *node_ids,
+                         int num_elem, const int *elem_types, const int *elem_conn,
+                         int num_marker, const char *const *marker_names,
+                         const int *marker_offsets, const int *marker_edge_nodes,
+                         int equation, fvmr_mesh **out);
+/* Deterministic synthetic meshes (see FVMR_GEN_*). */
+int fvmr_mesh_generate(int family, const fvmr_gen_params *g, int equation, fvmr_mesh **out);
+/* Write a mesh produced by fvmr_mesh_generate/fvmr_mesh_from_lists as SU2 text. */
+int fvmr_mesh_write_su2(const fvmr_mesh *m, const char *path);
+int fvmr_mesh_soa_view(const fvmr_mesh *m, fvmr_mesh_soa *out);
+/* BoundaryV2 edge sets (FVMR/src/gpu/datatype/BoundaryV2.h:13-23): name, type and edge list of set s (0-based). */
+int fvmr_mesh_boundary_set(const fvmr_mesh *m, int s, const char **name, int *type,
+                           const int **edges, int *num_edges);
+/* Post-CCW-swap element node IDs (1-based file ids +1, as FVM_2D keeps them). */
+int fvmr_mesh_element_node_ids(const fvmr_mesh *m, const int **ids3);
+void fvmr_mesh_free(fvmr_mesh *m);
+
+/* FieldInitializer::setInitialAndBoundaryCondition (FVMR/src/input/FieldInitializer.cpp:22-69):
+ * args: vortex {x, y, strength}; shock tube {x, y, nx, ny}; DSR {shock_x, shock_y, angle_deg}. */
+int fvmr_field_init(const fvmr_mesh_soa *m, const fvmr_params *p, int type, const double *args,
+                    double *const U[4]);
+
+/* ---------------------------------------------------------------------- */
+/* device solver: GPUSolver2 + CIteration::iteration_device               */
+/* ---------------------------------------------------------------------- */
+/* GPUSolver2::allocateMemory + initializeData + host_to_device (GPUSolver2.cpp:54-165,232-239).
+ * Copies the mesh to the device (renumbered when p->renumber), state set to zero. */
+int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out);
+/* GPUSolver2::freeMemory (GPUSolver2.cpp:241-292). */
+int fvmr_destroy(fvmr_handle *h);
+/* elementField U host->device / device->host (GPUSolver2.cpp:232-239; CDriver.cpp:107-112);
+ * arrays are in the caller's (reference) element order. */
+int fvmr_set_state(fvmr_handle *h, const double *const U[4]);
+int fvmr_get_state(fvmr_handle *h, double *const U[4]);
+/* CIteration::iteration_device (CIteration.cpp:89-161): ruvp/Uold -> global dt ->
+ * clamp to T - *t -> evolve (forward Euler or SSP-RK3) -> restriction.  *t advanced. */
+int fvmr_step(fvmr_handle *h, double *t, double T, fvmr_step_info *info);
+/* nsteps x fvmr_step with dt and t kept on the device (no host round trip per step). */
+int fvmr_advance(fvmr_handle *h, int nsteps, double *t, double T, fvmr_step_info *info);
+/* parity hook: calculateFunctionF_device (EvolveGPU.cu:40-51): rhs = dU/dt of the current state. */
+int fvmr_stage_rhs(fvmr_handle *h, double *const rhs[4]);
+/* parity hook: Gradient_2 (GradientGPU.cu:513-551): limited gradients of the current state. */
+int fvmr_gradient(fvmr_handle *h, double *const Ux[4], double *const Uy[4]);
+/* parity hook: get_global_dt_device (CalculateDtGPU.cu:97-103) of the current state (no clamp). */
+int fvmr_global_dt(fvmr_handle *h, double *dt);
+/* get_residual_host (Residual.cpp:4-43) of the last step's rhs, computed on the device. */
+int fvmr_residual(fvmr_handle *h, int norm, double out[4]);
+/* The cell permutation used on the device: perm[new] = reference index. */
+int fvmr_permutation(fvmr_handle *h, int *perm);
+
+/* ---------------------------------------------------------------------- */
+/* measurement                                                             */
+/* ---------------------------------------------------------------------- */
+/* 1: record HIP events around every kernel launch on the handle's stream. */
+int fvmr_profile_enable(fvmr_handle *h, int on);
+/* Average ms per launch and launch count of a kernel class since the last reset:
+ * kernel: 0 = flux/update (stage), 1 = gradient+limiter, 2 = dt, 3 = restriction, 4 = residual. */
+int fvmr_profile_query(fvmr_handle *h, int kernel, double *avg_ms, int *count);
+int fvmr_profile_reset(fvmr_handle *h);
+/* Wait for all work of the handle. */
+int fvmr_synchronize(fvmr_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FVMR_H */

@@ -1,0 +1,272 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the CPU oracle (oracle/fvmr_oracle.cpp).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this
+package, and only as the checker (or the timed CPU baseline), never as the product.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from cfd_solver_2d_gpu_amd._abi import MeshSoA, Params, StepInfo  # struct layouts of include/fvmr.h
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libfvmr_oracle.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_SIG = {
+    "orc_last_error": (C.c_char_p, []),
+    "orc_mesh_read_su2": (C.c_void_p, [C.c_char_p, C.c_int, C.c_double]),
+    "orc_mesh_from_lists": (C.c_void_p, [C.c_int, _dp, _dp, _ip, C.c_int, _ip, _ip, C.c_int,
+                                         C.POINTER(C.c_char_p), _ip, _ip, C.c_int]),
+    "orc_mesh_free": (None, [C.c_void_p]),
+    "orc_mesh_soa": (None, [C.c_void_p, C.POINTER(MeshSoA)]),
+    "orc_mesh_element_node_ids": (_ip, [C.c_void_p]),
+    "orc_mesh_num_sets": (C.c_int, [C.c_void_p]),
+    "orc_mesh_set_name": (C.c_char_p, [C.c_void_p, C.c_int]),
+    "orc_mesh_set_type": (C.c_int, [C.c_void_p, C.c_int]),
+    "orc_mesh_set_size": (C.c_int, [C.c_void_p, C.c_int]),
+    "orc_mesh_set_edges": (_ip, [C.c_void_p, C.c_int]),
+    "orc_field_init": (C.c_int, [C.POINTER(MeshSoA), C.POINTER(Params), C.c_int, _dp, _dp * 4]),
+    "orc_solver_create": (C.c_void_p, [C.POINTER(MeshSoA), C.POINTER(Params)]),
+    "orc_solver_free": (None, [C.c_void_p]),
+    "orc_set_state": (None, [C.c_void_p, _dp * 4]),
+    "orc_get_state": (None, [C.c_void_p, _dp * 4]),
+    "orc_gradient": (C.c_int, [C.c_void_p, _dp * 4, _dp * 4]),
+    "orc_stage_rhs": (C.c_int, [C.c_void_p, C.c_double, _dp * 4]),
+    "orc_edge_flux": (None, [C.c_void_p, _dp * 4]),
+    "orc_global_dt": (C.c_double, [C.c_void_p]),
+    "orc_step": (C.c_int, [C.c_void_p, _dp, C.c_double, C.POINTER(StepInfo)]),
+    "orc_residual": (None, [C.c_void_p, C.c_int, _dp]),
+    "orc_restrict": (C.c_int, [C.c_void_p]),
+    "orc_roe": (None, [_dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
+    "orc_llf": (None, [_dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
+    "orc_farfield": (None, [_dp, _dp, C.c_double, C.c_double, C.c_double, _dp]),
+    "orc_slip": (None, [_dp, C.c_double, C.c_double, _dp]),
+    "orc_u2ruvp": (None, [_dp, C.c_double, _dp]),
+    "orc_ruvp2u": (None, [_dp, C.c_double, _dp]),
+    "orc_inv2x2": (None, [_dp, C.c_int]),
+}
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        for k, (r, a) in _SIG.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _err(what):
+    raise RuntimeError(f"oracle {what} failed: {lib().orc_last_error().decode()}")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _rows(U):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    return U, (_dp * 4)(*[_dptr(U[k]) for k in range(4)])
+
+
+def vec(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, _dptr(a)
+
+
+class OracleMesh:
+    """Connectivity built by the reference algorithm (linear-search edge lookup)."""
+
+    def __init__(self, h):
+        if not h:
+            _err("mesh build")
+        self._h = C.c_void_p(h)
+        self.soa = MeshSoA()
+        lib().orc_mesh_soa(self._h, C.byref(self.soa))
+
+    @classmethod
+    def read_su2(cls, path, equation=1, scale=1.0):
+        return cls(lib().orc_mesh_read_su2(str(path).encode(), equation, scale))
+
+    @classmethod
+    def from_lists(cls, x, y, node_ids, elem_types, elem_conn, markers, equation=1):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        ids = np.ascontiguousarray(node_ids, dtype=np.int32)
+        et = np.ascontiguousarray(elem_types, dtype=np.int32)
+        ec = np.full((len(et), 4), -1, dtype=np.int32)
+        conn = np.asarray(elem_conn)
+        ec[:, :conn.shape[1]] = conn
+        names = (C.c_char_p * max(1, len(markers)))(*[m[0].encode() for m in markers])
+        offs = np.zeros(len(markers) + 1, dtype=np.int32)
+        pairs = []
+        for k, (_, e) in enumerate(markers):
+            pairs.extend(e)
+            offs[k + 1] = len(pairs)
+        en = np.ascontiguousarray(np.array(pairs, dtype=np.int32).reshape(-1, 2)) if pairs else np.zeros((1, 2), np.int32)
+        return cls(lib().orc_mesh_from_lists(len(x), _dptr(x), _dptr(y), _iptr(ids), len(et), _iptr(et), _iptr(ec),
+                                             len(markers), names, _iptr(offs), _iptr(en), equation))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().orc_mesh_free(self._h)
+            self._h = C.c_void_p()
+
+    @property
+    def num_element(self):
+        return self.soa.num_element
+
+    def arrays(self):
+        from cfd_solver_2d_gpu_amd import soa_arrays
+        return soa_arrays(self.soa)
+
+    def element_node_ids(self):
+        p = lib().orc_mesh_element_node_ids(self._h)
+        return np.ctypeslib.as_array(p, shape=(self.num_element * 3,)).reshape(-1, 3).copy()
+
+    def boundary_sets(self):
+        out = []
+        for s in range(lib().orc_mesh_num_sets(self._h)):
+            n = lib().orc_mesh_set_size(self._h, s)
+            e = np.ctypeslib.as_array(lib().orc_mesh_set_edges(self._h, s), shape=(n,)).copy() if n else np.zeros(0, np.int32)
+            out.append((lib().orc_mesh_set_name(self._h, s).decode(), lib().orc_mesh_set_type(self._h, s), e))
+        return out
+
+
+def field_init(soa, params, init_type, args=()):
+    n = soa.num_element
+    U, rows = _rows(np.zeros((4, n)))
+    a = np.zeros(max(4, len(args)))
+    a[:len(args)] = args
+    if lib().orc_field_init(C.byref(soa), C.byref(params), init_type, _dptr(a), rows):
+        _err("field_init")
+    return U
+
+
+class OracleSolver:
+    """FP64 CPU restatement of CIteration::iteration_host (forward Euler) plus SSP-RK3."""
+
+    def __init__(self, mesh, params):
+        self.mesh = mesh
+        self.n = mesh.soa.num_element
+        self.nf = mesh.soa.num_edge
+        self._h = C.c_void_p(lib().orc_solver_create(C.byref(mesh.soa), C.byref(params)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().orc_solver_free(self._h)
+            self._h = C.c_void_p()
+
+    def set_state(self, U):
+        U, rows = _rows(U)
+        lib().orc_set_state(self._h, rows)
+
+    def get_state(self):
+        U, rows = _rows(np.zeros((4, self.n)))
+        lib().orc_get_state(self._h, rows)
+        return U
+
+    def gradient(self):
+        gx, rx = _rows(np.zeros((4, self.n)))
+        gy, ry = _rows(np.zeros((4, self.n)))
+        if lib().orc_gradient(self._h, rx, ry):
+            _err("gradient")
+        return gx, gy
+
+    def stage_rhs(self, t_rk=0.0):
+        R, rows = _rows(np.zeros((4, self.n)))
+        if lib().orc_stage_rhs(self._h, t_rk, rows):
+            _err("stage_rhs")
+        return R
+
+    def edge_flux(self):
+        F, rows = _rows(np.zeros((4, self.nf)))
+        lib().orc_edge_flux(self._h, rows)
+        return F
+
+    def global_dt(self):
+        return lib().orc_global_dt(self._h)
+
+    def step(self, t, T, residual_norm=0):
+        tt = C.c_double(t)
+        info = StepInfo(residual_norm=residual_norm)
+        if lib().orc_step(self._h, C.byref(tt), T, C.byref(info)):
+            _err("step")
+        return tt.value, info
+
+    def residual(self, norm=3):
+        out = (C.c_double * 4)()
+        lib().orc_residual(self._h, norm, out)
+        return np.array(out[:])
+
+    def restrict(self):
+        return lib().orc_restrict(self._h)
+
+
+def roe(UL, UR, nx, ny, length, gamma=1.4):
+    UL, pl = vec(UL)
+    UR, pr = vec(UR)
+    F, pf = vec(np.zeros(4))
+    lib().orc_roe(pl, pr, nx, ny, length, gamma, pf)
+    return F
+
+
+def llf(UL, UR, nx, ny, length, gamma=1.4):
+    UL, pl = vec(UL)
+    UR, pr = vec(UR)
+    F, pf = vec(np.zeros(4))
+    lib().orc_llf(pl, pr, nx, ny, length, gamma, pf)
+    return F
+
+
+def farfield(UL, ruvp_inf, nx, ny, gamma=1.4):
+    UL, pl = vec(UL)
+    ri, pi = vec(ruvp_inf)
+    UR, pr = vec(np.zeros(4))
+    lib().orc_farfield(pl, pi, nx, ny, gamma, pr)
+    return UR
+
+
+def slip(UL, nx, ny):
+    UL, pl = vec(UL)
+    UR, pr = vec(np.zeros(4))
+    lib().orc_slip(pl, nx, ny, pr)
+    return UR
+
+
+def u2ruvp(U, gamma=1.4):
+    U, pu = vec(U)
+    r, pr = vec(np.zeros(4))
+    lib().orc_u2ruvp(pu, gamma, pr)
+    return r
+
+
+def ruvp2u(r, gamma=1.4):
+    r, pr = vec(r)
+    U, pu = vec(np.zeros(4))
+    lib().orc_ruvp2u(pr, gamma, pu)
+    return U
+
+
+def inv2x2(m, compat=False):
+    m, pm = vec(np.array(m, dtype=np.float64).reshape(4))
+    lib().orc_inv2x2(pm, 1 if compat else 0)
+    return m

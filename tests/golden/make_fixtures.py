@@ -1,0 +1,26 @@
+"""Copies the reference's own data files used as fixtures into tests/golden (gzip).
+
+Run in a checkout where the reference repository is at /root/reference.  The files are
+data: quick_start SU2 meshes and one continue file the reference wrote at step 20000
+(FVMR/src/output/FieldWriter.cpp:264-326), which pins the post-CCW element node lists,
+the reference edge numbering of the boundary edges and a realistic FP32 state.
+"""
+import gzip
+import os
+import shutil
+
+REF = "/root/reference/quick_start/U2NITS-10.0"
+HERE = os.path.dirname(os.path.abspath(__file__))
+FILES = [
+    ("2D_isentropic_vortex/ivortex50.su2", "ivortex50.su2.gz"),
+    ("2D_cylinder/cylinder_allfar_50.su2", "cylinder_allfar_50.su2.gz"),
+    ("2D_cylinder/pause_cylinder_allfar_50[20000].dat", "pause_cylinder_allfar_50_20000.dat.gz"),
+    ("1D_shocktube/shocktube_100.su2", "shocktube_100.su2.gz"),
+]
+
+if __name__ == "__main__":
+    for src, dst in FILES:
+        with open(os.path.join(REF, src), "rb") as f, open(os.path.join(HERE, dst), "wb") as raw:
+            with gzip.GzipFile(fileobj=raw, mode="wb", compresslevel=9, mtime=0) as g:
+                shutil.copyfileobj(f, g)
+        print(dst, os.path.getsize(os.path.join(HERE, dst)))
