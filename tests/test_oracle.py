@@ -1,0 +1,260 @@
+"""Pins the CPU oracle with closed forms, invariants and textbook special cases, each
+chosen so that a plausible slip (dropped term, wrong sign or index, transposed operand)
+fails.  The reference ships no tests or golden vectors for this path; the reference-held
+data files are used in test_mesh.py and the GPU parity tests."""
+import numpy as np
+import pytest
+
+import cfd_solver_2d_gpu_amd as F
+import oracle as O
+from helpers import vortex_case
+
+G = 1.4
+
+
+def phys_flux(U, nx, ny, g=G):
+    rho, ru, rv, rE = U
+    u, v = ru / rho, rv / rho
+    p = (g - 1) * (rE - 0.5 * rho * (u * u + v * v))
+    un = u * nx + v * ny
+    return np.array([rho * un, ru * un + p * nx, rv * un + p * ny, (rE + p) * un])
+
+
+def state(rho, u, v, p, g=G):
+    return np.array([rho, rho * u, rho * v, p / (g - 1) + 0.5 * rho * (u * u + v * v)])
+
+
+STATES = [state(1.0, 0.3, -0.2, 1.0), state(1.4, 2.0, 0.1, 1.0), state(0.5, -0.7, 0.4, 0.3)]
+NORMALS = [(1.0, 0.0), (0.6, 0.8), (-np.sqrt(0.5), np.sqrt(0.5))]
+
+
+def test_conversions_round_trip():
+    for U in STATES:
+        assert np.allclose(O.ruvp2u(O.u2ruvp(U)), U, rtol=1e-15, atol=1e-15)
+    r = O.u2ruvp(state(1.2, 0.5, -0.25, 0.9))
+    assert np.allclose(r, [1.2, 0.5, -0.25, 0.9], rtol=1e-14)
+
+
+@pytest.mark.parametrize("solver", [O.roe, O.llf])
+def test_riemann_consistency_and_conservation(solver):
+    for U in STATES:
+        for nx, ny in NORMALS:
+            # consistency: F(U, U) = length * physical flux
+            assert np.allclose(solver(U, U, nx, ny, 0.7), 0.7 * phys_flux(U, nx, ny), rtol=1e-13, atol=1e-13)
+    for UL, UR in [(STATES[0], STATES[1]), (STATES[1], STATES[2]), (STATES[2], STATES[0])]:
+        for nx, ny in NORMALS:
+            # conservation: the flux seen from the other side with the flipped normal is opposite
+            f = solver(UL, UR, nx, ny, 1.3)
+            g = solver(UR, UL, -nx, -ny, 1.3)
+            assert np.allclose(f, -g, rtol=1e-12, atol=1e-12)
+
+
+def test_roe_is_upwind_for_supersonic_mass_and_momentum():
+    # all eigenvalues positive: the dissipation makes mass and momentum the left flux;
+    # the energy component keeps the reference's own energy jump (documented quirk)
+    UL = state(1.0, 3.0, 0.1, 1.0)
+    UR = state(1.1, 2.9, 0.2, 1.2)
+    f = O.roe(UL, UR, 1.0, 0.0, 1.0)
+    assert np.allclose(f[:3], phys_flux(UL, 1.0, 0.0)[:3], rtol=1e-12)
+
+
+def test_llf_closed_form():
+    UL, UR = STATES[0], STATES[2]
+    nx, ny = 0.6, 0.8
+    rl, rr = O.u2ruvp(UL), O.u2ruvp(UR)
+    lam = max(abs(rl[1] * nx + rl[2] * ny) + np.sqrt(G * rl[3] / rl[0]),
+              abs(rr[1] * nx + rr[2] * ny) + np.sqrt(G * rr[3] / rr[0]))
+    expect = 0.5 * (phys_flux(UL, nx, ny) + phys_flux(UR, nx, ny) - lam * (UR - UL)) * 2.0
+    assert np.allclose(O.llf(UL, UR, nx, ny, 2.0), expect, rtol=1e-13)
+
+
+@pytest.mark.parametrize("nx,ny", [(1.0, 0.0), (-1.0, 0.0), (0.6, 0.8), (-0.8, 0.6)])
+@pytest.mark.parametrize("mach", [0.3, 2.0])
+def test_farfield_ghost_of_freestream_is_freestream(nx, ny, mach):
+    # all four regimes (sub/supersonic, inflow/outflow) return the freestream itself
+    inf = np.array([1.0, mach * np.sqrt(G), 0.0, 1.0])
+    U = O.ruvp2u(inf)
+    assert np.allclose(O.farfield(U, inf, nx, ny), U, rtol=1e-13, atol=1e-14)
+
+
+def test_slip_wall_reflects_normal_velocity():
+    U = state(1.3, 0.4, -0.9, 0.8)
+    nx, ny = 0.6, 0.8
+    g = O.slip(U, nx, ny)
+    r, rg = O.u2ruvp(U), O.u2ruvp(g)
+    un, ut = r[1] * nx + r[2] * ny, r[1] * ny - r[2] * nx
+    assert np.isclose(rg[1] * nx + rg[2] * ny, -un) and np.isclose(rg[1] * ny - rg[2] * nx, ut)
+    assert g[0] == U[0] and g[3] == U[3]
+
+
+def test_inv2x2_correct_and_reference_bug():
+    A = np.array([[2.0, 1.0], [1.0, 3.0]])
+    good = O.inv2x2(A).reshape(2, 2)
+    assert np.allclose(good @ A, np.eye(2), atol=1e-14)
+    bug = O.inv2x2(A, compat=True).reshape(2, 2)
+    # MatrixKernel.h:48-49: every det >= EPSILON is replaced by -EPSILON
+    assert np.allclose(bug, -np.array([[3.0, -1.0], [-1.0, 2.0]]) / 1e-10)
+    small = O.inv2x2(np.array([[1e-6, 0], [0, 1e-6]])).reshape(2, 2)
+    assert np.allclose(small, np.eye(2) / 1e-10 * 1e-6)
+
+
+def linear_field(m, a=(1.0, 0.3, -0.2), b=(0.5, 0.1, 0.05), c=(0.2, -0.1, 0.3), d=(2.5, 0.2, 0.1)):
+    xy = m.arrays()["element_xy"]
+    U = np.stack([k[0] + k[1] * xy[0] + k[2] * xy[1] for k in (a, b, c, d)])
+    return U, np.array([[k[1] for k in (a, b, c, d)], [k[2] for k in (a, b, c, d)]])
+
+
+def test_gradients_constant_and_linear_fields():
+    m = F.Mesh.generate(F.GEN_WEDGE_CHANNEL, 24, 12, a=3.0, b=1.0, c=0.5, d=15.0, jitter=0.1, seed=5)
+    a = m.arrays()
+    p = F.default_params(limiter=F.LIM_NONE)
+    s = O.OracleSolver(m, p)
+    s.set_state(np.ones((4, m.num_element)) * np.array([[1.0], [0.2], [0.1], [2.5]]))
+    gx, gy = s.gradient()
+    assert np.abs(gx).max() < 1e-12 and np.abs(gy).max() < 1e-12   # GG of a constant is zero everywhere
+    U, grad = linear_field(m)
+    nvalid = (a["element_neighbors"][:3] >= 0).sum(axis=0)
+    for gtype in (F.GRA_LEASTSQUARE,):
+        p = F.default_params(limiter=F.LIM_NONE, gradient=gtype)
+        s = O.OracleSolver(m, p)
+        s.set_state(U)
+        gx, gy = s.gradient()
+        ok = nvalid >= 2   # LSQ is exact for linear fields with two independent neighbours
+        assert np.allclose(gx[:, ok], grad[0][:, None], atol=1e-10)
+        assert np.allclose(gy[:, ok], grad[1][:, None], atol=1e-10)
+
+
+def test_barth_bounds_vertex_values():
+    m, p, U = vortex_case(32, limiter=F.LIM_BARTH)
+    a = m.arrays()
+    s = O.OracleSolver(m, p)
+    s.set_state(U)
+    gx, gy = s.gradient()
+    p0 = F.default_params(inf_ruvp=[1, 1, 1, 1], limiter=F.LIM_NONE)
+    s0 = O.OracleSolver(m, p0)
+    s0.set_state(U)
+    rx, ry = s0.gradient()
+    phi = np.where(np.abs(rx) > 0, gx / np.where(rx == 0, 1, rx), 1.0)
+    assert np.all(phi <= 1 + 1e-12) and np.all(phi >= -1e-12)
+    nb = a["element_neighbors"][:3]
+    xy = a["element_xy"]
+    for k in range(4):
+        vals = np.concatenate([U[k][None], np.where(nb >= 0, U[k][nb], U[k][None])])
+        lo, hi = vals.min(axis=0), vals.max(axis=0)
+        for j in range(3):
+            nd = a["element_nodes"][j]
+            v = U[k] + gx[k] * (a["node_xy"][0][nd] - xy[0]) + gy[k] * (a["node_xy"][1][nd] - xy[1])
+            assert np.all(v <= hi + 1e-9) and np.all(v >= lo - 1e-9)
+
+
+def test_venkatakrishnan_does_not_amplify():
+    m, p, U = vortex_case(32, limiter=F.LIM_VENKATAKRISHNAN)
+    s = O.OracleSolver(m, p)
+    s.set_state(U)
+    gx, _ = s.gradient()
+    s0 = O.OracleSolver(m, F.default_params(inf_ruvp=[1, 1, 1, 1], limiter=F.LIM_NONE))
+    s0.set_state(U)
+    rx, _ = s0.gradient()
+    assert np.all(np.abs(gx) <= np.abs(rx) * (1 + 1e-12) + 1e-300)
+
+
+def test_dt_closed_form_on_periodic_square_at_rest():
+    """u = v = 0: alpha_e = c sqrt(L).  A corner cell has two periodic faces, each counted a
+    second time through its pair edge (CalculateDt.cpp:51-52 with R = pair's L), which makes
+    it the minimum: dt = CFL (h^2/2) / (c (4 sqrt(h) + sqrt(sqrt(2) h)))."""
+    n, Lbox = 16, 10.0
+    m = F.Mesh.generate(F.GEN_PERIODIC_SQUARE, n, n, b=Lbox)
+    p = F.default_params(inf_ruvp=[1.0, 0.0, 0.0, 1.0], CFL=0.5)
+    U = F.field_init(m, p, F.INIT_UNIFORM)
+    s = O.OracleSolver(m, p)
+    s.set_state(U)
+    h = Lbox / n
+    c = np.sqrt(1.4)
+    # edge lengths are rounded through float32 in the reference
+    hf = float(np.float32(h))
+    df = float(np.float32(np.sqrt(2) * h))
+    expect = 0.5 * (h * h / 2) / (c * np.sqrt(hf) * 4 + c * np.sqrt(df))
+    assert np.isclose(s.global_dt(), expect, rtol=1e-12)
+
+
+def test_isentropic_vortex_closed_form():
+    m, p, U = vortex_case(16)
+    xy = m.arrays()["element_xy"]
+    PI = 3.1415926535897      # the reference's PI
+    r2 = (xy[0] - 5) ** 2 + (xy[1] - 5) ** 2
+    du = 5 / (2 * PI) * np.exp(0.5 * (1 - r2))
+    T = 1 - (G - 1) * 25 / (8 * G * PI ** 2) * np.exp(1 - r2)
+    rho = T ** (1 / (G - 1))
+    expect = np.stack([state(r, 1 - d * (y - 5), 1 + d * (x - 5), r * t)
+                       for r, d, x, y, t in zip(rho, du, xy[0], xy[1], T)], axis=1)
+    assert np.allclose(U, expect, rtol=1e-13)
+    assert np.array_equal(U, O.field_init(m.soa, p, F.INIT_ISENTROPIC_VORTEX, (5, 5, 5)))
+
+
+def test_uniform_flow_is_preserved_by_all_boundaries():
+    m = F.Mesh.generate(F.GEN_RECT_CHANNEL, 24, 12, a=3.0, b=1.0, jitter=0.1, seed=9)
+    ruvp = [1.4, 2.0, 0.0, 1.0]
+    for scheme in (F.SOL_ROE, F.SOL_LLF):
+        for gtype in (F.GRA_GREENGAUSS, F.GRA_LEASTSQUARE):
+            p = F.default_params(inf_ruvp=ruvp, inlet_ruvp=ruvp, outlet_ruvp=ruvp, flux_scheme=scheme, gradient=gtype)
+            s = O.OracleSolver(m, p)
+            s.set_state(F.field_init(m, p, F.INIT_UNIFORM))
+            assert np.abs(s.stage_rhs()).max() < 1e-11
+
+
+def test_restriction_volume_sum_quirk():
+    """A bad cell takes sum(U_nb V_nb) / sum over (4 x V_nb): the volume sum is accumulated once
+    per variable (Restrict.cpp:48-52), giving a quarter of the volume-weighted mean."""
+    m, p, U = vortex_case(16)
+    a = m.arrays()
+    c = 50
+    U = U.copy()
+    U[0, c] = -1.0
+    s = O.OracleSolver(m, p)
+    s.set_state(U)
+    assert s.restrict() == 1
+    nb = a["element_neighbors"][:3, c]
+    nb = nb[nb >= 0]
+    V = a["element_volume"][nb]
+    expect = (U[:, nb] * V).sum(axis=1) / (4 * V.sum())
+    assert np.allclose(s.get_state()[:, c], expect, rtol=1e-14)
+
+
+def test_ssp_rk3_is_third_order_in_time():
+    """Same mesh, smooth flow (no limiter, LLF): the time error of SSP-RK3 falls by ~8 per
+    halving of dt; a wrong stage coefficient falls to first order (~2)."""
+    m = F.Mesh.generate(F.GEN_PERIODIC_SQUARE, 16, 16, b=10.0)
+    T = 0.3
+
+    def run(cfl):
+        p = F.default_params(inf_ruvp=[1, 1, 1, 1], CFL=cfl, limiter=F.LIM_NONE, flux_scheme=F.SOL_LLF,
+                             time_advance=F.EVO_RK3)
+        s = O.OracleSolver(m, p)
+        s.set_state(F.field_init(m, p, F.INIT_ISENTROPIC_VORTEX, (5, 5, 5)))
+        t = 0.0
+        while t < T:
+            t, _ = s.step(t, T)
+        return s.get_state()
+
+    ref = run(0.05)
+    e1 = np.linalg.norm(run(0.8) - ref)
+    e2 = np.linalg.norm(run(0.4) - ref)
+    assert e1 / e2 > 6.0, e1 / e2
+
+
+def test_forward_euler_is_first_order_in_time():
+    m = F.Mesh.generate(F.GEN_PERIODIC_SQUARE, 16, 16, b=10.0)
+    T = 0.3
+
+    def run(cfl):
+        p = F.default_params(inf_ruvp=[1, 1, 1, 1], CFL=cfl, limiter=F.LIM_NONE, flux_scheme=F.SOL_LLF)
+        s = O.OracleSolver(m, p)
+        s.set_state(F.field_init(m, p, F.INIT_ISENTROPIC_VORTEX, (5, 5, 5)))
+        t = 0.0
+        while t < T:
+            t, _ = s.step(t, T)
+        return s.get_state()
+
+    ref = run(0.02)
+    r = np.linalg.norm(run(0.4) - ref) / np.linalg.norm(run(0.2) - ref)
+    assert 1.6 < r < 2.6, r
