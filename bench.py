@@ -116,12 +116,23 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="FP64 Roe only")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (calibrates rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
     import cfd_solver_2d_gpu_amd as F
+
+    if args.copy_probe_mib:
+        import torch
+        n = args.copy_probe_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     m, wall, U64, kern = run_gpu(F.SOL_ROE, F.PREC_FP64, args.steps, args.warmup, profile=True)
     nc, nf, nv = m.num_element, m.num_edge, m.num_node

@@ -56,12 +56,13 @@ struct Mesh {
     const G *emx, *emy;              // midpoint
     const int *epair;                // periodic pair edge or -1
     const int *etype;                // BC code of the edge's set, -1 interior
-    const int *dtedge;               // [kMaxDtSlots][nc] edges adding to alphaC, reference order
+    const int *dtcode;               // per cell: alphaC contributions in reference edge order
 };
 
 struct Phys {
     double gamma, R, Pr, Pr_visc, mu0, sutherland_C1, CFL, wall_T, venkat_K;
     double dsr_x, dsr_y, dsr_speed;
+    double g_ga1, inv_g;           // gamma/(gamma-1), 1/gamma
     double inf[4], inlet[4], outlet[4];
     int equation, scheme, reconstruct, gradient, limiter, mu_model, compat_inv;
 };
@@ -106,59 +107,80 @@ template <class T>
 __device__ __forceinline__ bool out_of_range(const T r[4]) {
     return r[0] < T(fvmr::kRHO_MIN) || r[0] > T(fvmr::kRHO_MAX) || r[3] < T(fvmr::kP_MIN) || r[3] > T(fvmr::kP_MAX);
 }
-__device__ __forceinline__ bool is_periodic(int t) { return t >= FVMR_BC_PERIODIC_0 && t <= FVMR_BC_PERIODIC_0 + 9; }
-
 // ---------------------------------------------------------------------------
 // Face physics (FluxGPU.cu:34-205, RoeGPU.cu, LocalLaxFriedrichsGPU.cu)
 // ---------------------------------------------------------------------------
-// get_Uvector_reconstruct_2_device (FluxGPU.cu:47-61)
+// A reconstructed or ghost state: conserved U and its primitives, computed once with one
+// reciprocal of rho (U2ruvp, PhysicalKernelGPU.h:11-16, with the divisions by U[0]
+// folded into that reciprocal; results differ from the reference by rounding only).
+template <class C>
+struct FaceState {
+    C U[4];
+    C r, u, v, p, ir;
+};
+
+template <class C>
+__device__ __forceinline__ void set_prim(FaceState<C> &s, C gamma) {
+    s.ir = C(1) / s.U[0];
+    s.r = s.U[0];
+    s.u = s.U[1] * s.ir;
+    s.v = s.U[2] * s.ir;
+    s.p = s.r * (gamma - 1) * (s.U[3] * s.ir - (s.u * s.u + s.v * s.v) * C(0.5));
+}
+
+__device__ __forceinline__ bool is_periodic(int t) { return t >= FVMR_BC_PERIODIC_0 && t <= FVMR_BC_PERIODIC_0 + 9; }
+
+// get_Uvector_reconstruct_2_device (FluxGPU.cu:47-61): linear (3002) or constant
+// reconstruction; back to the cell value when rho or p leave their range.
 template <class C, class G>
 __device__ __forceinline__ void reconstruct(const Mesh<G> &m, const Phys &ph, const double *__restrict__ U,
                                             const G *__restrict__ Ux, const G *__restrict__ Uy, int c, C x, C y,
-                                            C Ud[4]) {
+                                            FaceState<C> &s) {
     const int nc = m.nc;
+    const C gamma = (C)ph.gamma;
     C Uc[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) Uc[i] = (C)U[i * nc + c];
     if (ph.reconstruct == FVMR_REC_LINEAR) {
         const C dx = x - (C)m.cx[c], dy = y - (C)m.cy[c];
 #pragma unroll
-        for (int i = 0; i < 4; i++) Ud[i] = Uc[i] + (C)Ux[i * nc + c] * dx + (C)Uy[i * nc + c] * dy;
+        for (int i = 0; i < 4; i++) s.U[i] = Uc[i] + (C)Ux[i * nc + c] * dx + (C)Uy[i * nc + c] * dy;
     } else {
 #pragma unroll
-        for (int i = 0; i < 4; i++) Ud[i] = Uc[i];
+        for (int i = 0; i < 4; i++) s.U[i] = Uc[i];
     }
-    C r[4];
-    u2ruvp(Ud, r, (C)ph.gamma);
-    if (out_of_range(r)) {
+    set_prim(s, gamma);
+    const C rp[4] = {s.r, s.u, s.v, s.p};
+    if (out_of_range(rp)) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) Ud[i] = Uc[i];
+        for (int i = 0; i < 4; i++) s.U[i] = Uc[i];
+        set_prim(s, gamma);
     }
 }
 
 // get_UR_wallNonViscous_device (FluxGPU.cu:63-77)
 template <class C>
-__device__ __forceinline__ void ghost_slip(const C UL[4], C UR[4], C nx, C ny) {
-    C uxL = UL[1] / UL[0], uyL = UL[2] / UL[0];
-    C unL = uxL * nx + uyL * ny;
-    C utL = uxL * ny - uyL * nx;
-    C unR = -unL, utR = utL;
-    C uxR = unR * nx + utR * ny;
-    C uyR = unR * ny - utR * nx;
-    UR[0] = UL[0];
-    UR[1] = UR[0] * uxR;
-    UR[2] = UR[0] * uyR;
-    UR[3] = UL[3];
+__device__ __forceinline__ void ghost_slip(const FaceState<C> &L, FaceState<C> &R, C nx, C ny, C gamma) {
+    const C uxL = L.u, uyL = L.v;
+    const C unL = uxL * nx + uyL * ny;
+    const C utL = uxL * ny - uyL * nx;
+    const C unR = -unL, utR = utL;
+    const C uxR = unR * nx + utR * ny;
+    const C uyR = unR * ny - utR * nx;
+    R.U[0] = L.U[0];
+    R.U[1] = R.U[0] * uxR;
+    R.U[2] = R.U[0] * uyR;
+    R.U[3] = L.U[3];
+    set_prim(R, gamma);
 }
 
-// get_UR_farField_device (FluxGPU.cu:87-167)
+// get_UR_farField_device (FluxGPU.cu:87-167): Riemann-invariant ghost in four regimes
 template <class C>
-__device__ __forceinline__ void ghost_farfield(const C UL[4], const double *ruvp_inf, C UR[4], C nx, C ny, C gamma) {
+__device__ __forceinline__ void ghost_farfield(const FaceState<C> &L, const double *ruvp_inf, FaceState<C> &R, C nx,
+                                            C ny, C gamma) {
     const C ga1 = gamma - 1;
     const C rho_f = (C)ruvp_inf[0], u_f = (C)ruvp_inf[1], v_f = (C)ruvp_inf[2], p_f = (C)ruvp_inf[3];
-    C e[4];
-    u2ruvp(UL, e, gamma);
-    const C rho_e = e[0], u_e = e[1], v_e = e[2], p_e = e[3];
+    const C rho_e = L.r, u_e = L.u, v_e = L.v, p_e = L.p;
     const C a2_f = gamma * p_f / rho_f;
     const C af = sqrt(a2_f);
     const C ae = sqrt(gamma * p_e / rho_e);
@@ -186,84 +208,80 @@ __device__ __forceinline__ void ghost_farfield(const C UL[4], const double *ruvp
     } else {
         r0[0] = rho_e; r0[1] = u_e; r0[2] = v_e; r0[3] = p_e;
     }
-    ruvp2u(r0, UR, gamma);
+    ruvp2u(r0, R.U, gamma);
+    set_prim(R, gamma);
 }
 
 // New: no-slip isothermal wall ghost (u_R = -u_L, p_R = p_L, T_R = 2 T_w - T_L, floored).
 template <class C>
-__device__ __forceinline__ void ghost_isothermal(const C UL[4], C UR[4], C Tw, C R, C gamma) {
-    C r[4];
-    u2ruvp(UL, r, gamma);
-    C TL = r[3] / (r[0] * R);
+__device__ __forceinline__ void ghost_isothermal(const FaceState<C> &L, FaceState<C> &R, C Tw, C Rg, C gamma) {
+    const C TL = L.p / (L.r * Rg);
     C TR = C(2.0) * Tw - TL;
     if (TR < C(0.5) * Tw) TR = C(0.5) * Tw;
-    C g[4] = {r[3] / (R * TR), -r[1], -r[2], r[3]};
-    ruvp2u(g, UR, gamma);
+    const C g[4] = {L.p / (Rg * TR), -L.u, -L.v, L.p};
+    ruvp2u(g, R.U, gamma);
+    set_prim(R, gamma);
 }
 
-// LocalLaxFriedrichs2d (LocalLaxFriedrichsGPU.cu:4-49)
+// LocalLaxFriedrichs2d (LocalLaxFriedrichsGPU.cu:4-49) on precomputed primitives
 template <class C>
-__device__ __forceinline__ void flux_llf(const C UL[4], const C UR[4], C nx, C ny, C length, C flux[4], C gamma) {
-    C L[4], R[4];
-    u2ruvp(UL, L, gamma);
-    u2ruvp(UR, R, gamma);
-    const C rL = L[0], uL = L[1], vL = L[2], pL = L[3];
-    const C rR = R[0], uR = R[1], vR = R[2], pR = R[3];
-    const C unL = nx * uL + ny * vL, unR = nx * uR + ny * vR;
+__device__ __forceinline__ void flux_llf(const FaceState<C> &L, const FaceState<C> &R, C nx, C ny, C length,
+                                         C flux[4], C gamma, C g_ga1) {
+    const C unL = nx * L.u + ny * L.v, unR = nx * R.u + ny * R.v;
     C FnL[4], FnR[4];
-    FnL[0] = rL * unL; FnL[1] = rL * unL * uL + nx * pL; FnL[2] = rL * unL * vL + ny * pL;
-    const C hl = pL * gamma / (rL * (gamma - C(1.))) + C(0.5) * (uL * uL + vL * vL);
-    FnL[3] = rL * unL * hl;
-    FnR[0] = rR * unR; FnR[1] = rR * unR * uR + nx * pR; FnR[2] = rR * unR * vR + ny * pR;
-    const C hr = pR * gamma / (rR * (gamma - C(1.))) + C(0.5) * (uR * uR + vR * vR);
-    FnR[3] = rR * unR * hr;
-    const C lam = tmax(fabs(unL) + sqrt(gamma * pL / rL), fabs(unR) + sqrt(gamma * pR / rR));
+    FnL[0] = L.r * unL; FnL[1] = L.r * unL * L.u + nx * L.p; FnL[2] = L.r * unL * L.v + ny * L.p;
+    const C hl = L.p * g_ga1 * L.ir + C(0.5) * (L.u * L.u + L.v * L.v);
+    FnL[3] = L.r * unL * hl;
+    FnR[0] = R.r * unR; FnR[1] = R.r * unR * R.u + nx * R.p; FnR[2] = R.r * unR * R.v + ny * R.p;
+    const C hr = R.p * g_ga1 * R.ir + C(0.5) * (R.u * R.u + R.v * R.v);
+    FnR[3] = R.r * unR * hr;
+    const C lam = tmax(fabs(unL) + sqrt(gamma * L.p * L.ir), fabs(unR) + sqrt(gamma * R.p * R.ir));
 #pragma unroll
-    for (int i = 0; i < 4; i++) flux[i] = C(0.5) * (FnL[i] + FnR[i] - lam * (UR[i] - UL[i])) * length;
+    for (int i = 0; i < 4; i++) flux[i] = C(0.5) * (FnL[i] + FnR[i] - lam * (R.U[i] - L.U[i])) * length;
 }
 
-// ConvectRoeCommon2d + RoeDissapationTerm2d (RoeGPU.cu:4-159): no entropy fix
+// ConvectRoeCommon2d + RoeDissapationTerm2d (RoeGPU.cu:4-159): no entropy fix, the
+// reference energy jump Etm; divisions folded into five reciprocals
 template <class C>
-__device__ __forceinline__ void flux_roe(const C UL[4], const C UR[4], C nx, C ny, C area, C F[4], C gamma) {
-    C L[4], R[4];
-    u2ruvp(UL, L, gamma);
-    u2ruvp(UR, R, gamma);
-    const C rL = L[0], uL = L[1], vL = L[2], pL = L[3];
-    const C rR = R[0], uR = R[1], vR = R[2], pR = R[3];
+__device__ __forceinline__ void flux_roe(const FaceState<C> &L, const FaceState<C> &R, C nx, C ny, C area, C F[4],
+                                         C gamma, C g_ga1, C inv_g) {
+    const C rL = L.r, uL = L.u, vL = L.v, pL = L.p;
+    const C rR = R.r, uR = R.u, vR = R.v, pR = R.p;
     const C ul = nx * uL + ny * vL, ur = nx * uR + ny * vR;
     const C rul = rL * ul, rur = rR * ur;
     const C V2L = uL * uL + vL * vL, V2R = uR * uR + vR * vR;
     const C ga1 = gamma - C(1.0);
-    const C HL = pL * gamma / (rL * ga1) + C(0.5) * V2L;
-    const C HR = pR * gamma / (rR * ga1) + C(0.5) * V2R;
+    const C HL = pL * g_ga1 * L.ir + C(0.5) * V2L;
+    const C HR = pR * g_ga1 * R.ir + C(0.5) * V2R;
     const C psum = pL + pR;
     F[0] = C(0.5) * (rul + rur) * area;
     F[1] = C(0.5) * (rul * uL + rur * uR + nx * psum) * area;
     F[2] = C(0.5) * (rul * vL + rur * vR + ny * psum) * area;
     F[3] = C(0.5) * (rul * HL + rur * HR) * area;
-    const C rrorl = sqrt(rR / rL);
-    const C rrorlp1 = rrorl + C(1.0);
-    const C rm = sqrt(rR * rL);
-    const C um = (uL + uR * rrorl) / rrorlp1;
-    const C vm = (vL + vR * rrorl) / rrorlp1;
-    const C Hm = (HL + HR * rrorl) / rrorlp1;
+    const C rrorl = sqrt(rR * L.ir);
+    const C inv_p1 = C(1.0) / (rrorl + C(1.0));
+    const C rm = rL * rrorl;
+    const C um = (uL + uR * rrorl) * inv_p1;
+    const C vm = (vL + vR * rrorl) * inv_p1;
+    const C Hm = (HL + HR * rrorl) * inv_p1;
     const C Vm2 = um * um + vm * vm;
     const C am2 = ga1 * (Hm - C(0.5) * Vm2);
     const C am = sqrt(am2);
+    const C inv_am = C(1.0) / am;
     const C unm = um * nx + vm * ny;
     const C eig1 = fabs(unm);
     const C eig2 = fabs(unm + am);
     const C eig3 = fabs(unm - am);
-    const C ER = HR - pR / rR, EL = HL - pL / rL;
-    const C Etm = Hm / gamma + ga1 / gamma * C(0.5) * Vm2;
+    const C ER = HR - pR * R.ir, EL = HL - pL * L.ir;
+    const C Etm = Hm * inv_g + ga1 * inv_g * C(0.5) * Vm2;
     const C dW0 = rR - rL;
     const C dW1 = rm * (uR - uL) + dW0 * um;
     const C dW2 = rm * (vR - vL) + dW0 * vm;
     const C dW3 = rm * (ER - EL) + dW0 * Etm;
-    const C astar = (eig2 + eig3) / C(2.0);
-    const C Mstar = (eig2 - eig3) / C(2.0) / am;
+    const C astar = (eig2 + eig3) * C(0.5);
+    const C Mstar = (eig2 - eig3) * C(0.5) * inv_am;
     const C dun = nx * (uR - uL) + ny * (vR - vL);
-    const C dUroe = Mstar * dun + (astar - eig1) * (pR - pL) / rm / am2;
+    const C dUroe = Mstar * dun + (astar - eig1) * (pR - pL) / rm * (inv_am * inv_am);
     const C dProe = Mstar * (pR - pL) + (astar - eig1) * rm * dun;
     F[0] -= C(0.5) * (eig1 * dW0 + dUroe * rm) * area;
     F[1] -= C(0.5) * (eig1 * dW1 + dUroe * rm * um + dProe * nx) * area;
@@ -330,7 +348,7 @@ __device__ __forceinline__ void flux_viscous(const Mesh<G> &m, const Phys &ph, c
 
 // edge_convection_flux_device_kernel body (FluxGPU.cu:207-283) (+ viscous part for NS):
 // the canonical flux of edge e, oriented L -> R.
-template <class C, class G>
+template <class C, class G, bool NS>
 __device__ __forceinline__ void edge_flux(const Mesh<G> &m, const Phys &ph, const double *__restrict__ U,
                                           const G *__restrict__ Ux, const G *__restrict__ Uy, double t_rk, int e,
                                           C F[4]) {
@@ -339,37 +357,42 @@ __device__ __forceinline__ void edge_flux(const Mesh<G> &m, const Phys &ph, cons
     const C xe = (C)m.emx[e], ye = (C)m.emy[e];
     const int2 lr = m.eLR[e];
     const C gamma = (C)ph.gamma;
-    C UL[4], UR[4] = {C(0), C(0), C(0), C(0)};
-    reconstruct<C, G>(m, ph, U, Ux, Uy, lr.x, xe, ye, UL);
-    switch (bType) {
-    case FVMR_BC_SYMMETRY:
-    case FVMR_BC_WALL_NONVISCOUS: ghost_slip(UL, UR, nx, ny); break;
-    case FVMR_BC_WALL_ADIABAT:  // FluxGPU.cu:79-85 (momenta of a zeroed U_R negated)
-        UR[0] = UL[0]; UR[1] = -C(0); UR[2] = -C(0); UR[3] = UL[3];
-        break;
-    case FVMR_BC_WALL_ISOTHERMAL: ghost_isothermal(UL, UR, (C)ph.wall_T, (C)ph.R, gamma); break;
-    case FVMR_BC_INLET: ghost_farfield(UL, ph.inlet, UR, nx, ny, gamma); break;
-    case FVMR_BC_OUTLET: ghost_farfield(UL, ph.outlet, UR, nx, ny, gamma); break;
-    case FVMR_BC_INF: ghost_farfield(UL, ph.inf, UR, nx, ny, gamma); break;
-    case FVMR_BC_DSR: {
-        const double right = ph.dsr_x + ((double)m.emy[e] + ph.dsr_speed * t_rk) / sqrt(3.0);
-        if ((double)m.emx[e] < right) ghost_farfield(UL, ph.inlet, UR, nx, ny, gamma);
-        else ghost_farfield(UL, ph.outlet, UR, nx, ny, gamma);
-        break;
-    }
-    default:
+    FaceState<C> L, R;
+    reconstruct<C, G>(m, ph, U, Ux, Uy, lr.x, xe, ye, L);
+    if (bType == -1 || is_periodic(bType)) {
+        // interior and periodic edges (get_UR_inner_and_periodic_device, FluxGPU.cu:169-188)
         if (lr.y != -1) {
+            C xr = xe, yr = ye;
             const int pe = m.epair[e];
-            if (bType != -1) {  // periodic: R reconstructed at the pair's midpoint
-                if (pe >= 0 && pe < m.nf) reconstruct<C, G>(m, ph, U, Ux, Uy, lr.y, (C)m.emx[pe], (C)m.emy[pe], UR);
-            } else {
-                reconstruct<C, G>(m, ph, U, Ux, Uy, lr.y, xe, ye, UR);
-            }
+            if (bType != -1 && pe >= 0 && pe < m.nf) { xr = (C)m.emx[pe]; yr = (C)m.emy[pe]; }
+            reconstruct<C, G>(m, ph, U, Ux, Uy, lr.y, xr, yr, R);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) R.U[i] = C(0);
+            set_prim(R, gamma);
         }
+    } else if (bType == FVMR_BC_SYMMETRY || bType == FVMR_BC_WALL_NONVISCOUS) {
+        ghost_slip(L, R, nx, ny, gamma);
+    } else if (bType == FVMR_BC_WALL_ADIABAT) {
+        // FluxGPU.cu:79-85: momenta of a zeroed U_R negated
+        R.U[0] = L.U[0]; R.U[1] = -C(0); R.U[2] = -C(0); R.U[3] = L.U[3];
+        set_prim(R, gamma);
+    } else if (bType == FVMR_BC_WALL_ISOTHERMAL) {
+        ghost_isothermal(L, R, (C)ph.wall_T, (C)ph.R, gamma);
+    } else {
+        const double *ruvp = ph.inf;
+        if (bType == FVMR_BC_INLET) ruvp = ph.inlet;
+        else if (bType == FVMR_BC_OUTLET) ruvp = ph.outlet;
+        else if (bType == FVMR_BC_DSR) {
+            const double right = ph.dsr_x + ((double)m.emy[e] + ph.dsr_speed * t_rk) / sqrt(3.0);
+            ruvp = ((double)m.emx[e] < right) ? ph.inlet : ph.outlet;
+        }
+        ghost_farfield(L, ruvp, R, nx, ny, gamma);
     }
-    if (ph.scheme == FVMR_SOL_LLF) flux_llf(UL, UR, nx, ny, len, F, gamma);
-    else flux_roe(UL, UR, nx, ny, len, F, gamma);
-    if (ph.equation == FVMR_EQ_NS) {
+    const C g_ga1 = (C)ph.g_ga1;
+    if (ph.scheme == FVMR_SOL_LLF) flux_llf(L, R, nx, ny, len, F, gamma, g_ga1);
+    else flux_roe(L, R, nx, ny, len, F, gamma, g_ga1, (C)ph.inv_g);
+    if (NS) {
         C Fv[4];
         flux_viscous<C, G>(m, ph, U, Ux, Uy, e, lr.x, lr.y, nx, ny, len, Fv);
 #pragma unroll
@@ -380,170 +403,260 @@ __device__ __forceinline__ void edge_flux(const Mesh<G> &m, const Phys &ph, cons
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Spectral radius of one edge for the cells a and b (b = -1: boundary, one-sided)
+// (update_alphaC_host, CalculateDt.cpp:24-62).  ra/rb: primitives of a and b.  Returns the
+// inviscid part in *inv and the viscous part divided by V_c in *vis (0 for Euler).
+template <class G>
+__device__ __forceinline__ void edge_alpha(const Mesh<G> &m, const Phys &ph, const double ra[4], const double rb[4],
+                                           bool has_b, int e, double vol_c, double *inv, double *vis) {
+    const double gamma = ph.gamma;
+    double rho = ra[0], u = ra[1], v = ra[2], p = ra[3];
+    if (has_b) {
+        rho = 0.5 * (ra[0] + rb[0]);
+        u = 0.5 * (ra[1] + rb[1]);
+        v = 0.5 * (ra[2] + rb[2]);
+        p = 0.5 * (ra[3] + rb[3]);
+    }
+    const double len = (double)m.elen[e];
+    const double dx = (double)m.enx[e] * len, dy = (double)m.eny[e] * len;
+    *inv = fabs(u * dx + v * dy) + sqrt(gamma * p / rho) * sqrt(len);
+    *vis = 0.0;
+    if (ph.equation == FVMR_EQ_NS) {
+        const double value_1 = fmax(4.0 / 3.0, gamma / ph.Pr);
+        const double T = p / ph.R / rho;
+        *vis = 2.0 * len * value_1 * mu_of_T(ph, T) / rho / vol_c;
+    }
+}
+
+// primitives (rho, u, v, p) of a conserved state with one reciprocal of rho
+__device__ __forceinline__ void prim4(const double U[4], double r[4], double gamma) {
+    const double ir = 1.0 / U[0];
+    r[0] = U[0];
+    r[1] = U[1] * ir;
+    r[2] = U[2] * ir;
+    r[3] = r[0] * (gamma - 1) * (U[3] * ir - (r[1] * r[1] + r[2] * r[2]) * 0.5);
+}
+
+// Work-group layout of the cell kernels: 3 wave64 x 64 cells; wave j handles local face
+// slot j (and vertex j) of the 64 cells, results meet in LDS.
+constexpr int kCellsPerGroup = 64;
+constexpr int kGroup3 = 3 * kCellsPerGroup;
+
 // Gradient_2 (GradientGPU.cu:513-551): GG (GradientGreenGaussKernel :223-425) or LSQ
-// (Gradient.cpp:160-246, correct inverse unless compat), then Barth (:453-502) or Venkatakrishnan.
-template <class C, class G>
-__global__ __launch_bounds__(kBlock) void k_gradient(Mesh<G> m, Phys ph, const double *__restrict__ U,
-                                                     G *__restrict__ Ux, G *__restrict__ Uy) {
-    const int c = xcd_block() * kBlock + threadIdx.x;
+// (Gradient.cpp:160-246, correct inverse unless compat), then Barth (:453-502) or
+// Venkatakrishnan.  Waves 0-2 gather face j's neighbour, geometry and vertex j; wave 0
+// then forms the gradient and the limiter.  With DT the kernel also forms alphaC of the
+// cell from the same loads, summed in the reference edge order, and writes the block
+// minimum of CFL V / alphaC (get_global_dt_device, CalculateDtGPU.cu:97-103).
+template <class C, class G, bool DT>
+__global__ __launch_bounds__(kGroup3) void k_gradient(Mesh<G> m, Phys ph, const double *__restrict__ U,
+                                                      G *__restrict__ Ux, G *__restrict__ Uy,
+                                                      double *__restrict__ partial) {
+    __shared__ C sUo[3][4][kCellsPerGroup];   // out-cell state of slot j (own state when none)
+    __shared__ C sA[3][3][kCellsPerGroup];    // GG: gC, Sx*s, Sy*s;  LSQ: dx, dy
+    __shared__ C sN[3][2][kCellsPerGroup];    // vertex j relative to the centroid
+    __shared__ int sFl[3][kCellsPerGroup];    // bit 0: slot has an out cell; bit 1: non-periodic neighbour
+    __shared__ double sAl[DT ? 3 : 1][4][kCellsPerGroup];  // dt: own inv, own vis, pair inv, pair vis
+    const int lane = threadIdx.x & 63, j = threadIdx.x >> 6;
+    const int blk = xcd_block();
+    const int c = blk * kCellsPerGroup + lane;
     const int nc = m.nc;
-    if (c >= nc) return;
-    C Uc[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) Uc[k] = (C)U[k * nc + c];
-    const C xc = (C)m.cx[c], yc = (C)m.cy[c];
-    int out[3], nbr[3];
-    C sgn[3], Sx[3], Sy[3], gC[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        const int e = m.cface[i * nc + c];
+    const bool live = c < nc;
+    if (live) {
+        const C xc = (C)m.cx[c], yc = (C)m.cy[c];
+        const int e = m.cface[j * nc + c];
         const int2 lr = m.eLR[e];
         const int pe = m.epair[e];
-        if (c != lr.y) { sgn[i] = C(1); out[i] = lr.y; }
-        else { sgn[i] = C(-1); out[i] = lr.x; }
-        // element neighbours exclude periodic faces (ElementSoA.h:22)
-        nbr[i] = (pe >= 0) ? -1 : out[i];
-        const C len = (C)m.elen[e];
-        Sx[i] = (C)m.enx[e] * len;
-        Sy[i] = (C)m.eny[e] * len;
-        gC[i] = C(0.5);
-        if (ph.gradient == FVMR_GRA_GREENGAUSS && out[i] != -1) {
-            const C fx = (C)m.emx[e], fy = (C)m.emy[e];
-            C Fx = (C)m.cx[out[i]], Fy = (C)m.cy[out[i]];
-            if (m.etype[e] != -1 && pe >= 0) {
-                Fx += fx - (C)m.emx[pe];
-                Fy += fy - (C)m.emy[pe];
+        const int out = (c != lr.y) ? lr.y : lr.x;
+        const C sgn = (c != lr.y) ? C(1) : C(-1);
+        double Uo[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) Uo[k] = U[k * nc + (out != -1 ? out : c)];
+#pragma unroll
+        for (int k = 0; k < 4; k++) sUo[j][k][lane] = (C)Uo[k];
+        const int nd = m.cnode[j * nc + c];
+        sN[j][0][lane] = (C)m.nx_[nd] - xc;
+        sN[j][1][lane] = (C)m.ny_[nd] - yc;
+        if (ph.gradient == FVMR_GRA_GREENGAUSS) {
+            const C len = (C)m.elen[e];
+            C gC = C(0.5);
+            if (out != -1) {
+                const C fx = (C)m.emx[e], fy = (C)m.emy[e];
+                C Fx = (C)m.cx[out], Fy = (C)m.cy[out];
+                if (m.etype[e] != -1 && pe >= 0) {
+                    Fx += fx - (C)m.emx[pe];
+                    Fy += fy - (C)m.emy[pe];
+                }
+                const C ax = fx - Fx, ay = fy - Fy, bx = xc - Fx, by = yc - Fy;
+                const C dFf = sqrt(ax * ax + ay * ay);
+                C dFC = sqrt(bx * bx + by * by);
+                if (dFC < C(fvmr::kEPS)) dFC = C(fvmr::kEPS);
+                gC = dFf / dFC;
             }
-            const C ax = fx - Fx, ay = fy - Fy, bx = xc - Fx, by = yc - Fy;
-            const C dFf = sqrt(ax * ax + ay * ay);
-            C dFC = sqrt(bx * bx + by * by);
-            if (dFC < C(fvmr::kEPS)) dFC = C(fvmr::kEPS);
-            gC[i] = dFf / dFC;
+            sA[j][0][lane] = gC;
+            sA[j][1][lane] = (C)m.enx[e] * len * sgn;
+            sA[j][2][lane] = (C)m.eny[e] * len * sgn;
+        } else {
+            const int nbr = (pe >= 0) ? -1 : out;   // element neighbours exclude periodic faces
+            sA[j][0][lane] = (nbr != -1) ? (C)m.cx[nbr] - xc : C(0);
+            sA[j][1][lane] = (nbr != -1) ? (C)m.cy[nbr] - yc : C(0);
+        }
+        sFl[j][lane] = ((out != -1) ? 1 : 0) | ((out != -1 && pe < 0) ? 2 : 0);
+        if (DT) {
+            double Uc[4], rc[4], ro[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) Uc[k] = U[k * nc + c];
+            prim4(Uc, rc, ph.gamma);
+            prim4(Uo, ro, ph.gamma);
+            const double vol_c = (double)m.vol[c];
+            double a0, v0, a1 = 0.0, v1 = 0.0;
+            edge_alpha(m, ph, rc, ro, out != -1, e, vol_c, &a0, &v0);
+            if (pe >= 0) edge_alpha(m, ph, rc, ro, true, pe, vol_c, &a1, &v1);
+            sAl[j][0][lane] = a0;
+            sAl[j][1][lane] = v0;
+            sAl[j][2][lane] = a1;
+            sAl[j][3][lane] = v1;
         }
     }
-    C Uo[3][4];
+    __syncthreads();
+    if (j != 0) return;
+    double dtc = 1e10;
+    if (live) {
+        C Uc[4];
 #pragma unroll
-    for (int i = 0; i < 3; i++)
+        for (int k = 0; k < 4; k++) Uc[k] = (C)U[k * nc + c];
+        int fl[3];
 #pragma unroll
-        for (int k = 0; k < 4; k++) Uo[i][k] = (out[i] != -1) ? (C)U[k * nc + out[i]] : Uc[k];
-    C gx[4], gy[4];
-    if (ph.gradient == FVMR_GRA_GREENGAUSS) {
-        const C inv_vol = C(1) / (C)m.vol[c];
+        for (int i = 0; i < 3; i++) fl[i] = sFl[i][lane];
+        C gx[4], gy[4];
+        if (ph.gradient == FVMR_GRA_GREENGAUSS) {
+            const C inv_vol = C(1) / (C)m.vol[c];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            C s0 = C(0), s1 = C(0);
+            for (int k = 0; k < 4; k++) {
+                C s0 = C(0), s1 = C(0);
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
-                C phif;
-                if (out[i] != -1) phif = gC[i] * Uc[k] + (C(1.0) - gC[i]) * Uo[i][k];
-                else phif = Uc[k];
-                s0 += phif * Sx[i] * sgn[i];
-                s1 += phif * Sy[i] * sgn[i];
+                for (int i = 0; i < 3; i++) {
+                    const C g = sA[i][0][lane];
+                    const C phif = (fl[i] & 1) ? g * Uc[k] + (C(1.0) - g) * sUo[i][k][lane] : Uc[k];
+                    s0 += phif * sA[i][1][lane];
+                    s1 += phif * sA[i][2][lane];
+                }
+                gx[k] = inv_vol * s0;
+                gy[k] = inv_vol * s1;
             }
-            gx[k] = inv_vol * s0;
-            gy[k] = inv_vol * s1;
-        }
-    } else {
-        // least squares over the valid (non-periodic) neighbours in slot order; sums skip
-        // the invalid slots, so the order equals the reference's compacted neighbour list
-        C dX[3][2];
+        } else {
+            C A[4];
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            dX[i][0] = (nbr[i] != -1) ? (C)m.cx[nbr[i]] - xc : C(0);
-            dX[i][1] = (nbr[i] != -1) ? (C)m.cy[nbr[i]] - yc : C(0);
-        }
-        C A[4];
+            for (int x = 0; x < 2; x++)
 #pragma unroll
-        for (int x = 0; x < 2; x++)
+                for (int y = 0; y < 2; y++) {
+                    C acc = C(0);
 #pragma unroll
-            for (int y = 0; y < 2; y++) {
-                C acc = C(0);
+                    for (int i = 0; i < 3; i++)
+                        if (fl[i] & 2) acc += sA[i][x][lane] * sA[i][y][lane];
+                    A[x * 2 + y] = acc;
+                }
+            C det = A[0] * A[3] - A[1] * A[2];
+            const C eps = C(fvmr::kEPS);
+            if (ph.compat_inv) {  // MatrixKernel.h:48-49 as written
+                if (det >= C(0) && det < eps) det = eps;
+                else if (det > -eps) det = -eps;
+            } else {
+                if (det >= C(0) && det < eps) det = eps;
+                else if (det < C(0) && det > -eps) det = -eps;
+            }
+            const C idet = C(1.0) / det;
+            const C i00 = A[3] * idet, i01 = -A[1] * idet, i10 = -A[2] * idet, i11 = A[0] * idet;
+            const bool any = (fl[0] | fl[1] | fl[2]) & 2;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                C a = C(0), b = C(0);
 #pragma unroll
                 for (int i = 0; i < 3; i++)
-                    if (nbr[i] != -1) acc += dX[i][x] * dX[i][y];
-                A[x * 2 + y] = acc;
+                    if (fl[i] & 2) a += (i00 * sA[i][0][lane] + i01 * sA[i][1][lane]) * (sUo[i][k][lane] - Uc[k]);
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+                    if (fl[i] & 2) b += (i10 * sA[i][0][lane] + i11 * sA[i][1][lane]) * (sUo[i][k][lane] - Uc[k]);
+                gx[k] = any ? a : C(0);
+                gy[k] = any ? b : C(0);
             }
-        C det = A[0] * A[3] - A[1] * A[2];
-        const C eps = C(fvmr::kEPS);
-        if (ph.compat_inv) {  // MatrixKernel.h:48-49 as written
-            if (det >= C(0) && det < eps) det = eps;
-            else if (det > -eps) det = -eps;
-        } else {
-            if (det >= C(0) && det < eps) det = eps;
-            else if (det < C(0) && det > -eps) det = -eps;
         }
-        const C idet = C(1.0) / det;
-        const C i00 = A[3] * idet, i01 = -A[1] * idet, i10 = -A[2] * idet, i11 = A[0] * idet;
-        C B0[3], B1[3];
+        if (ph.limiter != FVMR_LIM_NONE) {
+            C eps2 = C(0);
+            if (ph.limiter == FVMR_LIM_VENKATAKRISHNAN) {
+                const C Kh = (C)ph.venkat_K * sqrt((C)m.vol[c]);
+                eps2 = Kh * Kh * Kh;
+            }
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            B0[i] = i00 * dX[i][0] + i01 * dX[i][1];
-            B1[i] = i10 * dX[i][0] + i11 * dX[i][1];
-        }
-        bool any = false;
+            for (int k = 0; k < 4; k++) {
+                C lo = Uc[k], hi = Uc[k];
 #pragma unroll
-        for (int i = 0; i < 3; i++) any = any || (nbr[i] != -1);
+                for (int i = 0; i < 3; i++)
+                    if (fl[i] & 2) { lo = tmin(lo, sUo[i][k][lane]); hi = tmax(hi, sUo[i][k][lane]); }
+                C phi = C(1.0);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            C a = C(0), b = C(0);
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-                if (nbr[i] != -1) a += B0[i] * (Uo[i][k] - Uc[k]);
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-                if (nbr[i] != -1) b += B1[i] * (Uo[i][k] - Uc[k]);
-            gx[k] = any ? a : C(0);
-            gy[k] = any ? b : C(0);
-        }
-    }
-    if (ph.limiter == FVMR_LIM_BARTH || ph.limiter == FVMR_LIM_VENKATAKRISHNAN) {
-        C ndx[3], ndy[3];
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const int nd = m.cnode[j * nc + c];
-            ndx[j] = (C)m.nx_[nd] - xc;
-            ndy[j] = (C)m.ny_[nd] - yc;
-        }
-        C eps2 = C(0);
-        if (ph.limiter == FVMR_LIM_VENKATAKRISHNAN) {
-            const C Kh = (C)ph.venkat_K * sqrt((C)m.vol[c]);
-            eps2 = Kh * Kh * Kh;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            C lo = Uc[k], hi = Uc[k];
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-                if (nbr[i] != -1) { lo = tmin(lo, Uo[i][k]); hi = tmax(hi, Uo[i][k]); }
-            C phi = C(1.0);
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                C pn = C(1.0);
-                if (ph.limiter == FVMR_LIM_BARTH) {
-                    const C Un = Uc[k] + gx[k] * ndx[j] + gy[k] * ndy[j];
-                    C rel = Un - Uc[k];
-                    const C hrel = hi - Uc[k], lrel = lo - Uc[k];
-                    if (rel > C(0)) { rel += C(fvmr::kEPS); pn = tmin(C(1), hrel / rel); }
-                    else if (rel < C(0)) { rel -= C(fvmr::kEPS); pn = tmin(C(1), lrel / rel); }
-                } else {
-                    const C d2 = gx[k] * ndx[j] + gy[k] * ndy[j];
-                    if (d2 != C(0)) {
-                        const C d1 = (d2 > C(0)) ? (hi - Uc[k]) : (lo - Uc[k]);
-                        const C num = (d1 * d1 + eps2) * d2 + C(2.0) * d2 * d2 * d1;
-                        const C den = d1 * d1 + C(2.0) * d2 * d2 + d1 * d2 + eps2;
-                        pn = num / (den * d2);
+                for (int jn = 0; jn < 3; jn++) {
+                    const C ndx = sN[jn][0][lane], ndy = sN[jn][1][lane];
+                    C pn = C(1.0);
+                    if (ph.limiter == FVMR_LIM_BARTH) {
+                        const C Un = Uc[k] + gx[k] * ndx + gy[k] * ndy;
+                        C rel = Un - Uc[k];
+                        const C hrel = hi - Uc[k], lrel = lo - Uc[k];
+                        if (rel > C(0)) { rel += C(fvmr::kEPS); pn = tmin(C(1), hrel / rel); }
+                        else if (rel < C(0)) { rel -= C(fvmr::kEPS); pn = tmin(C(1), lrel / rel); }
+                    } else {
+                        const C d2 = gx[k] * ndx + gy[k] * ndy;
+                        if (d2 != C(0)) {
+                            const C d1 = (d2 > C(0)) ? (hi - Uc[k]) : (lo - Uc[k]);
+                            const C num = (d1 * d1 + eps2) * d2 + C(2.0) * d2 * d2 * d1;
+                            const C den = d1 * d1 + C(2.0) * d2 * d2 + d1 * d2 + eps2;
+                            pn = num / (den * d2);
+                        }
                     }
+                    phi = tmin(phi, pn);
                 }
-                phi = tmin(phi, pn);
+                gx[k] *= phi;
+                gy[k] *= phi;
             }
-            gx[k] *= phi;
-            gy[k] *= phi;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            Ux[k * nc + c] = (G)gx[k];
+            Uy[k * nc + c] = (G)gy[k];
+        }
+        if (DT) {
+            // alphaC in the reference edge order: count (3 bits) + 6 x (slot 2 bits, pair 1 bit)
+            const int code = m.dtcode[c];
+            const int cnt = code & 7;
+            double alpha_c = 0.0;
+            for (int q = 0; q < cnt; q++) {
+                const int ent = (code >> (3 + 3 * q)) & 7;
+                const int slot = ent & 3, pr = ent >> 2;
+                alpha_c += sAl[slot][2 * pr][lane];
+                if (ph.equation == FVMR_EQ_NS) alpha_c += sAl[slot][2 * pr + 1][lane];
+            }
+            dtc = ph.CFL * (double)m.vol[c] / alpha_c;
         }
     }
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        Ux[k * nc + c] = (G)gx[k];
-        Uy[k * nc + c] = (G)gy[k];
+    if (DT) {
+        const double bmin = wave_min(dtc);
+        if (lane == 0) partial[blk] = bmin;
     }
 }
 
@@ -551,33 +664,40 @@ __global__ __launch_bounds__(kBlock) void k_gradient(Mesh<G> m, Phys ph, const d
 // vector_weighted_divide_kernel, BasicAlgorithmGPU.cu:4-9), then
 //   stage 0:    Uout = Uin + dt rhs                      (vector_weighted_add_kernel)
 //   stage 1, 2: Uout = a Un + b (Uin + dt rhs)           (SSP-RK3)
-// Cells whose new state is NaN or out of range are appended to the restriction list.
-template <class C, class G>
-__global__ __launch_bounds__(kBlock) void k_stage(Mesh<G> m, Phys ph, const double *__restrict__ Uin,
-                                                  const double *__restrict__ Un, double *__restrict__ Uout,
-                                                  const G *__restrict__ Ux, const G *__restrict__ Uy,
-                                                  double *__restrict__ rhs_out, Scalars *__restrict__ sc,
-                                                  int *__restrict__ bad_list, double a, double b, int first) {
-    const int c = xcd_block() * kBlock + threadIdx.x;
+// Wave j evaluates the canonical flux of face slot j of 64 cells; wave 0 sums the three
+// signed fluxes in slot order and updates.  Cells whose new state is NaN or out of range
+// are appended to the restriction list.
+template <class C, class G, bool NS>
+__global__ __launch_bounds__(kGroup3) void k_stage(Mesh<G> m, Phys ph, const double *__restrict__ Uin,
+                                                   const double *__restrict__ Un, double *__restrict__ Uout,
+                                                   const G *__restrict__ Ux, const G *__restrict__ Uy,
+                                                   double *__restrict__ rhs_out, Scalars *__restrict__ sc,
+                                                   int *__restrict__ bad_list, double a, double b, int first) {
+    __shared__ double sF[3][4][kCellsPerGroup];
+    const int lane = threadIdx.x & 63, j = threadIdx.x >> 6;
+    const int c = xcd_block() * kCellsPerGroup + lane;
     const int nc = m.nc;
-    if (c >= nc) return;
-    const double dt = sc->dt;
-    const double t_rk = sc->t_rk;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        const int e = m.cface[i * nc + c];
+    if (c < nc) {
+        const double t_rk = sc->t_rk;
+        const int e = m.cface[j * nc + c];
         C F[4];
-        edge_flux<C, G>(m, ph, Uin, Ux, Uy, t_rk, e, F);
+        edge_flux<C, G, NS>(m, ph, Uin, Ux, Uy, t_rk, e, F);
         const double s = (c != m.eLR[e].y) ? 1.0 : -1.0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) acc[k] += s * (double)F[k];
+        for (int k = 0; k < 4; k++) sF[j][k][lane] = s * (double)F[k];
     }
+    __syncthreads();
+    if (j != 0 || c >= nc) return;
+    const double dt = sc->dt;
     const double w = -1.0 * (double)m.vol[c];
     double Unew[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        const double r = acc[k] / w;
+        double acc = 0.0;
+        acc += sF[0][k][lane];
+        acc += sF[1][k][lane];
+        acc += sF[2][k][lane];
+        const double r = acc / w;
         if (rhs_out) rhs_out[k * nc + c] = r;
         const double st = Uin[k * nc + c] + dt * r;
         Unew[k] = first ? st : a * Un[k * nc + c] + b * st;
@@ -633,94 +753,19 @@ __global__ __launch_bounds__(1024) void k_restrict(int nc, const int *__restrict
     }
 }
 
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-// get_global_dt_device (CalculateDtGPU.cu:97-103) with update_alphaC in the sequential
-// order of CalculateDt.cpp:10-64, then the clamp and t update of CIteration.cpp:134-139.
-template <class G>
-__global__ __launch_bounds__(kBlock) void k_dt(Mesh<G> m, Phys ph, const double *__restrict__ U,
-                                               double *__restrict__ partial, Scalars *__restrict__ sc, int advance) {
-    const int nc = m.nc;
-    const int blk = xcd_block();
-    const int c = blk * kBlock + threadIdx.x;
-    double dtc = 1e10;
-    if (c < nc) {
-        const double gamma = ph.gamma;
-        const double value_1 = fmax(4.0 / 3.0, gamma / ph.Pr);
-        double alpha_c = 0.0;
-        for (int k = 0; k < kMaxDtSlots; k++) {
-            const int e = m.dtedge[k * nc + c];
-            if (e < 0) break;
-            const int2 lr = m.eLR[e];
-            double rL[4], UL[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) UL[q] = U[q * nc + lr.x];
-            u2ruvp(UL, rL, gamma);
-            double rho, u, v, p;
-            if (lr.y != -1) {
-                double rR[4], UR[4];
-#pragma unroll
-                for (int q = 0; q < 4; q++) UR[q] = U[q * nc + lr.y];
-                u2ruvp(UR, rR, gamma);
-                rho = 0.5 * (rL[0] + rR[0]);
-                u = 0.5 * (rL[1] + rR[1]);
-                v = 0.5 * (rL[2] + rR[2]);
-                p = 0.5 * (rL[3] + rR[3]);
-            } else {
-                rho = rL[0]; u = rL[1]; v = rL[2]; p = rL[3];
-            }
-            const double len = (double)m.elen[e];
-            const double dx = (double)m.enx[e] * len, dy = (double)m.eny[e] * len;
-            const double un = u * dx + v * dy;
-            alpha_c += fabs(un) + sqrt(gamma * p / rho) * sqrt(len);
-            if (ph.equation == FVMR_EQ_NS) {
-                const double T = p / ph.R / rho;
-                const double mu = mu_of_T(ph, T);
-                const double aV = 2.0 * len * value_1 * mu / rho;
-                alpha_c += aV / (double)m.vol[c];
-            }
-        }
-        dtc = ph.CFL * (double)m.vol[c] / alpha_c;
-    }
-    __shared__ double smin[kBlock / 64];
-    __shared__ bool last;
-    double v = wave_min(dtc);
-    if ((threadIdx.x & 63) == 0) smin[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double b = smin[0];
-        for (int i = 1; i < kBlock / 64; i++) b = fmin(b, smin[i]);
-        partial[blk] = b;
-        __threadfence();
-        const unsigned int done = atomicAdd(&sc->counter, 1u);
-        last = (done == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+// Minimum of the block partials of k_gradient<DT>; then the clamp to T - t and the t
+// update of CIteration::iteration_device (CIteration.cpp:134-139).
+__global__ __launch_bounds__(1024) void k_dt_finish(int nparts, const double *__restrict__ partial,
+                                                    Scalars *__restrict__ sc, int advance) {
+    __shared__ double s[16];
     double b = 1e10;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) b = fmin(b, partial[i]);
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) b = fmin(b, partial[i]);
     b = wave_min(b);
-    if ((threadIdx.x & 63) == 0) smin[threadIdx.x >> 6] = b;
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = b;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double d = smin[0];
-        for (int i = 1; i < kBlock / 64; i++) d = fmin(d, smin[i]);
-        sc->counter = 0u;
+        double d = s[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++) d = fmin(d, s[i]);
         if (advance) {
             const double t = sc->t;
             if (t + d > sc->T) d = sc->T - t;
@@ -859,7 +904,7 @@ struct fvmr_solver {
     void *Ux = nullptr, *Uy = nullptr;
     double *rhs = nullptr;
     void *geo = nullptr;            // all floating geometry of type G in one block
-    int *cface = nullptr, *cnode = nullptr, *epair = nullptr, *etype = nullptr, *dtedge = nullptr;
+    int *cface = nullptr, *cnode = nullptr, *epair = nullptr, *etype = nullptr, *dtcode = nullptr;
     int2 *eLR = nullptr;
     double *partial = nullptr, *norm_out = nullptr, *fix = nullptr, *vol_d = nullptr;
     int *bad_list = nullptr;
@@ -885,7 +930,7 @@ Mesh<G> mesh_args(const fvmr_solver *h) {
     m.eLR = h->eLR;
     m.enx = (const G *)h->g_enx; m.eny = (const G *)h->g_eny; m.elen = (const G *)h->g_elen;
     m.emx = (const G *)h->g_emx; m.emy = (const G *)h->g_emy;
-    m.epair = h->epair; m.etype = h->etype; m.dtedge = h->dtedge;
+    m.epair = h->epair; m.etype = h->etype; m.dtcode = h->dtcode;
     return m;
 }
 
@@ -923,14 +968,27 @@ int prof_collect(fvmr_solver *h) {
 
 int grid_of(int n) { return (n + kBlock - 1) / kBlock; }
 
+int grid3_of(int n) { return (n + kCellsPerGroup - 1) / kCellsPerGroup; }
+
+// Gradient + limiter; with dt_mode >= 0 also the global dt (dt_mode 1: clamp and advance t).
 template <class C, class G>
-int launch_gradient(fvmr_solver *h, const double *U) {
+int launch_gradient(fvmr_solver *h, const double *U, int dt_mode) {
     hipEvent_t e0{};
     int rc = prof_begin(h, 1, &e0);
     if (rc) return rc;
-    k_gradient<C, G><<<grid_of(h->nc), kBlock, 0, h->stream>>>(mesh_args<G>(h), h->ph, U, (G *)h->Ux, (G *)h->Uy);
+    if (dt_mode >= 0)
+        k_gradient<C, G, true><<<grid3_of(h->nc), kGroup3, 0, h->stream>>>(mesh_args<G>(h), h->ph, U, (G *)h->Ux,
+                                                                          (G *)h->Uy, h->partial);
+    else
+        k_gradient<C, G, false><<<grid3_of(h->nc), kGroup3, 0, h->stream>>>(mesh_args<G>(h), h->ph, U, (G *)h->Ux,
+                                                                           (G *)h->Uy, h->partial);
     HIP_OK(hipGetLastError());
-    return prof_end(h, 1, e0);
+    if ((rc = prof_end(h, 1, e0))) return rc;
+    if (dt_mode < 0) return FVMR_OK;
+    if ((rc = prof_begin(h, 2, &e0))) return rc;
+    k_dt_finish<<<1, 1024, 0, h->stream>>>(grid3_of(h->nc), h->partial, h->sc, dt_mode);
+    HIP_OK(hipGetLastError());
+    return prof_end(h, 2, e0);
 }
 
 template <class C, class G>
@@ -939,8 +997,12 @@ int launch_stage(fvmr_solver *h, const double *Uin, const double *Un, double *Uo
     hipEvent_t e0{};
     int rc = prof_begin(h, 0, &e0);
     if (rc) return rc;
-    k_stage<C, G><<<grid_of(h->nc), kBlock, 0, h->stream>>>(mesh_args<G>(h), h->ph, Uin, Un, Uout, (const G *)h->Ux,
-                                                            (const G *)h->Uy, rhs_out, h->sc, h->bad_list, a, b, first);
+    if (h->ph.equation == FVMR_EQ_NS)
+        k_stage<C, G, true><<<grid3_of(h->nc), kGroup3, 0, h->stream>>>(mesh_args<G>(h), h->ph, Uin, Un, Uout,
+            (const G *)h->Ux, (const G *)h->Uy, rhs_out, h->sc, h->bad_list, a, b, first);
+    else
+        k_stage<C, G, false><<<grid3_of(h->nc), kGroup3, 0, h->stream>>>(mesh_args<G>(h), h->ph, Uin, Un, Uout,
+            (const G *)h->Ux, (const G *)h->Uy, rhs_out, h->sc, h->bad_list, a, b, first);
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 0, e0))) return rc;
     if ((rc = prof_begin(h, 3, &e0))) return rc;
@@ -950,35 +1012,24 @@ int launch_stage(fvmr_solver *h, const double *Uin, const double *Un, double *Uo
     return prof_end(h, 3, e0);
 }
 
-template <class G>
-int launch_dt(fvmr_solver *h, const double *U, int advance) {
-    hipEvent_t e0{};
-    int rc = prof_begin(h, 2, &e0);
-    if (rc) return rc;
-    k_dt<G><<<grid_of(h->nc), kBlock, 0, h->stream>>>(mesh_args<G>(h), h->ph, U, h->partial, h->sc, advance);
-    HIP_OK(hipGetLastError());
-    return prof_end(h, 2, e0);
-}
-
 // One step on the device (no host synchronisation).  store_rhs: keep stage-1 rhs.
 template <class C, class G>
 int device_step(fvmr_solver *h, bool store_rhs) {
     int rc;
     const int a = h->cur, b = (a + 1) % 3, c = (a + 2) % 3;
     double *Ua = h->Ubuf[a], *Ub = h->Ubuf[b], *Uc = h->Ubuf[c];
-    if ((rc = launch_dt<G>(h, Ua, 1))) return rc;
     if (h->p.time_advance == FVMR_EVO_EXPLICIT) {
-        if ((rc = launch_gradient<C, G>(h, Ua))) return rc;
+        if ((rc = launch_gradient<C, G>(h, Ua, 1))) return rc;
         if ((rc = launch_stage<C, G>(h, Ua, Ua, Ub, 0.0, 1.0, 1, store_rhs ? h->rhs : nullptr))) return rc;
         h->cur = b;
     } else {
         // SSP-RK3 (Shu-Osher): U1 = U + dt R(U); U2 = 3/4 U + 1/4 (U1 + dt R(U1));
         // U^{n+1} = 1/3 U + 2/3 (U2 + dt R(U2))
-        if ((rc = launch_gradient<C, G>(h, Ua))) return rc;
+        if ((rc = launch_gradient<C, G>(h, Ua, 1))) return rc;
         if ((rc = launch_stage<C, G>(h, Ua, Ua, Ub, 0.0, 1.0, 1, store_rhs ? h->rhs : nullptr))) return rc;
-        if ((rc = launch_gradient<C, G>(h, Ub))) return rc;
+        if ((rc = launch_gradient<C, G>(h, Ub, -1))) return rc;
         if ((rc = launch_stage<C, G>(h, Ub, Ua, Uc, 0.75, 0.25, 0, nullptr))) return rc;
-        if ((rc = launch_gradient<C, G>(h, Uc))) return rc;
+        if ((rc = launch_gradient<C, G>(h, Uc, -1))) return rc;
         if ((rc = launch_stage<C, G>(h, Uc, Ua, Ub, 1.0 / 3.0, 2.0 / 3.0, 0, nullptr))) return rc;
         h->cur = b;
     }
@@ -1022,7 +1073,7 @@ int build_geometry(fvmr_solver *h, const fvmr_mesh_soa *m, const std::vector<int
 
 void free_all(fvmr_solver *h) {
     for (auto &b : h->Ubuf) if (b) (void)hipFree(b);
-    void *ptrs[] = {h->Ux, h->Uy, h->rhs, h->geo, h->cface, h->cnode, h->epair, h->etype, h->dtedge, h->eLR,
+    void *ptrs[] = {h->Ux, h->Uy, h->rhs, h->geo, h->cface, h->cnode, h->epair, h->etype, h->dtcode, h->eLR,
                     h->partial, h->norm_out, h->fix, h->bad_list, h->sc, h->vol_d};
     for (void *q : ptrs) if (q) (void)hipFree(q);
     for (int k = 0; k < kNumProf; k++)
@@ -1097,6 +1148,7 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
     ph.gamma = p->gamma; ph.R = p->R; ph.Pr = p->Pr; ph.Pr_visc = p->Pr_viscous; ph.mu0 = p->mu0;
     ph.sutherland_C1 = p->mu0 / std::pow(p->T0, 1.5) * (p->T0 + 110.4);  // ViscousFluxGPU.h:26-28
     ph.CFL = p->CFL; ph.wall_T = p->wall_temperature; ph.venkat_K = p->venkat_K;
+    ph.g_ga1 = p->gamma / (p->gamma - 1.0); ph.inv_g = 1.0 / p->gamma;
     ph.dsr_x = p->dsr_shock_x; ph.dsr_y = p->dsr_shock_y; ph.dsr_speed = p->dsr_shock_speed;
     for (int k = 0; k < 4; k++) { ph.inf[k] = p->inf_ruvp[k]; ph.inlet[k] = p->inlet_ruvp[k]; ph.outlet[k] = p->outlet_ruvp[k]; }
     ph.equation = p->equation; ph.scheme = p->flux_scheme;
@@ -1132,7 +1184,7 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
         }
     for (int e = 0; e < nf; e++) if (enew[e] < 0) { enew[e] = (int)eperm.size(); eperm.push_back(e); }
     // connectivity in the new numbering
-    std::vector<int> cface(3 * (size_t)nc), cnode(3 * (size_t)nc), epair(nf), etype(nf), dtedge((size_t)kMaxDtSlots * nc, -1);
+    std::vector<int> cface(3 * (size_t)nc), cnode(3 * (size_t)nc), epair(nf), etype(nf);
     std::vector<int2> eLR(nf);
     for (int i = 0; i < nc; i++) {
         const int r = h->perm[i];
@@ -1149,19 +1201,28 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
         const int s = m->edge_setID[r];
         etype[e] = (s == -1) ? -1 : m->boundary_type[s - 1];
     }
-    // alphaC contributions in the reference edge order (CalculateDt.cpp:22-63)
+    // alphaC contributions in the reference edge order (CalculateDt.cpp:22-63), encoded per
+    // cell as count (3 bits) + 6 x (face slot 2 bits, pair-edge flag 1 bit)
+    std::vector<int> dtcode(nc, 0);
     {
         std::vector<int> cnt(nc, 0);
         for (int r = 0; r < nf; r++) {
-            const int L = m->edge_elementL[r], R = m->edge_elementR[r];
-            const int cells[2] = {L, R};
+            const int cells[2] = {m->edge_elementL[r], m->edge_elementR[r]};
             for (int q = 0; q < 2; q++) {
                 if (cells[q] < 0) continue;
-                const int ci = h->iperm[cells[q]];
+                const int ref = cells[q], ci = h->iperm[ref];
+                int ent = -1;
+                for (int j = 0; j < 3 && ent < 0; j++)
+                    if (m->element_edges[j][ref] == r) ent = j;
+                for (int j = 0; j < 3 && ent < 0; j++)
+                    if (m->edge_periodicPair[m->element_edges[j][ref]] == r) ent = 4 | j;
+                if (ent < 0) return bail(set_error(FVMR_ERR_MESH, "edge owner does not list the edge"));
                 if (cnt[ci] >= kMaxDtSlots) return bail(set_error(FVMR_ERR_MESH, "too many edges touch one cell"));
-                dtedge[(size_t)cnt[ci]++ * nc + ci] = enew[r];
+                dtcode[ci] |= ent << (3 + 3 * cnt[ci]);
+                cnt[ci]++;
             }
         }
+        for (int i = 0; i < nc; i++) dtcode[i] |= cnt[i];
     }
     // device memory
     const size_t n4 = 4 * (size_t)nc;
@@ -1175,7 +1236,7 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
         return bail(set_error(FVMR_ERR_HIP, "hipMalloc failed"));
     if ((rc = upload(&h->cface, cface, h->stream)) || (rc = upload(&h->cnode, cnode, h->stream)) ||
         (rc = upload(&h->epair, epair, h->stream)) || (rc = upload(&h->etype, etype, h->stream)) ||
-        (rc = upload(&h->dtedge, dtedge, h->stream)) || (rc = upload(&h->eLR, eLR, h->stream)))
+        (rc = upload(&h->dtcode, dtcode, h->stream)) || (rc = upload(&h->eLR, eLR, h->stream)))
         return bail(rc);
     rc = h->mixed ? build_geometry<float>(h, m, eperm) : build_geometry<double>(h, m, eperm);
     if (rc) return bail(rc);
@@ -1286,9 +1347,9 @@ int fvmr_stage_rhs(fvmr_handle *h, double *const rhs[4]) {
     const int a = h->cur, b = (a + 1) % 3;
     int rc;
     if (h->mixed) {
-        if ((rc = launch_gradient<float, float>(h, h->Ubuf[a])) || (rc = launch_stage<float, float>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs))) return rc;
+        if ((rc = launch_gradient<float, float>(h, h->Ubuf[a], -1)) || (rc = launch_stage<float, float>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs))) return rc;
     } else {
-        if ((rc = launch_gradient<double, double>(h, h->Ubuf[a])) || (rc = launch_stage<double, double>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs))) return rc;
+        if ((rc = launch_gradient<double, double>(h, h->Ubuf[a], -1)) || (rc = launch_stage<double, double>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs))) return rc;
     }
     HIP_OK(hipMemcpyAsync(h->sc, &s, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
     h->rhs_valid = true;
@@ -1298,7 +1359,7 @@ int fvmr_stage_rhs(fvmr_handle *h, double *const rhs[4]) {
 
 int fvmr_gradient(fvmr_handle *h, double *const Ux[4], double *const Uy[4]) {
     if (!h || !Ux || !Uy) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
-    int rc = h->mixed ? launch_gradient<float, float>(h, h->Ubuf[h->cur]) : launch_gradient<double, double>(h, h->Ubuf[h->cur]);
+    int rc = h->mixed ? launch_gradient<float, float>(h, h->Ubuf[h->cur], -1) : launch_gradient<double, double>(h, h->Ubuf[h->cur], -1);
     if (rc) return rc;
     h->grad_valid = true;
     if (!h->mixed) {
@@ -1320,7 +1381,8 @@ int fvmr_gradient(fvmr_handle *h, double *const Ux[4], double *const Uy[4]) {
 
 int fvmr_global_dt(fvmr_handle *h, double *dt) {
     if (!h || !dt) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
-    int rc = h->mixed ? launch_dt<float>(h, h->Ubuf[h->cur], 0) : launch_dt<double>(h, h->Ubuf[h->cur], 0);
+    int rc = h->mixed ? launch_gradient<float, float>(h, h->Ubuf[h->cur], 0) : launch_gradient<double, double>(h, h->Ubuf[h->cur], 0);
+    h->grad_valid = true;
     if (rc) return rc;
     Scalars s;
     HIP_OK(hipMemcpyAsync(&s, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
