@@ -1,18 +1,27 @@
 // Device path of the FVMR explicit residual-and-update step for MI355X (gfx950).
 //
 // One step (CIteration::iteration_device, FVMR/src/time/CIteration.cpp:89-161):
-//   dt kernel      per cell: spectral radius summed over the edges touching the cell in
-//                  the reference edge order, dt_c = CFL V / alpha; block min, last block
-//                  finishes the min, clamps to T - t and advances t on the device.
-//   grad kernel    per cell: Green-Gauss or least-squares gradient, Barth or
-//                  Venkatakrishnan limiter; writes the limited Ux, Uy.
-//   stage kernel   per cell: for each of its faces the canonical edge flux (linear
-//                  reconstruction, BC ghost state, Roe/LLF, viscous part) is evaluated
-//                  and summed with the reference's sign and slot order (cell-owned
-//                  gather, no atomics), divided by -V and folded into the RK update.
-//   restrict       single work-group, only when the stage kernel flagged cells:
-//                  repaired values from the neighbours (Jacobi reads), then written.
-// The state lives in three SoA buffers (U^n, stage in, stage out) that rotate.
+//   k_gradient<DT>  per cell, 3 wave64 x 64 cells: wave j gathers the neighbour, geometry
+//                   and vertex of face slot j; wave 0 forms the Green-Gauss or least-squares
+//                   gradient and the Barth/Venkatakrishnan limiter.  In the first stage of a
+//                   step it also sums the spectral radius alphaC in the reference edge order
+//                   and writes the block minimum of CFL V / alphaC.
+//   k_dt_finish     one work-group: global minimum, clamp to T - t, t advanced on the device.
+//   k_stage         per cell, 3 wave64 x 64 cells: wave j evaluates the canonical flux of
+//                   face slot j (reconstruction, BC ghost state, Roe/LLF, viscous part); wave 0
+//                   sums the signed fluxes in slot order (cell-owned gather, no atomics),
+//                   divides by -V and applies the RK update.
+//   k_restrict      one work-group: cells flagged by k_stage are repaired from their
+//                   neighbours (Jacobi reads), then written.
+//
+// Device layout: cells and edges in device order (identity or reverse Cuthill-McKee);
+// gathered data is packed per cell / per edge so that a neighbour costs 1-2 cache lines:
+//   state     {rho, rho u, rho v, rho E}       32 B per cell, three rotating buffers
+//   gradient  {Ux[4], Uy[4]}                   8 G per cell
+//   cell geo  {cx, cy, volume, 0}              4 G per cell
+//   edge geo  {nx, ny, length, 0}, {mx, my}    6 G per edge
+//   edge int  {L, R, BC type (-1 interior), periodic pair (-1)}
+// with G = double (plain FP64) or float (mixed precision).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,9 +39,11 @@ using fvmr::set_error;
 
 namespace {
 
-constexpr int kBlock = 256;      // 4 wave64 per work-group
+constexpr int kBlock = 256;      // work-group of the 1-thread-per-cell kernels
 constexpr int kMaxDtSlots = 6;   // own 3 edges + pair edges of periodic faces
 constexpr int kNumProf = 5;
+constexpr int kCellsPerGroup = 64;              // cells of one cell-kernel work-group
+constexpr int kGroup3 = 3 * kCellsPerGroup;     // one wave64 per face slot
 
 #define HIP_OK(expr)                                                                   \
     do {                                                                               \
@@ -41,22 +52,25 @@ constexpr int kNumProf = 5;
             return set_error(FVMR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+template <class T> struct alignas(4 * sizeof(T)) V4 { T x, y, z, w; };
+template <class T> struct alignas(2 * sizeof(T)) V2 { T x, y; };
+template <class T> struct alignas(4 * sizeof(T)) Grad { T x[4]; T y[4]; };
+using S4 = V4<double>;
+
 // ---------------------------------------------------------------------------
 // Device-side arguments (plain pointers, passed by value).
 // ---------------------------------------------------------------------------
 template <class G>
 struct Mesh {
     int nc, nf;
-    const G *cx, *cy, *vol;          // centroid, area
-    const int *cface;                // [3][nc] edge of local slot j
-    const int *cnode;                // [3][nc]
-    const G *nx_, *ny_;              // node coordinates
-    const int2 *eLR;                 // L, R (R = pair's L for periodic, -1 boundary)
-    const G *enx, *eny, *elen;       // normal (L->R) and length
-    const G *emx, *emy;              // midpoint
-    const int *epair;                // periodic pair edge or -1
-    const int *etype;                // BC code of the edge's set, -1 interior
-    const int *dtcode;               // per cell: alphaC contributions in reference edge order
+    const V4<G> *cgeo;     // cx, cy, volume, 0
+    const int *cface;      // [3][nc] edge of local slot j
+    const int *cnode;      // [3][nc] vertex j
+    const V2<G> *nxy;      // node coordinates
+    const V4<G> *egeo;     // nx, ny, length, 0 (normal L -> R)
+    const V2<G> *emid;     // midpoint
+    const int4 *eint;      // L, R (periodic: pair's L, boundary: -1), BC type, periodic pair
+    const int *dtcode;     // per cell: alphaC contributions in reference edge order
 };
 
 struct Phys {
@@ -70,8 +84,8 @@ struct Phys {
 // Device scalars of a handle: t, dt, T, t at the start of the step.
 struct Scalars {
     double t, dt, T, t_rk;
-    unsigned int counter;       // last-block-done counter of the dt reduction
-    unsigned int norm_counter;  // same for the residual norms
+    unsigned int counter;
+    unsigned int spare;
     int bad_count;              // cells flagged by the stage kernel
     int n_restricted;           // cells repaired in the current step
 };
@@ -87,14 +101,17 @@ __device__ __forceinline__ int xcd_block() {
 template <class T> __device__ __forceinline__ T tmin(T a, T b) { return (a < b) ? a : b; }
 template <class T> __device__ __forceinline__ T tmax(T a, T b) { return (a > b) ? a : b; }
 
-// PhysicalKernelGPU.h:11-22
-template <class T>
-__device__ __forceinline__ void u2ruvp(const T U[4], T r[4], T gamma) {
-    r[0] = U[0];
-    r[1] = U[1] / U[0];
-    r[2] = U[2] / U[0];
-    r[3] = r[0] * (gamma - 1) * (U[3] / U[0] - (r[1] * r[1] + r[2] * r[2]) * T(0.5));
+__device__ __forceinline__ void ld(const S4 *p, int i, double u[4]) {
+    const S4 v = p[i];
+    u[0] = v.x; u[1] = v.y; u[2] = v.z; u[3] = v.w;
 }
+__device__ __forceinline__ void st(S4 *p, int i, const double u[4]) {
+    S4 v;
+    v.x = u[0]; v.y = u[1]; v.z = u[2]; v.w = u[3];
+    p[i] = v;
+}
+
+// ruvp2U (PhysicalKernelGPU.h:17-22)
 template <class T>
 __device__ __forceinline__ void ruvp2u(const T r[4], T U[4], T gamma) {
     U[0] = r[0];
@@ -102,17 +119,27 @@ __device__ __forceinline__ void ruvp2u(const T r[4], T U[4], T gamma) {
     U[2] = r[0] * r[2];
     U[3] = r[3] / (gamma - 1) + T(0.5) * r[0] * (r[1] * r[1] + r[2] * r[2]);
 }
-// RestrictGPU.h:11-19
+// outOfRange_device (RestrictGPU.h:11-19)
 template <class T>
-__device__ __forceinline__ bool out_of_range(const T r[4]) {
-    return r[0] < T(fvmr::kRHO_MIN) || r[0] > T(fvmr::kRHO_MAX) || r[3] < T(fvmr::kP_MIN) || r[3] > T(fvmr::kP_MAX);
+__device__ __forceinline__ bool out_of_range(T rho, T p) {
+    return rho < T(fvmr::kRHO_MIN) || rho > T(fvmr::kRHO_MAX) || p < T(fvmr::kP_MIN) || p > T(fvmr::kP_MAX);
 }
+// U2ruvp (PhysicalKernelGPU.h:11-16) with the divisions by U[0] folded into one
+// reciprocal (results differ from the reference by rounding only)
+template <class T>
+__device__ __forceinline__ void prim(const T U[4], T r[4], T gamma) {
+    const T ir = T(1) / U[0];
+    r[0] = U[0];
+    r[1] = U[1] * ir;
+    r[2] = U[2] * ir;
+    r[3] = r[0] * (gamma - 1) * (U[3] * ir - (r[1] * r[1] + r[2] * r[2]) * T(0.5));
+}
+__device__ __forceinline__ bool is_periodic(int t) { return t >= FVMR_BC_PERIODIC_0 && t <= FVMR_BC_PERIODIC_0 + 9; }
+
 // ---------------------------------------------------------------------------
 // Face physics (FluxGPU.cu:34-205, RoeGPU.cu, LocalLaxFriedrichsGPU.cu)
 // ---------------------------------------------------------------------------
-// A reconstructed or ghost state: conserved U and its primitives, computed once with one
-// reciprocal of rho (U2ruvp, PhysicalKernelGPU.h:11-16, with the divisions by U[0]
-// folded into that reciprocal; results differ from the reference by rounding only).
+// A reconstructed or ghost state: conserved U and its primitives, computed once.
 template <class C>
 struct FaceState {
     C U[4];
@@ -128,30 +155,29 @@ __device__ __forceinline__ void set_prim(FaceState<C> &s, C gamma) {
     s.p = s.r * (gamma - 1) * (s.U[3] * s.ir - (s.u * s.u + s.v * s.v) * C(0.5));
 }
 
-__device__ __forceinline__ bool is_periodic(int t) { return t >= FVMR_BC_PERIODIC_0 && t <= FVMR_BC_PERIODIC_0 + 9; }
-
 // get_Uvector_reconstruct_2_device (FluxGPU.cu:47-61): linear (3002) or constant
 // reconstruction; back to the cell value when rho or p leave their range.
 template <class C, class G>
-__device__ __forceinline__ void reconstruct(const Mesh<G> &m, const Phys &ph, const double *__restrict__ U,
-                                            const G *__restrict__ Ux, const G *__restrict__ Uy, int c, C x, C y,
-                                            FaceState<C> &s) {
-    const int nc = m.nc;
+__device__ __forceinline__ void reconstruct(const Mesh<G> &m, const Phys &ph, const S4 *__restrict__ U,
+                                            const Grad<G> *__restrict__ grad, int c, C x, C y, FaceState<C> &s) {
     const C gamma = (C)ph.gamma;
+    double Ud[4];
+    ld(U, c, Ud);
     C Uc[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) Uc[i] = (C)U[i * nc + c];
+    for (int i = 0; i < 4; i++) Uc[i] = (C)Ud[i];
     if (ph.reconstruct == FVMR_REC_LINEAR) {
-        const C dx = x - (C)m.cx[c], dy = y - (C)m.cy[c];
+        const V4<G> g = m.cgeo[c];
+        const Grad<G> gr = grad[c];
+        const C dx = x - (C)g.x, dy = y - (C)g.y;
 #pragma unroll
-        for (int i = 0; i < 4; i++) s.U[i] = Uc[i] + (C)Ux[i * nc + c] * dx + (C)Uy[i * nc + c] * dy;
+        for (int i = 0; i < 4; i++) s.U[i] = Uc[i] + (C)gr.x[i] * dx + (C)gr.y[i] * dy;
     } else {
 #pragma unroll
         for (int i = 0; i < 4; i++) s.U[i] = Uc[i];
     }
     set_prim(s, gamma);
-    const C rp[4] = {s.r, s.u, s.v, s.p};
-    if (out_of_range(rp)) {
+    if (out_of_range(s.r, s.p)) {
 #pragma unroll
         for (int i = 0; i < 4; i++) s.U[i] = Uc[i];
         set_prim(s, gamma);
@@ -161,9 +187,8 @@ __device__ __forceinline__ void reconstruct(const Mesh<G> &m, const Phys &ph, co
 // get_UR_wallNonViscous_device (FluxGPU.cu:63-77)
 template <class C>
 __device__ __forceinline__ void ghost_slip(const FaceState<C> &L, FaceState<C> &R, C nx, C ny, C gamma) {
-    const C uxL = L.u, uyL = L.v;
-    const C unL = uxL * nx + uyL * ny;
-    const C utL = uxL * ny - uyL * nx;
+    const C unL = L.u * nx + L.v * ny;
+    const C utL = L.u * ny - L.v * nx;
     const C unR = -unL, utR = utL;
     const C uxR = unR * nx + utR * ny;
     const C uyR = unR * ny - utR * nx;
@@ -177,7 +202,7 @@ __device__ __forceinline__ void ghost_slip(const FaceState<C> &L, FaceState<C> &
 // get_UR_farField_device (FluxGPU.cu:87-167): Riemann-invariant ghost in four regimes
 template <class C>
 __device__ __forceinline__ void ghost_farfield(const FaceState<C> &L, const double *ruvp_inf, FaceState<C> &R, C nx,
-                                            C ny, C gamma) {
+                                               C ny, C gamma) {
     const C ga1 = gamma - 1;
     const C rho_f = (C)ruvp_inf[0], u_f = (C)ruvp_inf[1], v_f = (C)ruvp_inf[2], p_f = (C)ruvp_inf[3];
     const C rho_e = L.r, u_e = L.u, v_e = L.v, p_e = L.p;
@@ -193,15 +218,14 @@ __device__ __forceinline__ void ghost_farfield(const FaceState<C> &L, const doub
     C r0[4];
     if (qn <= -as) {
         r0[0] = rho_f; r0[1] = u_f; r0[2] = v_f; r0[3] = p_f;
-    } else if (qn < 0) {
-        const C son = p_f / pow(rho_f, gamma);
-        const C qtx = u_f - qnf * nx, qty = v_f - qnf * ny;
-        r0[0] = pow((as * as / son / gamma), C(1.0) / ga1);
-        r0[1] = qtx + qn * nx; r0[2] = qty + qn * ny;
-        r0[3] = as * as * r0[0] / gamma;
     } else if (qn < as) {
-        const C son = p_e / pow(rho_e, gamma);
-        const C qtx = u_e - qne * nx, qty = v_e - qne * ny;
+        // subsonic inflow (qn < 0) takes entropy and tangential velocity from the freestream,
+        // subsonic outflow from the interior
+        const bool in = qn < 0;
+        const C s_p = in ? p_f : p_e, s_r = in ? rho_f : rho_e;
+        const C s_u = in ? u_f : u_e, s_v = in ? v_f : v_e, s_qn = in ? qnf : qne;
+        const C son = s_p / pow(s_r, gamma);
+        const C qtx = s_u - s_qn * nx, qty = s_v - s_qn * ny;
         r0[0] = pow((as * as / son / gamma), C(1.0) / ga1);
         r0[1] = qtx + qn * nx; r0[2] = qty + qn * ny;
         r0[3] = as * as * r0[0] / gamma;
@@ -241,7 +265,7 @@ __device__ __forceinline__ void flux_llf(const FaceState<C> &L, const FaceState<
 }
 
 // ConvectRoeCommon2d + RoeDissapationTerm2d (RoeGPU.cu:4-159): no entropy fix, the
-// reference energy jump Etm; divisions folded into five reciprocals
+// reference energy jump Etm; divisions folded into reciprocals
 template <class C>
 __device__ __forceinline__ void flux_roe(const FaceState<C> &L, const FaceState<C> &R, C nx, C ny, C area, C F[4],
                                          C gamma, C g_ga1, C inv_g) {
@@ -297,25 +321,36 @@ __device__ __forceinline__ double mu_of_T(const Phys &ph, double T) {
 
 // viscous_flux_device_kernel body (ViscousFluxGPU.cu:139-229); returns F_v (to be subtracted)
 template <class C, class G>
-__device__ __forceinline__ void flux_viscous(const Mesh<G> &m, const Phys &ph, const double *__restrict__ U,
-                                             const G *__restrict__ Ux, const G *__restrict__ Uy, int e, int L,
-                                             int R, C nx, C ny, C area, C F[4]) {
+__device__ __forceinline__ void flux_viscous(const Mesh<G> &m, const Phys &ph, const S4 *__restrict__ U,
+                                             const Grad<G> *__restrict__ grad, int L, int R, C nx, C ny, C area,
+                                             C F[4]) {
     const int nc = m.nc;
     const C ga1 = (C)ph.gamma - C(1.0);
     const C Cv = (C)ph.R / ga1;
     const C Cp = Cv * (C)ph.gamma;
     const bool hasR = !(R < 0 || R >= nc);
     C gx[4], gy[4], Um[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
+    {
+        const Grad<G> gL = grad[L];
+        double uL[4];
+        ld(U, L, uL);
         if (hasR) {
-            gx[k] = C(0.5) * ((C)Ux[k * nc + L] + (C)Ux[k * nc + R]);
-            gy[k] = C(0.5) * ((C)Uy[k * nc + L] + (C)Uy[k * nc + R]);
-            Um[k] = C(0.5) * ((C)U[k * nc + L] + (C)U[k * nc + R]);
+            const Grad<G> gR = grad[R];
+            double uR[4];
+            ld(U, R, uR);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                gx[k] = C(0.5) * ((C)gL.x[k] + (C)gR.x[k]);
+                gy[k] = C(0.5) * ((C)gL.y[k] + (C)gR.y[k]);
+                Um[k] = C(0.5) * ((C)uL[k] + (C)uR[k]);
+            }
         } else {
-            gx[k] = (C)Ux[k * nc + L];
-            gy[k] = (C)Uy[k * nc + L];
-            Um[k] = (C)U[k * nc + L];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                gx[k] = (C)gL.x[k];
+                gy[k] = (C)gL.y[k];
+                Um[k] = (C)uL[k];
+            }
         }
     }
     const C one_on_rho = C(1.0) / Um[0];
@@ -349,23 +384,27 @@ __device__ __forceinline__ void flux_viscous(const Mesh<G> &m, const Phys &ph, c
 // edge_convection_flux_device_kernel body (FluxGPU.cu:207-283) (+ viscous part for NS):
 // the canonical flux of edge e, oriented L -> R.
 template <class C, class G, bool NS>
-__device__ __forceinline__ void edge_flux(const Mesh<G> &m, const Phys &ph, const double *__restrict__ U,
-                                          const G *__restrict__ Ux, const G *__restrict__ Uy, double t_rk, int e,
+__device__ __forceinline__ void edge_flux(const Mesh<G> &m, const Phys &ph, const S4 *__restrict__ U,
+                                          const Grad<G> *__restrict__ grad, double t_rk, int e, const int4 ei,
                                           C F[4]) {
-    const int bType = m.etype[e];
-    const C nx = (C)m.enx[e], ny = (C)m.eny[e], len = (C)m.elen[e];
-    const C xe = (C)m.emx[e], ye = (C)m.emy[e];
-    const int2 lr = m.eLR[e];
+    const V4<G> eg = m.egeo[e];
+    const V2<G> em = m.emid[e];
+    const int bType = ei.z;
+    const C nx = (C)eg.x, ny = (C)eg.y, len = (C)eg.z;
+    const C xe = (C)em.x, ye = (C)em.y;
     const C gamma = (C)ph.gamma;
     FaceState<C> L, R;
-    reconstruct<C, G>(m, ph, U, Ux, Uy, lr.x, xe, ye, L);
+    reconstruct<C, G>(m, ph, U, grad, ei.x, xe, ye, L);
     if (bType == -1 || is_periodic(bType)) {
         // interior and periodic edges (get_UR_inner_and_periodic_device, FluxGPU.cu:169-188)
-        if (lr.y != -1) {
+        if (ei.y != -1) {
             C xr = xe, yr = ye;
-            const int pe = m.epair[e];
-            if (bType != -1 && pe >= 0 && pe < m.nf) { xr = (C)m.emx[pe]; yr = (C)m.emy[pe]; }
-            reconstruct<C, G>(m, ph, U, Ux, Uy, lr.y, xr, yr, R);
+            if (bType != -1 && ei.w >= 0 && ei.w < m.nf) {
+                const V2<G> pm = m.emid[ei.w];
+                xr = (C)pm.x;
+                yr = (C)pm.y;
+            }
+            reconstruct<C, G>(m, ph, U, grad, ei.y, xr, yr, R);
         } else {
 #pragma unroll
             for (int i = 0; i < 4; i++) R.U[i] = C(0);
@@ -384,8 +423,8 @@ __device__ __forceinline__ void edge_flux(const Mesh<G> &m, const Phys &ph, cons
         if (bType == FVMR_BC_INLET) ruvp = ph.inlet;
         else if (bType == FVMR_BC_OUTLET) ruvp = ph.outlet;
         else if (bType == FVMR_BC_DSR) {
-            const double right = ph.dsr_x + ((double)m.emy[e] + ph.dsr_speed * t_rk) / sqrt(3.0);
-            ruvp = ((double)m.emx[e] < right) ? ph.inlet : ph.outlet;
+            const double right = ph.dsr_x + ((double)em.y + ph.dsr_speed * t_rk) / sqrt(3.0);
+            ruvp = ((double)em.x < right) ? ph.inlet : ph.outlet;
         }
         ghost_farfield(L, ruvp, R, nx, ny, gamma);
     }
@@ -394,7 +433,7 @@ __device__ __forceinline__ void edge_flux(const Mesh<G> &m, const Phys &ph, cons
     else flux_roe(L, R, nx, ny, len, F, gamma, g_ga1, (C)ph.inv_g);
     if (NS) {
         C Fv[4];
-        flux_viscous<C, G>(m, ph, U, Ux, Uy, e, lr.x, lr.y, nx, ny, len, Fv);
+        flux_viscous<C, G>(m, ph, U, grad, ei.x, ei.y, nx, ny, len, Fv);
 #pragma unroll
         for (int k = 0; k < 4; k++) F[k] -= Fv[k];
     }
@@ -419,12 +458,11 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// Spectral radius of one edge for the cells a and b (b = -1: boundary, one-sided)
+// Spectral radius of one edge for the cells a and b (has_b false: boundary, one-sided)
 // (update_alphaC_host, CalculateDt.cpp:24-62).  ra/rb: primitives of a and b.  Returns the
 // inviscid part in *inv and the viscous part divided by V_c in *vis (0 for Euler).
-template <class G>
-__device__ __forceinline__ void edge_alpha(const Mesh<G> &m, const Phys &ph, const double ra[4], const double rb[4],
-                                           bool has_b, int e, double vol_c, double *inv, double *vis) {
+__device__ __forceinline__ void edge_alpha(const Phys &ph, const double ra[4], const double rb[4], bool has_b,
+                                           double nx, double ny, double len, double vol_c, double *inv, double *vis) {
     const double gamma = ph.gamma;
     double rho = ra[0], u = ra[1], v = ra[2], p = ra[3];
     if (has_b) {
@@ -433,8 +471,7 @@ __device__ __forceinline__ void edge_alpha(const Mesh<G> &m, const Phys &ph, con
         v = 0.5 * (ra[2] + rb[2]);
         p = 0.5 * (ra[3] + rb[3]);
     }
-    const double len = (double)m.elen[e];
-    const double dx = (double)m.enx[e] * len, dy = (double)m.eny[e] * len;
+    const double dx = nx * len, dy = ny * len;
     *inv = fabs(u * dx + v * dy) + sqrt(gamma * p / rho) * sqrt(len);
     *vis = 0.0;
     if (ph.equation == FVMR_EQ_NS) {
@@ -444,20 +481,6 @@ __device__ __forceinline__ void edge_alpha(const Mesh<G> &m, const Phys &ph, con
     }
 }
 
-// primitives (rho, u, v, p) of a conserved state with one reciprocal of rho
-__device__ __forceinline__ void prim4(const double U[4], double r[4], double gamma) {
-    const double ir = 1.0 / U[0];
-    r[0] = U[0];
-    r[1] = U[1] * ir;
-    r[2] = U[2] * ir;
-    r[3] = r[0] * (gamma - 1) * (U[3] * ir - (r[1] * r[1] + r[2] * r[2]) * 0.5);
-}
-
-// Work-group layout of the cell kernels: 3 wave64 x 64 cells; wave j handles local face
-// slot j (and vertex j) of the 64 cells, results meet in LDS.
-constexpr int kCellsPerGroup = 64;
-constexpr int kGroup3 = 3 * kCellsPerGroup;
-
 // Gradient_2 (GradientGPU.cu:513-551): GG (GradientGreenGaussKernel :223-425) or LSQ
 // (Gradient.cpp:160-246, correct inverse unless compat), then Barth (:453-502) or
 // Venkatakrishnan.  Waves 0-2 gather face j's neighbour, geometry and vertex j; wave 0
@@ -465,9 +488,8 @@ constexpr int kGroup3 = 3 * kCellsPerGroup;
 // cell from the same loads, summed in the reference edge order, and writes the block
 // minimum of CFL V / alphaC (get_global_dt_device, CalculateDtGPU.cu:97-103).
 template <class C, class G, bool DT>
-__global__ __launch_bounds__(kGroup3) void k_gradient(Mesh<G> m, Phys ph, const double *__restrict__ U,
-                                                      G *__restrict__ Ux, G *__restrict__ Uy,
-                                                      double *__restrict__ partial) {
+__global__ __launch_bounds__(kGroup3) void k_gradient(Mesh<G> m, Phys ph, const S4 *__restrict__ U,
+                                                      Grad<G> *__restrict__ grad, double *__restrict__ partial) {
     __shared__ C sUo[3][4][kCellsPerGroup];   // out-cell state of slot j (own state when none)
     __shared__ C sA[3][3][kCellsPerGroup];    // GG: gC, Sx*s, Sy*s;  LSQ: dx, dy
     __shared__ C sN[3][2][kCellsPerGroup];    // vertex j relative to the centroid
@@ -479,29 +501,33 @@ __global__ __launch_bounds__(kGroup3) void k_gradient(Mesh<G> m, Phys ph, const 
     const int nc = m.nc;
     const bool live = c < nc;
     if (live) {
-        const C xc = (C)m.cx[c], yc = (C)m.cy[c];
+        const V4<G> cg = m.cgeo[c];
+        const C xc = (C)cg.x, yc = (C)cg.y;
         const int e = m.cface[j * nc + c];
-        const int2 lr = m.eLR[e];
-        const int pe = m.epair[e];
-        const int out = (c != lr.y) ? lr.y : lr.x;
-        const C sgn = (c != lr.y) ? C(1) : C(-1);
+        const int4 ei = m.eint[e];
+        const int pe = ei.w;
+        const int out = (c != ei.y) ? ei.y : ei.x;
+        const C sgn = (c != ei.y) ? C(1) : C(-1);
         double Uo[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) Uo[k] = U[k * nc + (out != -1 ? out : c)];
+        ld(U, out != -1 ? out : c, Uo);
 #pragma unroll
         for (int k = 0; k < 4; k++) sUo[j][k][lane] = (C)Uo[k];
-        const int nd = m.cnode[j * nc + c];
-        sN[j][0][lane] = (C)m.nx_[nd] - xc;
-        sN[j][1][lane] = (C)m.ny_[nd] - yc;
+        const V2<G> nd = m.nxy[m.cnode[j * nc + c]];
+        sN[j][0][lane] = (C)nd.x - xc;
+        sN[j][1][lane] = (C)nd.y - yc;
+        const V4<G> eg = m.egeo[e];
         if (ph.gradient == FVMR_GRA_GREENGAUSS) {
-            const C len = (C)m.elen[e];
+            const C len = (C)eg.z;
             C gC = C(0.5);
             if (out != -1) {
-                const C fx = (C)m.emx[e], fy = (C)m.emy[e];
-                C Fx = (C)m.cx[out], Fy = (C)m.cy[out];
-                if (m.etype[e] != -1 && pe >= 0) {
-                    Fx += fx - (C)m.emx[pe];
-                    Fy += fy - (C)m.emy[pe];
+                const V2<G> em = m.emid[e];
+                const V4<G> og = m.cgeo[out];
+                const C fx = (C)em.x, fy = (C)em.y;
+                C Fx = (C)og.x, Fy = (C)og.y;
+                if (ei.z != -1 && pe >= 0) {
+                    const V2<G> pm = m.emid[pe];
+                    Fx += fx - (C)pm.x;
+                    Fy += fy - (C)pm.y;
                 }
                 const C ax = fx - Fx, ay = fy - Fy, bx = xc - Fx, by = yc - Fy;
                 const C dFf = sqrt(ax * ax + ay * ay);
@@ -510,24 +536,32 @@ __global__ __launch_bounds__(kGroup3) void k_gradient(Mesh<G> m, Phys ph, const 
                 gC = dFf / dFC;
             }
             sA[j][0][lane] = gC;
-            sA[j][1][lane] = (C)m.enx[e] * len * sgn;
-            sA[j][2][lane] = (C)m.eny[e] * len * sgn;
+            sA[j][1][lane] = (C)eg.x * len * sgn;
+            sA[j][2][lane] = (C)eg.y * len * sgn;
         } else {
             const int nbr = (pe >= 0) ? -1 : out;   // element neighbours exclude periodic faces
-            sA[j][0][lane] = (nbr != -1) ? (C)m.cx[nbr] - xc : C(0);
-            sA[j][1][lane] = (nbr != -1) ? (C)m.cy[nbr] - yc : C(0);
+            C dxn = C(0), dyn = C(0);
+            if (nbr != -1) {
+                const V4<G> og = m.cgeo[nbr];
+                dxn = (C)og.x - xc;
+                dyn = (C)og.y - yc;
+            }
+            sA[j][0][lane] = dxn;
+            sA[j][1][lane] = dyn;
         }
         sFl[j][lane] = ((out != -1) ? 1 : 0) | ((out != -1 && pe < 0) ? 2 : 0);
         if (DT) {
             double Uc[4], rc[4], ro[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) Uc[k] = U[k * nc + c];
-            prim4(Uc, rc, ph.gamma);
-            prim4(Uo, ro, ph.gamma);
-            const double vol_c = (double)m.vol[c];
+            ld(U, c, Uc);
+            prim(Uc, rc, ph.gamma);
+            prim(Uo, ro, ph.gamma);
+            const double vol_c = (double)cg.z;
             double a0, v0, a1 = 0.0, v1 = 0.0;
-            edge_alpha(m, ph, rc, ro, out != -1, e, vol_c, &a0, &v0);
-            if (pe >= 0) edge_alpha(m, ph, rc, ro, true, pe, vol_c, &a1, &v1);
+            edge_alpha(ph, rc, ro, out != -1, (double)eg.x, (double)eg.y, (double)eg.z, vol_c, &a0, &v0);
+            if (pe >= 0) {
+                const V4<G> pg = m.egeo[pe];
+                edge_alpha(ph, rc, ro, true, (double)pg.x, (double)pg.y, (double)pg.z, vol_c, &a1, &v1);
+            }
             sAl[j][0][lane] = a0;
             sAl[j][1][lane] = v0;
             sAl[j][2][lane] = a1;
@@ -538,15 +572,18 @@ __global__ __launch_bounds__(kGroup3) void k_gradient(Mesh<G> m, Phys ph, const 
     if (j != 0) return;
     double dtc = 1e10;
     if (live) {
+        double Ud[4];
+        ld(U, c, Ud);
         C Uc[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) Uc[k] = (C)U[k * nc + c];
+        for (int k = 0; k < 4; k++) Uc[k] = (C)Ud[k];
+        const V4<G> cg = m.cgeo[c];
         int fl[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) fl[i] = sFl[i][lane];
         C gx[4], gy[4];
         if (ph.gradient == FVMR_GRA_GREENGAUSS) {
-            const C inv_vol = C(1) / (C)m.vol[c];
+            const C inv_vol = C(1) / (C)cg.z;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 C s0 = C(0), s1 = C(0);
@@ -600,7 +637,7 @@ __global__ __launch_bounds__(kGroup3) void k_gradient(Mesh<G> m, Phys ph, const 
         if (ph.limiter != FVMR_LIM_NONE) {
             C eps2 = C(0);
             if (ph.limiter == FVMR_LIM_VENKATAKRISHNAN) {
-                const C Kh = (C)ph.venkat_K * sqrt((C)m.vol[c]);
+                const C Kh = (C)ph.venkat_K * sqrt((C)cg.z);
                 eps2 = Kh * Kh * Kh;
             }
 #pragma unroll
@@ -635,11 +672,13 @@ __global__ __launch_bounds__(kGroup3) void k_gradient(Mesh<G> m, Phys ph, const 
                 gy[k] *= phi;
             }
         }
+        Grad<G> out;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            Ux[k * nc + c] = (G)gx[k];
-            Uy[k * nc + c] = (G)gy[k];
+            out.x[k] = (G)gx[k];
+            out.y[k] = (G)gy[k];
         }
+        grad[c] = out;
         if (DT) {
             // alphaC in the reference edge order: count (3 bits) + 6 x (slot 2 bits, pair 1 bit)
             const int code = m.dtcode[c];
@@ -651,7 +690,7 @@ __global__ __launch_bounds__(kGroup3) void k_gradient(Mesh<G> m, Phys ph, const 
                 alpha_c += sAl[slot][2 * pr][lane];
                 if (ph.equation == FVMR_EQ_NS) alpha_c += sAl[slot][2 * pr + 1][lane];
             }
-            dtc = ph.CFL * (double)m.vol[c] / alpha_c;
+            dtc = ph.CFL * (double)cg.z / alpha_c;
         }
     }
     if (DT) {
@@ -668,11 +707,11 @@ __global__ __launch_bounds__(kGroup3) void k_gradient(Mesh<G> m, Phys ph, const 
 // signed fluxes in slot order and updates.  Cells whose new state is NaN or out of range
 // are appended to the restriction list.
 template <class C, class G, bool NS>
-__global__ __launch_bounds__(kGroup3) void k_stage(Mesh<G> m, Phys ph, const double *__restrict__ Uin,
-                                                   const double *__restrict__ Un, double *__restrict__ Uout,
-                                                   const G *__restrict__ Ux, const G *__restrict__ Uy,
-                                                   double *__restrict__ rhs_out, Scalars *__restrict__ sc,
-                                                   int *__restrict__ bad_list, double a, double b, int first) {
+__global__ __launch_bounds__(kGroup3) void k_stage(Mesh<G> m, Phys ph, const S4 *__restrict__ Uin,
+                                                   const S4 *__restrict__ Un, S4 *__restrict__ Uout,
+                                                   const Grad<G> *__restrict__ grad, S4 *__restrict__ rhs_out,
+                                                   Scalars *__restrict__ sc, int *__restrict__ bad_list, double a,
+                                                   double b, int first) {
     __shared__ double sF[3][4][kCellsPerGroup];
     const int lane = threadIdx.x & 63, j = threadIdx.x >> 6;
     const int c = xcd_block() * kCellsPerGroup + lane;
@@ -680,32 +719,35 @@ __global__ __launch_bounds__(kGroup3) void k_stage(Mesh<G> m, Phys ph, const dou
     if (c < nc) {
         const double t_rk = sc->t_rk;
         const int e = m.cface[j * nc + c];
+        const int4 ei = m.eint[e];
         C F[4];
-        edge_flux<C, G, NS>(m, ph, Uin, Ux, Uy, t_rk, e, F);
-        const double s = (c != m.eLR[e].y) ? 1.0 : -1.0;
+        edge_flux<C, G, NS>(m, ph, Uin, grad, t_rk, e, ei, F);
+        const double s = (c != ei.y) ? 1.0 : -1.0;
 #pragma unroll
         for (int k = 0; k < 4; k++) sF[j][k][lane] = s * (double)F[k];
     }
     __syncthreads();
     if (j != 0 || c >= nc) return;
     const double dt = sc->dt;
-    const double w = -1.0 * (double)m.vol[c];
-    double Unew[4];
+    const double w = -1.0 * (double)m.cgeo[c].z;
+    double Ui[4], Ub[4] = {0, 0, 0, 0}, r[4], Unew[4];
+    ld(Uin, c, Ui);
+    if (!first) ld(Un, c, Ub);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         double acc = 0.0;
         acc += sF[0][k][lane];
         acc += sF[1][k][lane];
         acc += sF[2][k][lane];
-        const double r = acc / w;
-        if (rhs_out) rhs_out[k * nc + c] = r;
-        const double st = Uin[k * nc + c] + dt * r;
-        Unew[k] = first ? st : a * Un[k * nc + c] + b * st;
-        Uout[k * nc + c] = Unew[k];
+        r[k] = acc / w;
+        const double s = Ui[k] + dt * r[k];
+        Unew[k] = first ? s : a * Ub[k] + b * s;
     }
+    if (rhs_out) st(rhs_out, c, r);
+    st(Uout, c, Unew);
     double rv[4];
-    u2ruvp(Unew, rv, ph.gamma);
-    bool bad = out_of_range(rv);
+    prim(Unew, rv, ph.gamma);
+    bool bad = out_of_range(rv[0], rv[3]);
 #pragma unroll
     for (int k = 0; k < 4; k++) bad = bad || isnan(rv[k]);
     if (bad) {
@@ -716,36 +758,36 @@ __global__ __launch_bounds__(kGroup3) void k_stage(Mesh<G> m, Phys ph, const dou
 
 // modifyElementFieldU2d_device_kernel (RestrictGPU.cu:6-84) with Jacobi reads: one work-group,
 // phase 1 computes all repairs from the unrepaired field, phase 2 writes them.
-__global__ __launch_bounds__(1024) void k_restrict(int nc, const int *__restrict__ cface, const int2 *__restrict__ eLR,
-                                                   const int *__restrict__ epair, const double *__restrict__ vol_d,
-                                                   const float *__restrict__ vol_f, double *__restrict__ U,
-                                                   Scalars *__restrict__ sc, const int *__restrict__ bad_list,
-                                                   double *__restrict__ fix) {
+template <class G>
+__global__ __launch_bounds__(1024) void k_restrict(Mesh<G> m, S4 *__restrict__ U, Scalars *__restrict__ sc,
+                                                   const int *__restrict__ bad_list, S4 *__restrict__ fix) {
+    const int nc = m.nc;
     const int count = min(sc->bad_count, nc);
     for (int i = threadIdx.x; i < count; i += blockDim.x) {
         const int c = bad_list[i];
         int nvalid = 0;
         double Usum[4] = {0, 0, 0, 0}, vs = 0.0;
         for (int j = 0; j < 3; j++) {
-            const int e = cface[j * nc + c];
-            const int2 lr = eLR[e];
-            int nb = (c != lr.y) ? lr.y : lr.x;
-            if (epair[e] >= 0) nb = -1;
+            const int4 ei = m.eint[m.cface[j * nc + c]];
+            int nb = (c != ei.y) ? ei.y : ei.x;
+            if (ei.w >= 0) nb = -1;   // periodic neighbours are not element neighbours
             if (nb == -1) continue;
             nvalid++;
-            const double v = vol_d ? vol_d[nb] : (double)vol_f[nb];
+            const double v = (double)m.cgeo[nb].z;
+            double Unb[4];
+            ld(U, nb, Unb);
             for (int k = 0; k < 4; k++) {
-                Usum[k] += U[k * nc + nb] * v;
+                Usum[k] += Unb[k] * v;
                 vs += v;
             }
         }
-        for (int k = 0; k < 4; k++) fix[4 * i + k] = (nvalid == 0) ? U[k * nc + c] : Usum[k] / vs;
+        double res[4];
+        if (nvalid == 0) ld(U, c, res);
+        else for (int k = 0; k < 4; k++) res[k] = Usum[k] / vs;
+        st(fix, i, res);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < count; i += blockDim.x) {
-        const int c = bad_list[i];
-        for (int k = 0; k < 4; k++) U[k * nc + c] = fix[4 * i + k];
-    }
+    for (int i = threadIdx.x; i < count; i += blockDim.x) U[bad_list[i]] = fix[i];
     __syncthreads();
     if (threadIdx.x == 0) {
         sc->n_restricted += count;
@@ -777,44 +819,40 @@ __global__ __launch_bounds__(1024) void k_dt_finish(int nparts, const double *__
     }
 }
 
-// get_residual_host (Residual.cpp:4-43) on the device: 1 = L1, 2 = L2, 3 = Linf, per variable.
-__global__ __launch_bounds__(kBlock) void k_norm(int nc, const double *__restrict__ rhs, int norm,
-                                                 double *__restrict__ partial, double *__restrict__ out,
-                                                 Scalars *__restrict__ sc) {
+// get_residual_host (Residual.cpp:4-43) on the device: 1 = L1, 2 = L2, 3 = Linf, per
+// variable; block partials, then one work-group finishes (k_norm_finish).
+__global__ __launch_bounds__(kBlock) void k_norm(int nc, const S4 *__restrict__ rhs, int norm,
+                                                 double *__restrict__ partial) {
     const int blk = blockIdx.x;
     const int c = blk * kBlock + threadIdx.x;
     __shared__ double s[4][kBlock / 64];
-    __shared__ bool last;
+    double r[4] = {0, 0, 0, 0};
+    if (c < nc) ld(rhs, c, r);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        double f = (c < nc) ? fabs(rhs[k * nc + c]) : 0.0;
-        double v = (norm == FVMR_NORM_INF) ? wave_max(f) : wave_sum(norm == FVMR_NORM_2 ? f * f : f);
+        const double f = fabs(r[k]);
+        const double v = (norm == FVMR_NORM_INF) ? wave_max(f) : wave_sum(norm == FVMR_NORM_2 ? f * f : f);
         if ((threadIdx.x & 63) == 0) s[k][threadIdx.x >> 6] = v;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < 4; k++) {
-            double b = s[k][0];
-            for (int i = 1; i < kBlock / 64; i++) b = (norm == FVMR_NORM_INF) ? fmax(b, s[k][i]) : b + s[k][i];
-            partial[4 * blk + k] = b;
-        }
-        __threadfence();
-        last = (atomicAdd(&sc->norm_counter, 1u) == gridDim.x - 1);
+    if (threadIdx.x < 4) {
+        const int k = threadIdx.x;
+        double b = s[k][0];
+        for (int i = 1; i < kBlock / 64; i++) b = (norm == FVMR_NORM_INF) ? fmax(b, s[k][i]) : b + s[k][i];
+        partial[4 * blk + k] = b;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+}
+__global__ __launch_bounds__(64) void k_norm_finish(int nparts, int norm, const double *__restrict__ partial,
+                                                    double *__restrict__ out) {
     if (threadIdx.x < 4) {
         const int k = threadIdx.x;
         double b = 0.0;
-        for (int i = 0; i < (int)gridDim.x; i++) {
+        for (int i = 0; i < nparts; i++) {
             const double v = partial[4 * i + k];
             b = (norm == FVMR_NORM_INF) ? fmax(b, v) : b + v;
         }
         out[k] = (norm == FVMR_NORM_2) ? sqrt(b) : b;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) sc->norm_counter = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -837,7 +875,6 @@ std::vector<int> rcm_order(int nc, const std::vector<int> &adj_off, const std::v
     std::vector<int> level(nc, -1);
     auto bfs_last = [&](int s, int &ecc) {
         std::vector<int> q{s};
-        std::vector<int> touched{s};
         level[s] = 0;
         size_t h = 0;
         int last = s;
@@ -846,14 +883,14 @@ std::vector<int> rcm_order(int nc, const std::vector<int> &adj_off, const std::v
             last = c;
             for (int k = adj_off[c]; k < adj_off[c + 1]; k++) {
                 int d = adj[k];
-                if (level[d] < 0) { level[d] = level[c] + 1; q.push_back(d); touched.push_back(d); }
+                if (level[d] < 0) { level[d] = level[c] + 1; q.push_back(d); }
             }
         }
         // among the last level pick the minimum degree (then lowest index)
         ecc = level[last];
         int best = last;
         for (int c : q) if (level[c] == ecc && (deg(c) < deg(best) || (deg(c) == deg(best) && c < best))) best = c;
-        for (int c : touched) level[c] = -1;
+        for (int c : q) level[c] = -1;
         return best;
     };
     for (int s0 = 0; s0 < nc; s0++) {
@@ -883,7 +920,6 @@ std::vector<int> rcm_order(int nc, const std::vector<int> &adj_off, const std::v
 }
 
 struct Prof {
-    std::vector<hipEvent_t> ev;  // pairs
     double total_ms = 0;
     int count = 0;
 };
@@ -899,20 +935,17 @@ struct fvmr_solver {
     bool mixed = false;
     std::vector<int> perm, iperm;   // perm[new] = reference cell; iperm[reference] = new
     // device storage
-    double *Ubuf[3] = {nullptr, nullptr, nullptr};
+    S4 *Ubuf[3] = {nullptr, nullptr, nullptr};
     int cur = 0;                    // index of the buffer holding the current state
-    void *Ux = nullptr, *Uy = nullptr;
-    double *rhs = nullptr;
-    void *geo = nullptr;            // all floating geometry of type G in one block
-    int *cface = nullptr, *cnode = nullptr, *epair = nullptr, *etype = nullptr, *dtcode = nullptr;
-    int2 *eLR = nullptr;
-    double *partial = nullptr, *norm_out = nullptr, *fix = nullptr, *vol_d = nullptr;
+    void *grad = nullptr;           // Grad<G>[nc]
+    S4 *rhs = nullptr, *fix = nullptr;
+    void *cgeo = nullptr, *nxy = nullptr, *egeo = nullptr, *emid = nullptr;
+    int *cface = nullptr, *cnode = nullptr, *dtcode = nullptr;
+    int4 *eint = nullptr;
+    double *partial = nullptr, *norm_out = nullptr;
     int *bad_list = nullptr;
     Scalars *sc = nullptr;
-    // geometry pointers into geo
-    void *g_cx, *g_cy, *g_vol, *g_nx, *g_ny, *g_enx, *g_eny, *g_elen, *g_emx, *g_emy;
     bool rhs_valid = false;
-    bool grad_valid = false;
     bool profiling = false;
     Prof prof[kNumProf];
     std::vector<hipEvent_t> pending[kNumProf];
@@ -924,13 +957,13 @@ template <class G>
 Mesh<G> mesh_args(const fvmr_solver *h) {
     Mesh<G> m;
     m.nc = h->nc; m.nf = h->nf;
-    m.cx = (const G *)h->g_cx; m.cy = (const G *)h->g_cy; m.vol = (const G *)h->g_vol;
+    m.cgeo = (const V4<G> *)h->cgeo;
     m.cface = h->cface; m.cnode = h->cnode;
-    m.nx_ = (const G *)h->g_nx; m.ny_ = (const G *)h->g_ny;
-    m.eLR = h->eLR;
-    m.enx = (const G *)h->g_enx; m.eny = (const G *)h->g_eny; m.elen = (const G *)h->g_elen;
-    m.emx = (const G *)h->g_emx; m.emy = (const G *)h->g_emy;
-    m.epair = h->epair; m.etype = h->etype; m.dtcode = h->dtcode;
+    m.nxy = (const V2<G> *)h->nxy;
+    m.egeo = (const V4<G> *)h->egeo;
+    m.emid = (const V2<G> *)h->emid;
+    m.eint = h->eint;
+    m.dtcode = h->dtcode;
     return m;
 }
 
@@ -967,21 +1000,20 @@ int prof_collect(fvmr_solver *h) {
 }
 
 int grid_of(int n) { return (n + kBlock - 1) / kBlock; }
-
 int grid3_of(int n) { return (n + kCellsPerGroup - 1) / kCellsPerGroup; }
 
 // Gradient + limiter; with dt_mode >= 0 also the global dt (dt_mode 1: clamp and advance t).
 template <class C, class G>
-int launch_gradient(fvmr_solver *h, const double *U, int dt_mode) {
+int launch_gradient(fvmr_solver *h, const S4 *U, int dt_mode) {
     hipEvent_t e0{};
     int rc = prof_begin(h, 1, &e0);
     if (rc) return rc;
     if (dt_mode >= 0)
-        k_gradient<C, G, true><<<grid3_of(h->nc), kGroup3, 0, h->stream>>>(mesh_args<G>(h), h->ph, U, (G *)h->Ux,
-                                                                          (G *)h->Uy, h->partial);
+        k_gradient<C, G, true><<<grid3_of(h->nc), kGroup3, 0, h->stream>>>(mesh_args<G>(h), h->ph, U,
+                                                                          (Grad<G> *)h->grad, h->partial);
     else
-        k_gradient<C, G, false><<<grid3_of(h->nc), kGroup3, 0, h->stream>>>(mesh_args<G>(h), h->ph, U, (G *)h->Ux,
-                                                                           (G *)h->Uy, h->partial);
+        k_gradient<C, G, false><<<grid3_of(h->nc), kGroup3, 0, h->stream>>>(mesh_args<G>(h), h->ph, U,
+                                                                           (Grad<G> *)h->grad, h->partial);
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 1, e0))) return rc;
     if (dt_mode < 0) return FVMR_OK;
@@ -992,22 +1024,20 @@ int launch_gradient(fvmr_solver *h, const double *U, int dt_mode) {
 }
 
 template <class C, class G>
-int launch_stage(fvmr_solver *h, const double *Uin, const double *Un, double *Uout, double a, double b, int first,
-                 double *rhs_out) {
+int launch_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, double a, double b, int first, S4 *rhs_out) {
     hipEvent_t e0{};
     int rc = prof_begin(h, 0, &e0);
     if (rc) return rc;
     if (h->ph.equation == FVMR_EQ_NS)
         k_stage<C, G, true><<<grid3_of(h->nc), kGroup3, 0, h->stream>>>(mesh_args<G>(h), h->ph, Uin, Un, Uout,
-            (const G *)h->Ux, (const G *)h->Uy, rhs_out, h->sc, h->bad_list, a, b, first);
+            (const Grad<G> *)h->grad, rhs_out, h->sc, h->bad_list, a, b, first);
     else
         k_stage<C, G, false><<<grid3_of(h->nc), kGroup3, 0, h->stream>>>(mesh_args<G>(h), h->ph, Uin, Un, Uout,
-            (const G *)h->Ux, (const G *)h->Uy, rhs_out, h->sc, h->bad_list, a, b, first);
+            (const Grad<G> *)h->grad, rhs_out, h->sc, h->bad_list, a, b, first);
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 0, e0))) return rc;
     if ((rc = prof_begin(h, 3, &e0))) return rc;
-    k_restrict<<<1, 1024, 0, h->stream>>>(h->nc, h->cface, h->eLR, h->epair, h->mixed ? nullptr : (const double *)h->g_vol,
-                                          h->mixed ? (const float *)h->g_vol : nullptr, Uout, h->sc, h->bad_list, h->fix);
+    k_restrict<G><<<1, 1024, 0, h->stream>>>(mesh_args<G>(h), Uout, h->sc, h->bad_list, h->fix);
     HIP_OK(hipGetLastError());
     return prof_end(h, 3, e0);
 }
@@ -1017,23 +1047,18 @@ template <class C, class G>
 int device_step(fvmr_solver *h, bool store_rhs) {
     int rc;
     const int a = h->cur, b = (a + 1) % 3, c = (a + 2) % 3;
-    double *Ua = h->Ubuf[a], *Ub = h->Ubuf[b], *Uc = h->Ubuf[c];
-    if (h->p.time_advance == FVMR_EVO_EXPLICIT) {
-        if ((rc = launch_gradient<C, G>(h, Ua, 1))) return rc;
-        if ((rc = launch_stage<C, G>(h, Ua, Ua, Ub, 0.0, 1.0, 1, store_rhs ? h->rhs : nullptr))) return rc;
-        h->cur = b;
-    } else {
+    S4 *Ua = h->Ubuf[a], *Ub = h->Ubuf[b], *Uc = h->Ubuf[c];
+    if ((rc = launch_gradient<C, G>(h, Ua, 1))) return rc;
+    if ((rc = launch_stage<C, G>(h, Ua, Ua, Ub, 0.0, 1.0, 1, store_rhs ? h->rhs : nullptr))) return rc;
+    if (h->p.time_advance == FVMR_EVO_RK3) {
         // SSP-RK3 (Shu-Osher): U1 = U + dt R(U); U2 = 3/4 U + 1/4 (U1 + dt R(U1));
         // U^{n+1} = 1/3 U + 2/3 (U2 + dt R(U2))
-        if ((rc = launch_gradient<C, G>(h, Ua, 1))) return rc;
-        if ((rc = launch_stage<C, G>(h, Ua, Ua, Ub, 0.0, 1.0, 1, store_rhs ? h->rhs : nullptr))) return rc;
         if ((rc = launch_gradient<C, G>(h, Ub, -1))) return rc;
         if ((rc = launch_stage<C, G>(h, Ub, Ua, Uc, 0.75, 0.25, 0, nullptr))) return rc;
         if ((rc = launch_gradient<C, G>(h, Uc, -1))) return rc;
         if ((rc = launch_stage<C, G>(h, Uc, Ua, Ub, 1.0 / 3.0, 2.0 / 3.0, 0, nullptr))) return rc;
-        h->cur = b;
     }
-    h->grad_valid = false;
+    h->cur = b;
     if (store_rhs) h->rhs_valid = true;
     return FVMR_OK;
 }
@@ -1047,34 +1072,30 @@ template <class G>
 int build_geometry(fvmr_solver *h, const fvmr_mesh_soa *m, const std::vector<int> &eperm) {
     // eperm[new edge] = reference edge
     const int nc = h->nc, nf = h->nf, nv = h->nv;
-    const size_t total = (size_t)3 * nc + 2 * nv + 5 * (size_t)nf;
-    std::vector<G> g(total);
-    G *cx = g.data(), *cy = cx + nc, *vol = cy + nc, *nx = vol + nc, *ny = nx + nv;
-    G *enx = ny + nv, *eny = enx + nf, *elen = eny + nf, *emx = elen + nf, *emy = emx + nf;
+    std::vector<V4<G>> cg(nc), eg(nf);
+    std::vector<V2<G>> nx(nv), em(nf);
     for (int i = 0; i < nc; i++) {
         const int r = h->perm[i];
-        cx[i] = (G)m->element_xy[0][r]; cy[i] = (G)m->element_xy[1][r]; vol[i] = (G)m->element_volume[r];
+        cg[i] = V4<G>{(G)m->element_xy[0][r], (G)m->element_xy[1][r], (G)m->element_volume[r], G(0)};
     }
-    for (int v = 0; v < nv; v++) { nx[v] = (G)m->node_xy[0][v]; ny[v] = (G)m->node_xy[1][v]; }
+    for (int v = 0; v < nv; v++) nx[v] = V2<G>{(G)m->node_xy[0][v], (G)m->node_xy[1][v]};
     for (int e = 0; e < nf; e++) {
         const int r = eperm[e];
-        enx[e] = (G)m->edge_normal[0][r]; eny[e] = (G)m->edge_normal[1][r]; elen[e] = (G)m->edge_length[r];
-        emx[e] = (G)m->edge_xy[0][r]; emy[e] = (G)m->edge_xy[1][r];
+        eg[e] = V4<G>{(G)m->edge_normal[0][r], (G)m->edge_normal[1][r], (G)m->edge_length[r], G(0)};
+        em[e] = V2<G>{(G)m->edge_xy[0][r], (G)m->edge_xy[1][r]};
     }
-    G *d = nullptr;
-    HIP_OK(hipMalloc((void **)&d, total * sizeof(G)));
-    HIP_OK(hipMemcpyAsync(d, g.data(), total * sizeof(G), hipMemcpyHostToDevice, h->stream));
-    h->geo = d;
-    h->g_cx = d; h->g_cy = d + nc; h->g_vol = d + 2 * (size_t)nc; h->g_nx = d + 3 * (size_t)nc; h->g_ny = d + 3 * (size_t)nc + nv;
-    G *eb = d + 3 * (size_t)nc + 2 * (size_t)nv;
-    h->g_enx = eb; h->g_eny = eb + nf; h->g_elen = eb + 2 * (size_t)nf; h->g_emx = eb + 3 * (size_t)nf; h->g_emy = eb + 4 * (size_t)nf;
+    int rc;
+    if ((rc = upload((V4<G> **)&h->cgeo, cg, h->stream)) || (rc = upload((V2<G> **)&h->nxy, nx, h->stream)) ||
+        (rc = upload((V4<G> **)&h->egeo, eg, h->stream)) || (rc = upload((V2<G> **)&h->emid, em, h->stream)))
+        return rc;
+    HIP_OK(hipStreamSynchronize(h->stream));  // the host vectors go out of scope
     return FVMR_OK;
 }
 
 void free_all(fvmr_solver *h) {
     for (auto &b : h->Ubuf) if (b) (void)hipFree(b);
-    void *ptrs[] = {h->Ux, h->Uy, h->rhs, h->geo, h->cface, h->cnode, h->epair, h->etype, h->dtcode, h->eLR,
-                    h->partial, h->norm_out, h->fix, h->bad_list, h->sc, h->vol_d};
+    void *ptrs[] = {h->grad, h->rhs, h->fix, h->cgeo, h->nxy, h->egeo, h->emid, h->cface, h->cnode, h->dtcode,
+                    h->eint, h->partial, h->norm_out, h->bad_list, h->sc};
     for (void *q : ptrs) if (q) (void)hipFree(q);
     for (int k = 0; k < kNumProf; k++)
         for (auto e : h->pending[k]) (void)hipEventDestroy(e);
@@ -1099,6 +1120,13 @@ int validate_params(const fvmr_params *p) {
 int validate_mesh(const fvmr_mesh_soa *m) {
     if (!m || m->num_element <= 0 || m->num_edge <= 0 || m->num_node <= 0) return set_error(FVMR_ERR_INVALID_ARG, "empty mesh");
     const int nc = m->num_element, nf = m->num_edge, nv = m->num_node;
+    for (int e = 0; e < nf; e++) {
+        const int L = m->edge_elementL[e], R = m->edge_elementR[e], s = m->edge_setID[e], pe = m->edge_periodicPair[e];
+        if (L < 0 || L >= nc || R < -1 || R >= nc) return set_error(FVMR_ERR_MESH, "edge owner out of range");
+        if (s != -1 && (s < 1 || s > m->num_boundary_set)) return set_error(FVMR_ERR_MESH, "edge setID out of range");
+        if (pe < -1 || pe >= nf) return set_error(FVMR_ERR_MESH, "periodic pair out of range");
+        if (R == -1 && s == -1) return set_error(FVMR_ERR_MESH, "boundary edge without a boundary set");
+    }
     for (int c = 0; c < nc; c++) {
         for (int j = 0; j < 3; j++) {
             const int e = m->element_edges[j][c], v = m->element_nodes[j][c];
@@ -1111,13 +1139,6 @@ int validate_mesh(const fvmr_mesh_soa *m) {
             if (m->element_neighbors[j][c] != expect)
                 return set_error(FVMR_ERR_MESH, "element neighbours disagree with the edge owners");
         }
-    }
-    for (int e = 0; e < nf; e++) {
-        const int L = m->edge_elementL[e], R = m->edge_elementR[e], s = m->edge_setID[e], pe = m->edge_periodicPair[e];
-        if (L < 0 || L >= nc || R < -1 || R >= nc) return set_error(FVMR_ERR_MESH, "edge owner out of range");
-        if (s != -1 && (s < 1 || s > m->num_boundary_set)) return set_error(FVMR_ERR_MESH, "edge setID out of range");
-        if (pe < -1 || pe >= nf) return set_error(FVMR_ERR_MESH, "periodic pair out of range");
-        if (R == -1 && s == -1) return set_error(FVMR_ERR_MESH, "boundary edge without a boundary set");
     }
     return FVMR_OK;
 }
@@ -1184,8 +1205,8 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
         }
     for (int e = 0; e < nf; e++) if (enew[e] < 0) { enew[e] = (int)eperm.size(); eperm.push_back(e); }
     // connectivity in the new numbering
-    std::vector<int> cface(3 * (size_t)nc), cnode(3 * (size_t)nc), epair(nf), etype(nf);
-    std::vector<int2> eLR(nf);
+    std::vector<int> cface(3 * (size_t)nc), cnode(3 * (size_t)nc);
+    std::vector<int4> eint(nf);
     for (int i = 0; i < nc; i++) {
         const int r = h->perm[i];
         for (int j = 0; j < 3; j++) {
@@ -1195,11 +1216,9 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
     }
     for (int e = 0; e < nf; e++) {
         const int r = eperm[e];
-        const int L = m->edge_elementL[r], R = m->edge_elementR[r];
-        eLR[e] = make_int2(h->iperm[L], R >= 0 ? h->iperm[R] : -1);
-        epair[e] = m->edge_periodicPair[r] >= 0 ? enew[m->edge_periodicPair[r]] : -1;
-        const int s = m->edge_setID[r];
-        etype[e] = (s == -1) ? -1 : m->boundary_type[s - 1];
+        const int L = m->edge_elementL[r], R = m->edge_elementR[r], s = m->edge_setID[r];
+        eint[e] = make_int4(h->iperm[L], R >= 0 ? h->iperm[R] : -1, (s == -1) ? -1 : m->boundary_type[s - 1],
+                            m->edge_periodicPair[r] >= 0 ? enew[m->edge_periodicPair[r]] : -1);
     }
     // alphaC contributions in the reference edge order (CalculateDt.cpp:22-63), encoded per
     // cell as count (3 bits) + 6 x (face slot 2 bits, pair-edge flag 1 bit)
@@ -1225,23 +1244,25 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
         for (int i = 0; i < nc; i++) dtcode[i] |= cnt[i];
     }
     // device memory
-    const size_t n4 = 4 * (size_t)nc;
-    const size_t gsz = h->mixed ? sizeof(float) : sizeof(double);
-    for (int k = 0; k < 3; k++) if (hipMalloc((void **)&h->Ubuf[k], n4 * sizeof(double)) != hipSuccess) return bail(set_error(FVMR_ERR_HIP, "hipMalloc U"));
-    if (hipMalloc(&h->Ux, n4 * gsz) != hipSuccess || hipMalloc(&h->Uy, n4 * gsz) != hipSuccess ||
-        hipMalloc((void **)&h->rhs, n4 * sizeof(double)) != hipSuccess || hipMalloc((void **)&h->fix, n4 * sizeof(double)) != hipSuccess ||
+    const size_t gsz = h->mixed ? sizeof(Grad<float>) : sizeof(Grad<double>);
+    for (int k = 0; k < 3; k++)
+        if (hipMalloc((void **)&h->Ubuf[k], (size_t)nc * sizeof(S4)) != hipSuccess) return bail(set_error(FVMR_ERR_HIP, "hipMalloc U"));
+    const int nparts = std::max(grid3_of(nc), 4 * grid_of(nc));
+    if (hipMalloc(&h->grad, (size_t)nc * gsz) != hipSuccess ||
+        hipMalloc((void **)&h->rhs, (size_t)nc * sizeof(S4)) != hipSuccess ||
+        hipMalloc((void **)&h->fix, (size_t)nc * sizeof(S4)) != hipSuccess ||
         hipMalloc((void **)&h->bad_list, (size_t)nc * sizeof(int)) != hipSuccess ||
-        hipMalloc((void **)&h->partial, (size_t)4 * grid_of(nc) * sizeof(double)) != hipSuccess ||
-        hipMalloc((void **)&h->norm_out, 4 * sizeof(double)) != hipSuccess || hipMalloc((void **)&h->sc, sizeof(Scalars)) != hipSuccess)
+        hipMalloc((void **)&h->partial, (size_t)nparts * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&h->norm_out, 4 * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&h->sc, sizeof(Scalars)) != hipSuccess)
         return bail(set_error(FVMR_ERR_HIP, "hipMalloc failed"));
     if ((rc = upload(&h->cface, cface, h->stream)) || (rc = upload(&h->cnode, cnode, h->stream)) ||
-        (rc = upload(&h->epair, epair, h->stream)) || (rc = upload(&h->etype, etype, h->stream)) ||
-        (rc = upload(&h->dtcode, dtcode, h->stream)) || (rc = upload(&h->eLR, eLR, h->stream)))
+        (rc = upload(&h->dtcode, dtcode, h->stream)) || (rc = upload(&h->eint, eint, h->stream)))
         return bail(rc);
     rc = h->mixed ? build_geometry<float>(h, m, eperm) : build_geometry<double>(h, m, eperm);
     if (rc) return bail(rc);
     Scalars s0{};
-    if (hipMemsetAsync(h->Ubuf[0], 0, n4 * sizeof(double), h->stream) != hipSuccess ||
+    if (hipMemsetAsync(h->Ubuf[0], 0, (size_t)nc * sizeof(S4), h->stream) != hipSuccess ||
         hipMemcpyAsync(h->sc, &s0, sizeof(Scalars), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
         return bail(set_error(FVMR_ERR_HIP, "initial copy failed"));
@@ -1259,76 +1280,66 @@ int fvmr_destroy(fvmr_handle *h) {
 
 int fvmr_set_state(fvmr_handle *h, const double *const U[4]) {
     if (!h || !U) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
-    const int nc = h->nc;
-    std::vector<double> buf(4 * (size_t)nc);
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < 4; k++)
         if (!U[k]) return set_error(FVMR_ERR_INVALID_ARG, "null state component");
-        for (int i = 0; i < nc; i++) buf[(size_t)k * nc + i] = U[k][h->perm[i]];
+    const int nc = h->nc;
+    std::vector<S4> buf(nc);
+    for (int i = 0; i < nc; i++) {
+        const int r = h->perm[i];
+        buf[i] = S4{U[0][r], U[1][r], U[2][r], U[3][r]};
     }
-    HIP_OK(hipMemcpyAsync(h->Ubuf[h->cur], buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(h->Ubuf[h->cur], buf.data(), buf.size() * sizeof(S4), hipMemcpyHostToDevice, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
     h->rhs_valid = false;
-    h->grad_valid = false;
+    return FVMR_OK;
+}
+
+static int read_back(fvmr_handle *h, const S4 *dev, double *const out[4]) {
+    for (int k = 0; k < 4; k++)
+        if (!out[k]) return set_error(FVMR_ERR_INVALID_ARG, "null output component");
+    const int nc = h->nc;
+    std::vector<S4> buf(nc);
+    HIP_OK(hipMemcpyAsync(buf.data(), dev, buf.size() * sizeof(S4), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < nc; i++) {
+        const int r = h->perm[i];
+        out[0][r] = buf[i].x; out[1][r] = buf[i].y; out[2][r] = buf[i].z; out[3][r] = buf[i].w;
+    }
     return FVMR_OK;
 }
 
 int fvmr_get_state(fvmr_handle *h, double *const U[4]) {
     if (!h || !U) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
-    const int nc = h->nc;
-    std::vector<double> buf(4 * (size_t)nc);
-    HIP_OK(hipMemcpyAsync(buf.data(), h->Ubuf[h->cur], buf.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(hipStreamSynchronize(h->stream));
-    for (int k = 0; k < 4; k++) {
-        if (!U[k]) return set_error(FVMR_ERR_INVALID_ARG, "null state component");
-        for (int i = 0; i < nc; i++) U[k][h->perm[i]] = buf[(size_t)k * nc + i];
-    }
-    return FVMR_OK;
+    return read_back(h, h->Ubuf[h->cur], U);
 }
 
-static int read_back(fvmr_handle *h, const double *dev, double *const out[4]) {
-    const int nc = h->nc;
-    std::vector<double> buf(4 * (size_t)nc);
-    HIP_OK(hipMemcpyAsync(buf.data(), dev, buf.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(hipStreamSynchronize(h->stream));
-    for (int k = 0; k < 4; k++)
-        for (int i = 0; i < nc; i++) out[k][h->perm[i]] = buf[(size_t)k * nc + i];
-    return FVMR_OK;
-}
-
-static int finish_info(fvmr_handle *h, fvmr_step_info *info, int steps, Scalars &s) {
+static int read_scalars(fvmr_handle *h, Scalars &s) {
     HIP_OK(hipMemcpyAsync(&s, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
-    if (info) {
-        info->t = s.t;
-        info->dt = s.dt;
-        info->n_restricted = s.n_restricted;
-        info->steps_done = steps;
-        if (info->residual_norm) {
-            int rc = fvmr_residual(h, info->residual_norm, info->residual);
-            if (rc) return rc;
-        }
-    }
     return FVMR_OK;
 }
 
 int fvmr_advance(fvmr_handle *h, int nsteps, double *t, double T, fvmr_step_info *info) {
     if (!h || !t || nsteps < 0) return set_error(FVMR_ERR_INVALID_ARG, "invalid argument");
     Scalars s{};
-    HIP_OK(hipMemcpyAsync(&s, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(hipStreamSynchronize(h->stream));
+    int rc = read_scalars(h, s);
+    if (rc) return rc;
     s.t = *t;
     s.T = T;
-    s.counter = 0; s.norm_counter = 0; s.bad_count = 0; s.n_restricted = 0;
+    s.counter = 0; s.bad_count = 0; s.n_restricted = 0;
     HIP_OK(hipMemcpyAsync(h->sc, &s, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
-    for (int i = 0; i < nsteps; i++) {
-        int rc = run_step(h, i == nsteps - 1);
-        if (rc) return rc;
+    for (int i = 0; i < nsteps; i++)
+        if ((rc = run_step(h, i == nsteps - 1))) return rc;
+    if ((rc = read_scalars(h, s))) return rc;
+    if (info) {
+        info->t = s.t;
+        info->dt = s.dt;
+        info->n_restricted = s.n_restricted;
+        info->steps_done = nsteps;
+        if (info->residual_norm && nsteps > 0 && (rc = fvmr_residual(h, info->residual_norm, info->residual))) return rc;
     }
-    Scalars s2;
-    int rc = finish_info(h, info, nsteps, s2);
-    if (rc) return rc;
     if (h->profiling && (rc = prof_collect(h))) return rc;
-    *t = s2.t;
+    *t = s.t;
     return FVMR_OK;
 }
 
@@ -1338,55 +1349,60 @@ int fvmr_stage_rhs(fvmr_handle *h, double *const rhs[4]) {
     if (!h || !rhs) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
     // one forward-Euler stage with dt = 0 into a scratch buffer: rhs of the current state
     Scalars s{};
-    HIP_OK(hipMemcpyAsync(&s, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(hipStreamSynchronize(h->stream));
+    int rc = read_scalars(h, s);
+    if (rc) return rc;
     Scalars z = s;
     z.dt = 0.0;
     z.t_rk = s.t;
     HIP_OK(hipMemcpyAsync(h->sc, &z, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
     const int a = h->cur, b = (a + 1) % 3;
-    int rc;
     if (h->mixed) {
-        if ((rc = launch_gradient<float, float>(h, h->Ubuf[a], -1)) || (rc = launch_stage<float, float>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs))) return rc;
+        if ((rc = launch_gradient<float, float>(h, h->Ubuf[a], -1)) ||
+            (rc = launch_stage<float, float>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs))) return rc;
     } else {
-        if ((rc = launch_gradient<double, double>(h, h->Ubuf[a], -1)) || (rc = launch_stage<double, double>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs))) return rc;
+        if ((rc = launch_gradient<double, double>(h, h->Ubuf[a], -1)) ||
+            (rc = launch_stage<double, double>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs))) return rc;
     }
     HIP_OK(hipMemcpyAsync(h->sc, &s, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
     h->rhs_valid = true;
-    h->grad_valid = true;
     return read_back(h, h->rhs, rhs);
 }
 
 int fvmr_gradient(fvmr_handle *h, double *const Ux[4], double *const Uy[4]) {
     if (!h || !Ux || !Uy) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
-    int rc = h->mixed ? launch_gradient<float, float>(h, h->Ubuf[h->cur], -1) : launch_gradient<double, double>(h, h->Ubuf[h->cur], -1);
-    if (rc) return rc;
-    h->grad_valid = true;
-    if (!h->mixed) {
-        if ((rc = read_back(h, (const double *)h->Ux, Ux))) return rc;
-        return read_back(h, (const double *)h->Uy, Uy);
-    }
-    const int nc = h->nc;
-    std::vector<float> bx(4 * (size_t)nc), by(4 * (size_t)nc);
-    HIP_OK(hipMemcpyAsync(bx.data(), h->Ux, bx.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(hipMemcpyAsync(by.data(), h->Uy, by.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(hipStreamSynchronize(h->stream));
     for (int k = 0; k < 4; k++)
+        if (!Ux[k] || !Uy[k]) return set_error(FVMR_ERR_INVALID_ARG, "null output component");
+    int rc = h->mixed ? launch_gradient<float, float>(h, h->Ubuf[h->cur], -1)
+                      : launch_gradient<double, double>(h, h->Ubuf[h->cur], -1);
+    if (rc) return rc;
+    const int nc = h->nc;
+    auto scatter = [&](auto &buf) {
         for (int i = 0; i < nc; i++) {
-            Ux[k][h->perm[i]] = bx[(size_t)k * nc + i];
-            Uy[k][h->perm[i]] = by[(size_t)k * nc + i];
+            const int r = h->perm[i];
+            for (int k = 0; k < 4; k++) { Ux[k][r] = buf[i].x[k]; Uy[k][r] = buf[i].y[k]; }
         }
+    };
+    if (h->mixed) {
+        std::vector<Grad<float>> buf(nc);
+        HIP_OK(hipMemcpyAsync(buf.data(), h->grad, buf.size() * sizeof(Grad<float>), hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        scatter(buf);
+    } else {
+        std::vector<Grad<double>> buf(nc);
+        HIP_OK(hipMemcpyAsync(buf.data(), h->grad, buf.size() * sizeof(Grad<double>), hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        scatter(buf);
+    }
     return FVMR_OK;
 }
 
 int fvmr_global_dt(fvmr_handle *h, double *dt) {
     if (!h || !dt) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
-    int rc = h->mixed ? launch_gradient<float, float>(h, h->Ubuf[h->cur], 0) : launch_gradient<double, double>(h, h->Ubuf[h->cur], 0);
-    h->grad_valid = true;
+    int rc = h->mixed ? launch_gradient<float, float>(h, h->Ubuf[h->cur], 0)
+                      : launch_gradient<double, double>(h, h->Ubuf[h->cur], 0);
     if (rc) return rc;
     Scalars s;
-    HIP_OK(hipMemcpyAsync(&s, h->sc, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(hipStreamSynchronize(h->stream));
+    if ((rc = read_scalars(h, s))) return rc;
     *dt = s.dt;
     return FVMR_OK;
 }
@@ -1398,7 +1414,9 @@ int fvmr_residual(fvmr_handle *h, int norm, double out[4]) {
     hipEvent_t e0{};
     int rc = prof_begin(h, 4, &e0);
     if (rc) return rc;
-    k_norm<<<grid_of(h->nc), kBlock, 0, h->stream>>>(h->nc, h->rhs, norm, h->partial, h->norm_out, h->sc);
+    k_norm<<<grid_of(h->nc), kBlock, 0, h->stream>>>(h->nc, h->rhs, norm, h->partial);
+    HIP_OK(hipGetLastError());
+    k_norm_finish<<<1, 64, 0, h->stream>>>(grid_of(h->nc), norm, h->partial, h->norm_out);
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 4, e0))) return rc;
     HIP_OK(hipMemcpyAsync(out, h->norm_out, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
