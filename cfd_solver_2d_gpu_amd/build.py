@@ -25,21 +25,25 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build_library(verbose=True):
-    """Compiles csrc/*.hip with hipcc and csrc/*.cpp with g++, links libfvmr.so."""
-    out_dir = os.path.join(HERE, "_build")
+def build_library(verbose=True, defines=(), lib_path=None):
+    """Compiles csrc/*.hip with hipcc and csrc/*.cpp with g++, links libfvmr.so.
+
+    defines: extra -D tuning knobs for the device code; lib_path: output path."""
+    lib_path = lib_path or LIB
+    tag = "_".join(d.replace("=", "-") for d in defines) if defines else "default"
+    out_dir = os.path.join(HERE, "_build", tag)
     os.makedirs(out_dir, exist_ok=True)
     objs = []
     mesh_o = os.path.join(out_dir, "fvmr_mesh.o")
     _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "fvmr_mesh.cpp"), "-o", mesh_o])
     objs.append(mesh_o)
     solver_o = os.path.join(out_dir, "fvmr_solver.o")
-    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "fvmr_solver.hip"), "-o", solver_o])
+    _run([HIPCC, *HIP_FLAGS, *["-D" + d for d in defines], "-c", os.path.join(CSRC, "fvmr_solver.hip"), "-o", solver_o])
     objs.append(solver_o)
-    tmp = LIB + ".tmp"
+    tmp = lib_path + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib_path)
+    return lib_path
 
 
 if __name__ == "__main__":

@@ -52,10 +52,39 @@ constexpr int kGroup3 = 3 * kCellsPerGroup;     // one wave64 per face slot
             return set_error(FVMR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Tuning knobs (compile time): minimum waves per SIMD requested for the cell kernels
+// (0: compiler default) and the XCD-aware block remap.
+#ifndef FVMR_STAGE_WPE
+#define FVMR_STAGE_WPE 0
+#endif
+#ifndef FVMR_GRAD_WPE
+#define FVMR_GRAD_WPE 0
+#endif
+#ifndef FVMR_XCD_REMAP
+#define FVMR_XCD_REMAP 1
+#endif
+#if FVMR_STAGE_WPE > 0
+#define FVMR_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(FVMR_STAGE_WPE)))
+#else
+#define FVMR_STAGE_ATTR
+#endif
+#if FVMR_GRAD_WPE > 0
+#define FVMR_GRAD_ATTR __attribute__((amdgpu_waves_per_eu(FVMR_GRAD_WPE)))
+#else
+#define FVMR_GRAD_ATTR
+#endif
+
 template <class T> struct alignas(4 * sizeof(T)) V4 { T x, y, z, w; };
 template <class T> struct alignas(2 * sizeof(T)) V2 { T x, y; };
 template <class T> struct alignas(4 * sizeof(T)) Grad { T x[4]; T y[4]; };
 using S4 = V4<double>;
+
+// flags of a face slot (cslot.x): the cell is the edge's R; the edge belongs to a boundary
+// set (its BC type and periodic pair are in eint); the edge is periodic
+constexpr int kSlotCellIsR = 1 << 30;
+constexpr int kSlotBoundary = 1 << 29;
+constexpr int kSlotPeriodic = 1 << 28;
+constexpr int kSlotEdgeMask = (1 << 28) - 1;
 
 // ---------------------------------------------------------------------------
 // Device-side arguments (plain pointers, passed by value).
@@ -64,7 +93,7 @@ template <class G>
 struct Mesh {
     int nc, nf;
     const V4<G> *cgeo;     // cx, cy, volume, 0
-    const int *cface;      // [3][nc] edge of local slot j
+    const int2 *cslot;     // [3][nc] face slot j: x = edge | flags, y = cell across the face (-1 none)
     const int *cnode;      // [3][nc] vertex j
     const V2<G> *nxy;      // node coordinates
     const V4<G> *egeo;     // nx, ny, length, 0 (normal L -> R)
@@ -93,6 +122,7 @@ struct Scalars {
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous
 // range of logical blocks so that neighbouring cells share an L2.
 __device__ __forceinline__ int xcd_block() {
+    if (!FVMR_XCD_REMAP) return blockIdx.x;
     const int nb = gridDim.x, b = blockIdx.x;
     const int q = nb >> 3, r = nb & 7, x = b & 7;
     return x * q + min(x, r) + (b >> 3);
@@ -385,26 +415,31 @@ __device__ __forceinline__ void flux_viscous(const Mesh<G> &m, const Phys &ph, c
 // the canonical flux of edge e, oriented L -> R.
 template <class C, class G, bool NS>
 __device__ __forceinline__ void edge_flux(const Mesh<G> &m, const Phys &ph, const S4 *__restrict__ U,
-                                          const Grad<G> *__restrict__ grad, double t_rk, int e, const int4 ei,
-                                          C F[4]) {
+                                          const Grad<G> *__restrict__ grad, double t_rk, int e, int Lc, int Rc,
+                                          bool bset, C F[4]) {
     const V4<G> eg = m.egeo[e];
     const V2<G> em = m.emid[e];
-    const int bType = ei.z;
+    int bType = -1, pe = -1;
+    if (bset) {
+        const int4 ei = m.eint[e];
+        bType = ei.z;
+        pe = ei.w;
+    }
     const C nx = (C)eg.x, ny = (C)eg.y, len = (C)eg.z;
     const C xe = (C)em.x, ye = (C)em.y;
     const C gamma = (C)ph.gamma;
     FaceState<C> L, R;
-    reconstruct<C, G>(m, ph, U, grad, ei.x, xe, ye, L);
+    reconstruct<C, G>(m, ph, U, grad, Lc, xe, ye, L);
     if (bType == -1 || is_periodic(bType)) {
         // interior and periodic edges (get_UR_inner_and_periodic_device, FluxGPU.cu:169-188)
-        if (ei.y != -1) {
+        if (Rc != -1) {
             C xr = xe, yr = ye;
-            if (bType != -1 && ei.w >= 0 && ei.w < m.nf) {
-                const V2<G> pm = m.emid[ei.w];
+            if (bType != -1 && pe >= 0 && pe < m.nf) {
+                const V2<G> pm = m.emid[pe];
                 xr = (C)pm.x;
                 yr = (C)pm.y;
             }
-            reconstruct<C, G>(m, ph, U, grad, ei.y, xr, yr, R);
+            reconstruct<C, G>(m, ph, U, grad, Rc, xr, yr, R);
         } else {
 #pragma unroll
             for (int i = 0; i < 4; i++) R.U[i] = C(0);
@@ -433,7 +468,7 @@ __device__ __forceinline__ void edge_flux(const Mesh<G> &m, const Phys &ph, cons
     else flux_roe(L, R, nx, ny, len, F, gamma, g_ga1, (C)ph.inv_g);
     if (NS) {
         C Fv[4];
-        flux_viscous<C, G>(m, ph, U, grad, ei.x, ei.y, nx, ny, len, Fv);
+        flux_viscous<C, G>(m, ph, U, grad, Lc, Rc, nx, ny, len, Fv);
 #pragma unroll
         for (int k = 0; k < 4; k++) F[k] -= Fv[k];
     }
@@ -488,7 +523,7 @@ __device__ __forceinline__ void edge_alpha(const Phys &ph, const double ra[4], c
 // cell from the same loads, summed in the reference edge order, and writes the block
 // minimum of CFL V / alphaC (get_global_dt_device, CalculateDtGPU.cu:97-103).
 template <class C, class G, bool DT>
-__global__ __launch_bounds__(kGroup3) void k_gradient(Mesh<G> m, Phys ph, const S4 *__restrict__ U,
+__global__ __launch_bounds__(kGroup3) FVMR_GRAD_ATTR void k_gradient(Mesh<G> m, Phys ph, const S4 *__restrict__ U,
                                                       Grad<G> *__restrict__ grad, double *__restrict__ partial) {
     __shared__ C sUo[3][4][kCellsPerGroup];   // out-cell state of slot j (own state when none)
     __shared__ C sA[3][3][kCellsPerGroup];    // GG: gC, Sx*s, Sy*s;  LSQ: dx, dy
@@ -503,11 +538,12 @@ __global__ __launch_bounds__(kGroup3) void k_gradient(Mesh<G> m, Phys ph, const 
     if (live) {
         const V4<G> cg = m.cgeo[c];
         const C xc = (C)cg.x, yc = (C)cg.y;
-        const int e = m.cface[j * nc + c];
-        const int4 ei = m.eint[e];
-        const int pe = ei.w;
-        const int out = (c != ei.y) ? ei.y : ei.x;
-        const C sgn = (c != ei.y) ? C(1) : C(-1);
+        const int2 sl = m.cslot[j * nc + c];
+        const int e = sl.x & kSlotEdgeMask;
+        const int out = sl.y;
+        const bool periodic = (sl.x & kSlotPeriodic) != 0;
+        const C sgn = (sl.x & kSlotCellIsR) ? C(-1) : C(1);
+        const int pe = periodic ? m.eint[e].w : -1;
         double Uo[4];
         ld(U, out != -1 ? out : c, Uo);
 #pragma unroll
@@ -524,7 +560,7 @@ __global__ __launch_bounds__(kGroup3) void k_gradient(Mesh<G> m, Phys ph, const 
                 const V4<G> og = m.cgeo[out];
                 const C fx = (C)em.x, fy = (C)em.y;
                 C Fx = (C)og.x, Fy = (C)og.y;
-                if (ei.z != -1 && pe >= 0) {
+                if (periodic && pe >= 0) {
                     const V2<G> pm = m.emid[pe];
                     Fx += fx - (C)pm.x;
                     Fy += fy - (C)pm.y;
@@ -539,7 +575,7 @@ __global__ __launch_bounds__(kGroup3) void k_gradient(Mesh<G> m, Phys ph, const 
             sA[j][1][lane] = (C)eg.x * len * sgn;
             sA[j][2][lane] = (C)eg.y * len * sgn;
         } else {
-            const int nbr = (pe >= 0) ? -1 : out;   // element neighbours exclude periodic faces
+            const int nbr = periodic ? -1 : out;   // element neighbours exclude periodic faces
             C dxn = C(0), dyn = C(0);
             if (nbr != -1) {
                 const V4<G> og = m.cgeo[nbr];
@@ -549,7 +585,7 @@ __global__ __launch_bounds__(kGroup3) void k_gradient(Mesh<G> m, Phys ph, const 
             sA[j][0][lane] = dxn;
             sA[j][1][lane] = dyn;
         }
-        sFl[j][lane] = ((out != -1) ? 1 : 0) | ((out != -1 && pe < 0) ? 2 : 0);
+        sFl[j][lane] = ((out != -1) ? 1 : 0) | ((out != -1 && !periodic) ? 2 : 0);
         if (DT) {
             double Uc[4], rc[4], ro[4];
             ld(U, c, Uc);
@@ -707,7 +743,7 @@ __global__ __launch_bounds__(kGroup3) void k_gradient(Mesh<G> m, Phys ph, const 
 // signed fluxes in slot order and updates.  Cells whose new state is NaN or out of range
 // are appended to the restriction list.
 template <class C, class G, bool NS>
-__global__ __launch_bounds__(kGroup3) void k_stage(Mesh<G> m, Phys ph, const S4 *__restrict__ Uin,
+__global__ __launch_bounds__(kGroup3) FVMR_STAGE_ATTR void k_stage(Mesh<G> m, Phys ph, const S4 *__restrict__ Uin,
                                                    const S4 *__restrict__ Un, S4 *__restrict__ Uout,
                                                    const Grad<G> *__restrict__ grad, S4 *__restrict__ rhs_out,
                                                    Scalars *__restrict__ sc, int *__restrict__ bad_list, double a,
@@ -718,11 +754,12 @@ __global__ __launch_bounds__(kGroup3) void k_stage(Mesh<G> m, Phys ph, const S4 
     const int nc = m.nc;
     if (c < nc) {
         const double t_rk = sc->t_rk;
-        const int e = m.cface[j * nc + c];
-        const int4 ei = m.eint[e];
+        const int2 sl = m.cslot[j * nc + c];
+        const bool cR = (sl.x & kSlotCellIsR) != 0;
         C F[4];
-        edge_flux<C, G, NS>(m, ph, Uin, grad, t_rk, e, ei, F);
-        const double s = (c != ei.y) ? 1.0 : -1.0;
+        edge_flux<C, G, NS>(m, ph, Uin, grad, t_rk, sl.x & kSlotEdgeMask, cR ? sl.y : c, cR ? c : sl.y,
+                            (sl.x & kSlotBoundary) != 0, F);
+        const double s = cR ? -1.0 : 1.0;   // +1 when the cell is not the edge's R
 #pragma unroll
         for (int k = 0; k < 4; k++) sF[j][k][lane] = s * (double)F[k];
     }
@@ -768,9 +805,8 @@ __global__ __launch_bounds__(1024) void k_restrict(Mesh<G> m, S4 *__restrict__ U
         int nvalid = 0;
         double Usum[4] = {0, 0, 0, 0}, vs = 0.0;
         for (int j = 0; j < 3; j++) {
-            const int4 ei = m.eint[m.cface[j * nc + c]];
-            int nb = (c != ei.y) ? ei.y : ei.x;
-            if (ei.w >= 0) nb = -1;   // periodic neighbours are not element neighbours
+            const int2 sl = m.cslot[j * nc + c];
+            const int nb = (sl.x & kSlotPeriodic) ? -1 : sl.y;   // periodic neighbours are not element neighbours
             if (nb == -1) continue;
             nvalid++;
             const double v = (double)m.cgeo[nb].z;
@@ -940,7 +976,8 @@ struct fvmr_solver {
     void *grad = nullptr;           // Grad<G>[nc]
     S4 *rhs = nullptr, *fix = nullptr;
     void *cgeo = nullptr, *nxy = nullptr, *egeo = nullptr, *emid = nullptr;
-    int *cface = nullptr, *cnode = nullptr, *dtcode = nullptr;
+    int *cnode = nullptr, *dtcode = nullptr;
+    int2 *cslot = nullptr;
     int4 *eint = nullptr;
     double *partial = nullptr, *norm_out = nullptr;
     int *bad_list = nullptr;
@@ -958,7 +995,7 @@ Mesh<G> mesh_args(const fvmr_solver *h) {
     Mesh<G> m;
     m.nc = h->nc; m.nf = h->nf;
     m.cgeo = (const V4<G> *)h->cgeo;
-    m.cface = h->cface; m.cnode = h->cnode;
+    m.cslot = h->cslot; m.cnode = h->cnode;
     m.nxy = (const V2<G> *)h->nxy;
     m.egeo = (const V4<G> *)h->egeo;
     m.emid = (const V2<G> *)h->emid;
@@ -1094,7 +1131,7 @@ int build_geometry(fvmr_solver *h, const fvmr_mesh_soa *m, const std::vector<int
 
 void free_all(fvmr_solver *h) {
     for (auto &b : h->Ubuf) if (b) (void)hipFree(b);
-    void *ptrs[] = {h->grad, h->rhs, h->fix, h->cgeo, h->nxy, h->egeo, h->emid, h->cface, h->cnode, h->dtcode,
+    void *ptrs[] = {h->grad, h->rhs, h->fix, h->cgeo, h->nxy, h->egeo, h->emid, h->cslot, h->cnode, h->dtcode,
                     h->eint, h->partial, h->norm_out, h->bad_list, h->sc};
     for (void *q : ptrs) if (q) (void)hipFree(q);
     for (int k = 0; k < kNumProf; k++)
@@ -1205,20 +1242,30 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
         }
     for (int e = 0; e < nf; e++) if (enew[e] < 0) { enew[e] = (int)eperm.size(); eperm.push_back(e); }
     // connectivity in the new numbering
-    std::vector<int> cface(3 * (size_t)nc), cnode(3 * (size_t)nc);
+    if (nf > kSlotEdgeMask) return bail(set_error(FVMR_ERR_UNSUPPORTED, "too many edges for the face-slot encoding"));
+    std::vector<int2> cslot(3 * (size_t)nc);
+    std::vector<int> cnode(3 * (size_t)nc);
     std::vector<int4> eint(nf);
-    for (int i = 0; i < nc; i++) {
-        const int r = h->perm[i];
-        for (int j = 0; j < 3; j++) {
-            cface[(size_t)j * nc + i] = enew[m->element_edges[j][r]];
-            cnode[(size_t)j * nc + i] = m->element_nodes[j][r];
-        }
-    }
     for (int e = 0; e < nf; e++) {
         const int r = eperm[e];
         const int L = m->edge_elementL[r], R = m->edge_elementR[r], s = m->edge_setID[r];
         eint[e] = make_int4(h->iperm[L], R >= 0 ? h->iperm[R] : -1, (s == -1) ? -1 : m->boundary_type[s - 1],
                             m->edge_periodicPair[r] >= 0 ? enew[m->edge_periodicPair[r]] : -1);
+    }
+    for (int i = 0; i < nc; i++) {
+        const int r = h->perm[i];
+        for (int j = 0; j < 3; j++) {
+            const int er = m->element_edges[j][r];
+            const int e = enew[er];
+            const int4 ei = eint[e];
+            // reference: the cell across is R unless the cell is R itself (GradientGPU.cu:264-271)
+            const bool cR = (i == ei.y);
+            int flags = cR ? kSlotCellIsR : 0;
+            if (m->edge_setID[er] != -1) flags |= kSlotBoundary;
+            if (m->edge_periodicPair[er] >= 0) flags |= kSlotPeriodic;
+            cslot[(size_t)j * nc + i] = make_int2(e | flags, cR ? ei.x : ei.y);
+            cnode[(size_t)j * nc + i] = m->element_nodes[j][r];
+        }
     }
     // alphaC contributions in the reference edge order (CalculateDt.cpp:22-63), encoded per
     // cell as count (3 bits) + 6 x (face slot 2 bits, pair-edge flag 1 bit)
@@ -1256,7 +1303,7 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
         hipMalloc((void **)&h->norm_out, 4 * sizeof(double)) != hipSuccess ||
         hipMalloc((void **)&h->sc, sizeof(Scalars)) != hipSuccess)
         return bail(set_error(FVMR_ERR_HIP, "hipMalloc failed"));
-    if ((rc = upload(&h->cface, cface, h->stream)) || (rc = upload(&h->cnode, cnode, h->stream)) ||
+    if ((rc = upload(&h->cslot, cslot, h->stream)) || (rc = upload(&h->cnode, cnode, h->stream)) ||
         (rc = upload(&h->dtcode, dtcode, h->stream)) || (rc = upload(&h->eint, eint, h->stream)))
         return bail(rc);
     rc = h->mixed ? build_geometry<float>(h, m, eperm) : build_geometry<double>(h, m, eperm);
