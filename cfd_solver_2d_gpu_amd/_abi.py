@@ -22,6 +22,8 @@ LIM_NONE, LIM_BARTH, LIM_VENKATAKRISHNAN = 0, 1, 4
 BC_WALL_NONVISCOUS, BC_WALL_ADIABAT, BC_WALL_ISOTHERMAL = 6001, 6002, 6003
 BC_INLET, BC_OUTLET, BC_INF, BC_DSR, BC_SYMMETRY, BC_PERIODIC_0 = 6011, 6012, 6013, 6014, 6021, 6100
 PREC_FP64, PREC_MIXED = 0, 1
+RENUMBER_NONE, RENUMBER_RCM, RENUMBER_HILBERT = 0, 1, 2
+KERNELS_AUTO, KERNELS_CELL, KERNELS_TILE = 0, 1, 2
 MU_SUTHERLAND, MU_CONSTANT = 0, 1
 NORM_1, NORM_2, NORM_INF = 1, 2, 3
 INIT_UNIFORM, INIT_ISENTROPIC_VORTEX, INIT_SHOCK_TUBE, INIT_DSR = 1, 2, 3, 4
@@ -51,6 +53,7 @@ class Params(C.Structure):
         ("equation", C.c_int), ("flux_scheme", C.c_int), ("reconstruct", C.c_int), ("gradient", C.c_int),
         ("limiter", C.c_int), ("time_advance", C.c_int), ("viscosity_model", C.c_int), ("precision", C.c_int),
         ("renumber", C.c_int), ("compat_inv2x2", C.c_int),
+        ("kernels", C.c_int), ("tile_cells", C.c_int),
         ("inf_ruvp", C.c_double * 4), ("inlet_ruvp", C.c_double * 4), ("outlet_ruvp", C.c_double * 4),
         ("wall_temperature", C.c_double), ("venkat_K", C.c_double),
         ("dsr_shock_x", C.c_double), ("dsr_shock_y", C.c_double), ("dsr_shock_speed", C.c_double),

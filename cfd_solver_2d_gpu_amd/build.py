@@ -26,7 +26,7 @@ def _run(cmd):
 
 
 def build_library(verbose=True, defines=(), lib_path=None):
-    """Compiles csrc/*.hip with hipcc and csrc/*.cpp with g++, links libfvmr.so.
+    """Compiles csrc/fvmr_solver.hip with hipcc and csrc/*.cpp with g++, links libfvmr.so.
 
     defines: extra -D tuning knobs for the device code; lib_path: output path."""
     lib_path = lib_path or LIB
@@ -34,9 +34,10 @@ def build_library(verbose=True, defines=(), lib_path=None):
     out_dir = os.path.join(HERE, "_build", tag)
     os.makedirs(out_dir, exist_ok=True)
     objs = []
-    mesh_o = os.path.join(out_dir, "fvmr_mesh.o")
-    _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "fvmr_mesh.cpp"), "-o", mesh_o])
-    objs.append(mesh_o)
+    for name in ("fvmr_mesh", "fvmr_tiles"):
+        obj = os.path.join(out_dir, name + ".o")
+        _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, name + ".cpp"), "-o", obj])
+        objs.append(obj)
     solver_o = os.path.join(out_dir, "fvmr_solver.o")
     _run([HIPCC, *HIP_FLAGS, *["-D" + d for d in defines], "-c", os.path.join(CSRC, "fvmr_solver.hip"), "-o", solver_o])
     objs.append(solver_o)

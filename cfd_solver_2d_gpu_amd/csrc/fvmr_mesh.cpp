@@ -536,8 +536,10 @@ int fvmr_params_default(fvmr_params *p) {
     p->time_advance = FVMR_EVO_EXPLICIT;
     p->viscosity_model = FVMR_MU_SUTHERLAND;
     p->precision = FVMR_PREC_FP64;
-    p->renumber = 0;
+    p->renumber = FVMR_RENUMBER_HILBERT;
     p->compat_inv2x2 = 0;
+    p->kernels = FVMR_KERNELS_AUTO;
+    p->tile_cells = 0;
     p->wall_temperature = 288.16;
     p->venkat_K = 5.0;
     p->dsr_shock_speed = 20.0;

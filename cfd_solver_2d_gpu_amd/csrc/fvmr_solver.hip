@@ -34,6 +34,7 @@
 
 #include "../../include/fvmr.h"
 #include "fvmr_internal.h"
+#include "fvmr_tiles.h"
 
 using fvmr::set_error;
 
@@ -185,20 +186,35 @@ __device__ __forceinline__ void set_prim(FaceState<C> &s, C gamma) {
     s.p = s.r * (gamma - 1) * (s.U[3] * s.ir - (s.u * s.u + s.v * s.v) * C(0.5));
 }
 
+// Where a face kernel finds the state, centroid and limited gradient of a cell: global
+// records (cell kernels) or a tile's LDS (tile kernels).  has(c): c is a cell (not -1).
+template <class G>
+struct GlobalCells {
+    const Mesh<G> &m;
+    const S4 *__restrict__ U;
+    const Grad<G> *__restrict__ grad;
+    __device__ __forceinline__ void state(int c, double u[4]) const { ld(U, c, u); }
+    __device__ __forceinline__ V2<G> centroid(int c) const {
+        const V4<G> g = m.cgeo[c];
+        return V2<G>{g.x, g.y};
+    }
+    __device__ __forceinline__ Grad<G> gradient(int c) const { return grad[c]; }
+    __device__ __forceinline__ bool has(int c) const { return !(c < 0 || c >= m.nc); }
+};
+
 // get_Uvector_reconstruct_2_device (FluxGPU.cu:47-61): linear (3002) or constant
 // reconstruction; back to the cell value when rho or p leave their range.
-template <class C, class G>
-__device__ __forceinline__ void reconstruct(const Mesh<G> &m, const Phys &ph, const S4 *__restrict__ U,
-                                            const Grad<G> *__restrict__ grad, int c, C x, C y, FaceState<C> &s) {
+template <class C, class G, class Cells>
+__device__ __forceinline__ void reconstruct(const Cells &cells, const Phys &ph, int c, C x, C y, FaceState<C> &s) {
     const C gamma = (C)ph.gamma;
     double Ud[4];
-    ld(U, c, Ud);
+    cells.state(c, Ud);
     C Uc[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) Uc[i] = (C)Ud[i];
     if (ph.reconstruct == FVMR_REC_LINEAR) {
-        const V4<G> g = m.cgeo[c];
-        const Grad<G> gr = grad[c];
+        const V2<G> g = cells.centroid(c);
+        const Grad<G> gr = cells.gradient(c);
         const C dx = x - (C)g.x, dy = y - (C)g.y;
 #pragma unroll
         for (int i = 0; i < 4; i++) s.U[i] = Uc[i] + (C)gr.x[i] * dx + (C)gr.y[i] * dy;
@@ -350,24 +366,22 @@ __device__ __forceinline__ double mu_of_T(const Phys &ph, double T) {
 }
 
 // viscous_flux_device_kernel body (ViscousFluxGPU.cu:139-229); returns F_v (to be subtracted)
-template <class C, class G>
-__device__ __forceinline__ void flux_viscous(const Mesh<G> &m, const Phys &ph, const S4 *__restrict__ U,
-                                             const Grad<G> *__restrict__ grad, int L, int R, C nx, C ny, C area,
+template <class C, class G, class Cells>
+__device__ __forceinline__ void flux_viscous(const Cells &cells, const Phys &ph, int L, int R, C nx, C ny, C area,
                                              C F[4]) {
-    const int nc = m.nc;
     const C ga1 = (C)ph.gamma - C(1.0);
     const C Cv = (C)ph.R / ga1;
     const C Cp = Cv * (C)ph.gamma;
-    const bool hasR = !(R < 0 || R >= nc);
+    const bool hasR = cells.has(R);
     C gx[4], gy[4], Um[4];
     {
-        const Grad<G> gL = grad[L];
+        const Grad<G> gL = cells.gradient(L);
         double uL[4];
-        ld(U, L, uL);
+        cells.state(L, uL);
         if (hasR) {
-            const Grad<G> gR = grad[R];
+            const Grad<G> gR = cells.gradient(R);
             double uR[4];
-            ld(U, R, uR);
+            cells.state(R, uR);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 gx[k] = C(0.5) * ((C)gL.x[k] + (C)gR.x[k]);
@@ -412,34 +426,25 @@ __device__ __forceinline__ void flux_viscous(const Mesh<G> &m, const Phys &ph, c
 }
 
 // edge_convection_flux_device_kernel body (FluxGPU.cu:207-283) (+ viscous part for NS):
-// the canonical flux of edge e, oriented L -> R.
-template <class C, class G, bool NS>
-__device__ __forceinline__ void edge_flux(const Mesh<G> &m, const Phys &ph, const S4 *__restrict__ U,
-                                          const Grad<G> *__restrict__ grad, double t_rk, int e, int Lc, int Rc,
-                                          bool bset, C F[4]) {
-    const V4<G> eg = m.egeo[e];
-    const V2<G> em = m.emid[e];
-    int bType = -1, pe = -1;
-    if (bset) {
-        const int4 ei = m.eint[e];
-        bType = ei.z;
-        pe = ei.w;
-    }
+// the canonical flux of a face, oriented L -> R.  bType: BC code of the face's set (-1
+// interior); (pmx, pmy): midpoint of the periodic pair face, where R is reconstructed.
+template <class C, class G, bool NS, class Cells>
+__device__ __forceinline__ void face_flux(const Cells &cells, const Phys &ph, double t_rk, int Lc, int Rc, int bType,
+                                          bool has_pair, const V4<G> eg, const V2<G> em, const V2<G> pm, C F[4]) {
     const C nx = (C)eg.x, ny = (C)eg.y, len = (C)eg.z;
     const C xe = (C)em.x, ye = (C)em.y;
     const C gamma = (C)ph.gamma;
     FaceState<C> L, R;
-    reconstruct<C, G>(m, ph, U, grad, Lc, xe, ye, L);
+    reconstruct<C, G>(cells, ph, Lc, xe, ye, L);
     if (bType == -1 || is_periodic(bType)) {
         // interior and periodic edges (get_UR_inner_and_periodic_device, FluxGPU.cu:169-188)
         if (Rc != -1) {
             C xr = xe, yr = ye;
-            if (bType != -1 && pe >= 0 && pe < m.nf) {
-                const V2<G> pm = m.emid[pe];
+            if (bType != -1 && has_pair) {
                 xr = (C)pm.x;
                 yr = (C)pm.y;
             }
-            reconstruct<C, G>(m, ph, U, grad, Rc, xr, yr, R);
+            reconstruct<C, G>(cells, ph, Rc, xr, yr, R);
         } else {
 #pragma unroll
             for (int i = 0; i < 4; i++) R.U[i] = C(0);
@@ -468,10 +473,31 @@ __device__ __forceinline__ void edge_flux(const Mesh<G> &m, const Phys &ph, cons
     else flux_roe(L, R, nx, ny, len, F, gamma, g_ga1, (C)ph.inv_g);
     if (NS) {
         C Fv[4];
-        flux_viscous<C, G>(m, ph, U, grad, Lc, Rc, nx, ny, len, Fv);
+        flux_viscous<C, G>(cells, ph, Lc, Rc, nx, ny, len, Fv);
 #pragma unroll
         for (int k = 0; k < 4; k++) F[k] -= Fv[k];
     }
+}
+
+// The flux of edge e from the global records; the BC type and pair are read only for
+// boundary-set edges (bset).
+template <class C, class G, bool NS>
+__device__ __forceinline__ void edge_flux(const Mesh<G> &m, const Phys &ph, const S4 *__restrict__ U,
+                                          const Grad<G> *__restrict__ grad, double t_rk, int e, int Lc, int Rc,
+                                          bool bset, C F[4]) {
+    const V4<G> eg = m.egeo[e];
+    const V2<G> em = m.emid[e];
+    int bType = -1, pe = -1;
+    if (bset) {
+        const int4 ei = m.eint[e];
+        bType = ei.z;
+        pe = ei.w;
+    }
+    const bool has_pair = pe >= 0 && pe < m.nf;
+    V2<G> pm = em;
+    if (bType != -1 && has_pair) pm = m.emid[pe];
+    const GlobalCells<G> cells{m, U, grad};
+    face_flux<C, G, NS>(cells, ph, t_rk, Lc, Rc, bType, has_pair, eg, em, pm, F);
 }
 
 // ---------------------------------------------------------------------------
@@ -513,6 +539,107 @@ __device__ __forceinline__ void edge_alpha(const Phys &ph, const double ra[4], c
         const double value_1 = fmax(4.0 / 3.0, gamma / ph.Pr);
         const double T = p / ph.R / rho;
         *vis = 2.0 * len * value_1 * mu_of_T(ph, T) / rho / vol_c;
+    }
+}
+
+// Gradient and limiter of one cell from its face-slot data (GradientGreenGaussKernel,
+// GradientGPU.cu:223-425, or the least-squares gradient, Gradient.cpp:160-246, with the
+// correct 2x2 inverse unless compat; then LimiterBarth_device_kernel, GradientGPU.cu:453-502,
+// or Venkatakrishnan).  Per slot i: fl bit 0 = a cell across, bit 1 = a non-periodic
+// neighbour; A = (gC, Sx s, Sy s) for Green-Gauss or (dx, dy) for least squares; Uo = the
+// state across (own state when none); N = vertex i relative to the centroid.
+template <class C, class Slots>
+__device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[4], const Slots &S, const int fl[3], C vol,
+                                           C gx[4], C gy[4]) {
+    if (ph.gradient == FVMR_GRA_GREENGAUSS) {
+        const C inv_vol = C(1) / vol;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            C s0 = C(0), s1 = C(0);
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const C g = S.A(i, 0);
+                const C phif = (fl[i] & 1) ? g * Uc[k] + (C(1.0) - g) * S.Uo(i, k) : Uc[k];
+                s0 += phif * S.A(i, 1);
+                s1 += phif * S.A(i, 2);
+            }
+            gx[k] = inv_vol * s0;
+            gy[k] = inv_vol * s1;
+        }
+    } else {
+        C M[4];   // A^T A
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 2; y++) {
+                C acc = C(0);
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+                    if (fl[i] & 2) acc += S.A(i, x) * S.A(i, y);
+                M[x * 2 + y] = acc;
+            }
+        C det = M[0] * M[3] - M[1] * M[2];
+        const C eps = C(fvmr::kEPS);
+        if (ph.compat_inv) {  // MatrixKernel.h:48-49 as written
+            if (det >= C(0) && det < eps) det = eps;
+            else if (det > -eps) det = -eps;
+        } else {
+            if (det >= C(0) && det < eps) det = eps;
+            else if (det < C(0) && det > -eps) det = -eps;
+        }
+        const C idet = C(1.0) / det;
+        const C i00 = M[3] * idet, i01 = -M[1] * idet, i10 = -M[2] * idet, i11 = M[0] * idet;
+        const bool any = (fl[0] | fl[1] | fl[2]) & 2;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            C a = C(0), b = C(0);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                if (fl[i] & 2) a += (i00 * S.A(i, 0) + i01 * S.A(i, 1)) * (S.Uo(i, k) - Uc[k]);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                if (fl[i] & 2) b += (i10 * S.A(i, 0) + i11 * S.A(i, 1)) * (S.Uo(i, k) - Uc[k]);
+            gx[k] = any ? a : C(0);
+            gy[k] = any ? b : C(0);
+        }
+    }
+    if (ph.limiter != FVMR_LIM_NONE) {
+        C eps2 = C(0);
+        if (ph.limiter == FVMR_LIM_VENKATAKRISHNAN) {
+            const C Kh = (C)ph.venkat_K * sqrt(vol);
+            eps2 = Kh * Kh * Kh;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            C lo = Uc[k], hi = Uc[k];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                if (fl[i] & 2) { lo = tmin(lo, S.Uo(i, k)); hi = tmax(hi, S.Uo(i, k)); }
+            C phi = C(1.0);
+#pragma unroll
+            for (int jn = 0; jn < 3; jn++) {
+                const C ndx = S.N(jn, 0), ndy = S.N(jn, 1);
+                C pn = C(1.0);
+                if (ph.limiter == FVMR_LIM_BARTH) {
+                    const C Un = Uc[k] + gx[k] * ndx + gy[k] * ndy;
+                    C rel = Un - Uc[k];
+                    const C hrel = hi - Uc[k], lrel = lo - Uc[k];
+                    if (rel > C(0)) { rel += C(fvmr::kEPS); pn = tmin(C(1), hrel / rel); }
+                    else if (rel < C(0)) { rel -= C(fvmr::kEPS); pn = tmin(C(1), lrel / rel); }
+                } else {
+                    const C d2 = gx[k] * ndx + gy[k] * ndy;
+                    if (d2 != C(0)) {
+                        const C d1 = (d2 > C(0)) ? (hi - Uc[k]) : (lo - Uc[k]);
+                        const C num = (d1 * d1 + eps2) * d2 + C(2.0) * d2 * d2 * d1;
+                        const C den = d1 * d1 + C(2.0) * d2 * d2 + d1 * d2 + eps2;
+                        pn = num / (den * d2);
+                    }
+                }
+                phi = tmin(phi, pn);
+            }
+            gx[k] *= phi;
+            gy[k] *= phi;
+        }
     }
 }
 
@@ -617,97 +744,15 @@ __global__ __launch_bounds__(kGroup3) FVMR_GRAD_ATTR void k_gradient(Mesh<G> m, 
         int fl[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) fl[i] = sFl[i][lane];
+        struct {
+            const C (*a)[3][kCellsPerGroup], (*u)[4][kCellsPerGroup], (*n)[2][kCellsPerGroup];
+            int lane;
+            __device__ __forceinline__ C A(int i, int q) const { return a[i][q][lane]; }
+            __device__ __forceinline__ C Uo(int i, int k) const { return u[i][k][lane]; }
+            __device__ __forceinline__ C N(int i, int q) const { return n[i][q][lane]; }
+        } slots{sA, sUo, sN, lane};
         C gx[4], gy[4];
-        if (ph.gradient == FVMR_GRA_GREENGAUSS) {
-            const C inv_vol = C(1) / (C)cg.z;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                C s0 = C(0), s1 = C(0);
-#pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    const C g = sA[i][0][lane];
-                    const C phif = (fl[i] & 1) ? g * Uc[k] + (C(1.0) - g) * sUo[i][k][lane] : Uc[k];
-                    s0 += phif * sA[i][1][lane];
-                    s1 += phif * sA[i][2][lane];
-                }
-                gx[k] = inv_vol * s0;
-                gy[k] = inv_vol * s1;
-            }
-        } else {
-            C A[4];
-#pragma unroll
-            for (int x = 0; x < 2; x++)
-#pragma unroll
-                for (int y = 0; y < 2; y++) {
-                    C acc = C(0);
-#pragma unroll
-                    for (int i = 0; i < 3; i++)
-                        if (fl[i] & 2) acc += sA[i][x][lane] * sA[i][y][lane];
-                    A[x * 2 + y] = acc;
-                }
-            C det = A[0] * A[3] - A[1] * A[2];
-            const C eps = C(fvmr::kEPS);
-            if (ph.compat_inv) {  // MatrixKernel.h:48-49 as written
-                if (det >= C(0) && det < eps) det = eps;
-                else if (det > -eps) det = -eps;
-            } else {
-                if (det >= C(0) && det < eps) det = eps;
-                else if (det < C(0) && det > -eps) det = -eps;
-            }
-            const C idet = C(1.0) / det;
-            const C i00 = A[3] * idet, i01 = -A[1] * idet, i10 = -A[2] * idet, i11 = A[0] * idet;
-            const bool any = (fl[0] | fl[1] | fl[2]) & 2;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                C a = C(0), b = C(0);
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-                    if (fl[i] & 2) a += (i00 * sA[i][0][lane] + i01 * sA[i][1][lane]) * (sUo[i][k][lane] - Uc[k]);
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-                    if (fl[i] & 2) b += (i10 * sA[i][0][lane] + i11 * sA[i][1][lane]) * (sUo[i][k][lane] - Uc[k]);
-                gx[k] = any ? a : C(0);
-                gy[k] = any ? b : C(0);
-            }
-        }
-        if (ph.limiter != FVMR_LIM_NONE) {
-            C eps2 = C(0);
-            if (ph.limiter == FVMR_LIM_VENKATAKRISHNAN) {
-                const C Kh = (C)ph.venkat_K * sqrt((C)cg.z);
-                eps2 = Kh * Kh * Kh;
-            }
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                C lo = Uc[k], hi = Uc[k];
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-                    if (fl[i] & 2) { lo = tmin(lo, sUo[i][k][lane]); hi = tmax(hi, sUo[i][k][lane]); }
-                C phi = C(1.0);
-#pragma unroll
-                for (int jn = 0; jn < 3; jn++) {
-                    const C ndx = sN[jn][0][lane], ndy = sN[jn][1][lane];
-                    C pn = C(1.0);
-                    if (ph.limiter == FVMR_LIM_BARTH) {
-                        const C Un = Uc[k] + gx[k] * ndx + gy[k] * ndy;
-                        C rel = Un - Uc[k];
-                        const C hrel = hi - Uc[k], lrel = lo - Uc[k];
-                        if (rel > C(0)) { rel += C(fvmr::kEPS); pn = tmin(C(1), hrel / rel); }
-                        else if (rel < C(0)) { rel -= C(fvmr::kEPS); pn = tmin(C(1), lrel / rel); }
-                    } else {
-                        const C d2 = gx[k] * ndx + gy[k] * ndy;
-                        if (d2 != C(0)) {
-                            const C d1 = (d2 > C(0)) ? (hi - Uc[k]) : (lo - Uc[k]);
-                            const C num = (d1 * d1 + eps2) * d2 + C(2.0) * d2 * d2 * d1;
-                            const C den = d1 * d1 + C(2.0) * d2 * d2 + d1 * d2 + eps2;
-                            pn = num / (den * d2);
-                        }
-                    }
-                    phi = tmin(phi, pn);
-                }
-                gx[k] *= phi;
-                gy[k] *= phi;
-            }
-        }
+        grad_limit<C>(ph, Uc, slots, fl, (C)cg.z, gx, gy);
         Grad<G> out;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -790,6 +835,307 @@ __global__ __launch_bounds__(kGroup3) FVMR_STAGE_ATTR void k_stage(Mesh<G> m, Ph
     if (bad) {
         const int slot = atomicAdd(&sc->bad_count, 1);
         if (slot < nc) bad_list[slot] = c;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tile kernels: one work-group per tile of consecutive cells (tile records built by
+// build_tiles on the host).  The work-group stages the states of the tile and two halo
+// rings, the nodes and the face records in LDS, recomputes centroids and face geometry from
+// the node coordinates with the host's arithmetic, forms the limited gradients of the tile
+// and the first ring in LDS, evaluates the flux of every face of the tile once, and sums
+// the fluxes per cell in slot order.  Gradients never go to HBM.
+// ---------------------------------------------------------------------------
+constexpr int kTileThreads = 256;
+constexpr int kTileMaxG = 256;    // cells with a gradient (tile + first ring)
+constexpr int kTileMaxL = 384;    // cells with a state (+ second ring)
+constexpr int kTileMaxN = 256;    // nodes
+constexpr int kTileMaxF = 384;    // faces
+constexpr int kTileMaxFF = 256;   // faces of tile cells
+constexpr int kTileDefaultCells = 160;
+static_assert(kTileMaxL <= fvmr::kLocalNone && kTileMaxN <= fvmr::kLocalNone && kTileMaxF <= fvmr::kLocalNone,
+              "local ids are 10-bit fields");
+
+template <class G>
+struct TileArgs {
+    const int *info;          // fvmr::kTileInfo ints per tile
+    const int *halo;          // device cell ids of halo cells
+    const V2<double> *nxy;    // node coordinates per tile
+    const int *lnodes;        // per local cell: 3 local node ids
+    const int4 *gslot;        // per gradient cell: 3 slot words + dtcode
+    const G *gvol;            // per gradient cell: volume
+    const int4 *face;         // per face: nodes | BC code, L | R | pair face, float length, edge
+};
+
+__device__ __forceinline__ int lfield(int w, int k) { return (w >> (fvmr::kLocalBits * k)) & fvmr::kLocalMask; }
+
+// Centroid of a cell from its nodes as the host computes it ((0 + x0) + x1 + x2) / 3.
+__device__ __forceinline__ V2<double> tile_centroid(const V2<double> *sN, int w) {
+#pragma clang fp contract(off)
+    const V2<double> a = sN[lfield(w, 0)], b = sN[lfield(w, 1)], c = sN[lfield(w, 2)];
+    double sx = 0.0, sy = 0.0;
+    sx += a.x; sx += b.x; sx += c.x;
+    sy += a.y; sy += b.y; sy += c.y;
+    return V2<double>{sx / 3.0, sy / 3.0};
+}
+
+// Face geometry from a face record as the host computes it: normal (dy, -dx) / L with the
+// stored (float) length L, midpoint (xa + xb) / 2; rounded to G like the global records.
+template <class G>
+__device__ __forceinline__ void tile_face_geom(const V2<double> *sN, int4 fr, V4<G> &eg, V2<G> &em) {
+#pragma clang fp contract(off)
+    const V2<double> a = sN[lfield(fr.x, 0)], b = sN[lfield(fr.x, 1)];
+    const double len = (double)__int_as_float(fr.z);
+    const double dx = b.x - a.x, dy = b.y - a.y;
+    eg = V4<G>{(G)(dy / len), (G)(-dx / len), (G)len, G(0)};
+    em = V2<G>{(G)((a.x + b.x) / 2.0), (G)((a.y + b.y) / 2.0)};
+}
+template <class G>
+__device__ __forceinline__ V2<G> tile_face_mid(const V2<double> *sN, int4 fr) {
+#pragma clang fp contract(off)
+    const V2<double> a = sN[lfield(fr.x, 0)], b = sN[lfield(fr.x, 1)];
+    return V2<G>{(G)((a.x + b.x) / 2.0), (G)((a.y + b.y) / 2.0)};
+}
+__device__ __forceinline__ int tile_btype(int4 fr) {
+    const int bc = (fr.x >> fvmr::kBcShift) & 127;
+    return bc ? bc + fvmr::kBcBase : -1;
+}
+
+template <class G>
+struct TileCells {
+    const S4 *sU;
+    const V2<G> *sC;
+    const Grad<G> *sGr;
+    __device__ __forceinline__ void state(int c, double u[4]) const { ld(sU, c, u); }
+    __device__ __forceinline__ V2<G> centroid(int c) const { return sC[c]; }
+    __device__ __forceinline__ Grad<G> gradient(int c) const { return sGr[c]; }
+    __device__ __forceinline__ bool has(int c) const { return c >= 0; }
+};
+
+// One RK stage over the tiles (the work of k_gradient + k_stage, see there).
+template <class C, class G, bool NS>
+__global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs<G> ta, Phys ph, const S4 *__restrict__ Uin,
+                                                        const S4 *__restrict__ Un, S4 *__restrict__ Uout,
+                                                        S4 *__restrict__ rhs_out, Scalars *__restrict__ sc,
+                                                        int *__restrict__ bad_list, int nc, double a, double b,
+                                                        int first) {
+    __shared__ S4 sU[kTileMaxL];
+    __shared__ V2<double> sN[kTileMaxN];
+    __shared__ V2<G> sC[kTileMaxL];
+    __shared__ int4 sFace[kTileMaxF];
+    __shared__ Grad<G> sGr[kTileMaxG];
+    __shared__ V4<C> sFl[kTileMaxFF];
+    const int tid = threadIdx.x;
+    const int *ti = ta.info + (size_t)xcd_block() * fvmr::kTileInfo;
+    const int c0 = ti[fvmr::kTiC0], nT = ti[fvmr::kTiNT], nG = ti[fvmr::kTiNG], nL = ti[fvmr::kTiNL];
+    const int nN = ti[fvmr::kTiNN], nF = ti[fvmr::kTiNF], nFF = ti[fvmr::kTiNFF];
+    const int offH = ti[fvmr::kTiOffH], offN = ti[fvmr::kTiOffN], offF = ti[fvmr::kTiOffF];
+    const int offL = ti[fvmr::kTiOffL], offG = ti[fvmr::kTiOffG];
+    // 1. stage nodes, states and face records
+    for (int i = tid; i < nN; i += kTileThreads) sN[i] = ta.nxy[offN + i];
+    for (int i = tid; i < nL; i += kTileThreads) sU[i] = Uin[i < nT ? c0 + i : ta.halo[offH + i - nT]];
+    for (int i = tid; i < nF; i += kTileThreads) sFace[i] = ta.face[offF + i];
+    __syncthreads();
+    // 2. centroids
+    for (int i = tid; i < nL; i += kTileThreads) {
+        const V2<double> ct = tile_centroid(sN, ta.lnodes[offL + i]);
+        sC[i] = V2<G>{(G)ct.x, (G)ct.y};
+    }
+    __syncthreads();
+    // 3. limited gradients of the tile and the first ring
+    for (int i = tid; i < nG; i += kTileThreads) {
+        const int4 gs = ta.gslot[offG + i];
+        const int sw[3] = {gs.x, gs.y, gs.z};
+        const C vol = (C)ta.gvol[offG + i];
+        const int nw = ta.lnodes[offL + i];
+        double Ud[4];
+        ld(sU, i, Ud);
+        C Uc[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) Uc[k] = (C)Ud[k];
+        const C xc = (C)sC[i].x, yc = (C)sC[i].y;
+        struct {
+            C a[3][3], u[3][4], n[3][2];
+            __device__ __forceinline__ C A(int j, int q) const { return a[j][q]; }
+            __device__ __forceinline__ C Uo(int j, int k) const { return u[j][k]; }
+            __device__ __forceinline__ C N(int j, int q) const { return n[j][q]; }
+        } sl;
+        int fl[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int f = sw[j] & fvmr::kLocalMask;
+            const int out = (sw[j] >> fvmr::kGsOutShift) & fvmr::kLocalMask;
+            const bool has_out = out != fvmr::kLocalNone;
+            const bool periodic = (sw[j] & fvmr::kGsPeriodic) != 0;
+            const C sgn = (sw[j] & fvmr::kGsIsR) ? C(-1) : C(1);
+            const int4 fr = sFace[f];
+            V4<G> eg;
+            V2<G> em;
+            tile_face_geom<G>(sN, fr, eg, em);
+            double Uo[4];
+            ld(sU, has_out ? out : i, Uo);
+#pragma unroll
+            for (int k = 0; k < 4; k++) sl.u[j][k] = (C)Uo[k];
+            const V2<double> nd = sN[lfield(nw, j)];
+            sl.n[j][0] = (C)(G)nd.x - xc;
+            sl.n[j][1] = (C)(G)nd.y - yc;
+            if (ph.gradient == FVMR_GRA_GREENGAUSS) {
+                const C len = (C)eg.z;
+                C gC = C(0.5);
+                if (has_out) {
+                    const C fx = (C)em.x, fy = (C)em.y;
+                    C Fx = (C)sC[out].x, Fy = (C)sC[out].y;
+                    const int pl = lfield(fr.y, 2);
+                    if (periodic && pl != fvmr::kLocalNone) {
+                        const V2<G> pm = tile_face_mid<G>(sN, sFace[pl]);
+                        Fx += fx - (C)pm.x;
+                        Fy += fy - (C)pm.y;
+                    }
+                    const C ax = fx - Fx, ay = fy - Fy, bx = xc - Fx, by = yc - Fy;
+                    const C dFf = sqrt(ax * ax + ay * ay);
+                    C dFC = sqrt(bx * bx + by * by);
+                    if (dFC < C(fvmr::kEPS)) dFC = C(fvmr::kEPS);
+                    gC = dFf / dFC;
+                }
+                sl.a[j][0] = gC;
+                sl.a[j][1] = (C)eg.x * len * sgn;
+                sl.a[j][2] = (C)eg.y * len * sgn;
+            } else {
+                C dxn = C(0), dyn = C(0);
+                if (has_out && !periodic) {   // element neighbours exclude periodic faces
+                    dxn = (C)sC[out].x - xc;
+                    dyn = (C)sC[out].y - yc;
+                }
+                sl.a[j][0] = dxn;
+                sl.a[j][1] = dyn;
+                sl.a[j][2] = C(0);
+            }
+            fl[j] = (has_out ? 1 : 0) | ((has_out && !periodic) ? 2 : 0);
+        }
+        C gx[4], gy[4];
+        grad_limit<C>(ph, Uc, sl, fl, vol, gx, gy);
+        Grad<G> g;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            g.x[k] = (G)gx[k];
+            g.y[k] = (G)gy[k];
+        }
+        sGr[i] = g;
+    }
+    __syncthreads();
+    // 4. one canonical flux per face of the tile
+    const double t_rk = sc->t_rk;
+    const TileCells<G> cells{sU, sC, sGr};
+    for (int f = tid; f < nFF; f += kTileThreads) {
+        const int4 fr = sFace[f];
+        V4<G> eg;
+        V2<G> em;
+        tile_face_geom<G>(sN, fr, eg, em);
+        const int L = lfield(fr.y, 0), R = lfield(fr.y, 1), pl = lfield(fr.y, 2);
+        const int bType = tile_btype(fr);
+        const bool has_pair = pl != fvmr::kLocalNone;
+        const V2<G> pm = (bType != -1 && has_pair) ? tile_face_mid<G>(sN, sFace[pl]) : em;
+        C F[4];
+        face_flux<C, G, NS>(cells, ph, t_rk, L, R == fvmr::kLocalNone ? -1 : R, bType, has_pair, eg, em, pm, F);
+        sFl[f] = V4<C>{F[0], F[1], F[2], F[3]};
+    }
+    __syncthreads();
+    // 5. cell-owned sum in slot order, /(-V), RK update (as k_stage)
+    const double dt = sc->dt;
+    for (int i = tid; i < nT; i += kTileThreads) {
+        const int c = c0 + i;
+        const int4 gs = ta.gslot[offG + i];
+        const int sw[3] = {gs.x, gs.y, gs.z};
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const V4<C> fv = sFl[sw[j] & fvmr::kLocalMask];
+            const double s = (sw[j] & fvmr::kGsIsR) ? -1.0 : 1.0;
+            acc[0] += s * (double)fv.x;
+            acc[1] += s * (double)fv.y;
+            acc[2] += s * (double)fv.z;
+            acc[3] += s * (double)fv.w;
+        }
+        const double w = -1.0 * (double)ta.gvol[offG + i];
+        double Ui[4], Ub[4] = {0, 0, 0, 0}, r[4], Unew[4];
+        ld(sU, i, Ui);
+        if (!first) ld(Un, c, Ub);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            r[k] = acc[k] / w;
+            const double s = Ui[k] + dt * r[k];
+            Unew[k] = first ? s : a * Ub[k] + b * s;
+        }
+        if (rhs_out) st(rhs_out, c, r);
+        st(Uout, c, Unew);
+        double rv[4];
+        prim(Unew, rv, ph.gamma);
+        bool bad = out_of_range(rv[0], rv[3]);
+#pragma unroll
+        for (int k = 0; k < 4; k++) bad = bad || isnan(rv[k]);
+        if (bad) {
+            const int slot = atomicAdd(&sc->bad_count, 1);
+            if (slot < nc) bad_list[slot] = c;
+        }
+    }
+}
+
+// Block minima of CFL V / alphaC over the tiles (the dt part of k_gradient<DT>).
+template <class G>
+__global__ __launch_bounds__(kTileThreads) void k_tile_dt(TileArgs<G> ta, Phys ph, const S4 *__restrict__ U,
+                                                           double *__restrict__ partial) {
+    __shared__ S4 sU[kTileMaxG];
+    __shared__ V2<double> sN[kTileMaxN];
+    __shared__ double sMin[kTileThreads / 64];
+    const int tid = threadIdx.x;
+    const int blk = xcd_block();
+    const int *ti = ta.info + (size_t)blk * fvmr::kTileInfo;
+    const int c0 = ti[fvmr::kTiC0], nT = ti[fvmr::kTiNT], nG = ti[fvmr::kTiNG];
+    const int nN = ti[fvmr::kTiNN];
+    const int offH = ti[fvmr::kTiOffH], offN = ti[fvmr::kTiOffN], offF = ti[fvmr::kTiOffF], offG = ti[fvmr::kTiOffG];
+    for (int i = tid; i < nN; i += kTileThreads) sN[i] = ta.nxy[offN + i];
+    for (int i = tid; i < nG; i += kTileThreads) sU[i] = U[i < nT ? c0 + i : ta.halo[offH + i - nT]];
+    __syncthreads();
+    double dtc = 1e10;
+    for (int i = tid; i < nT; i += kTileThreads) {
+        const int4 gs = ta.gslot[offG + i];
+        const int sw[3] = {gs.x, gs.y, gs.z};
+        const int code = gs.w;
+        const double vol_c = (double)ta.gvol[offG + i];
+        double Uc[4], rc[4];
+        ld(sU, i, Uc);
+        prim(Uc, rc, ph.gamma);
+        // alphaC in the reference edge order: count (3 bits) + 6 x (slot 2 bits, pair 1 bit)
+        const int cnt = code & 7;
+        double alpha_c = 0.0;
+        for (int q = 0; q < cnt; q++) {
+            const int ent = (code >> (3 + 3 * q)) & 7;
+            const int w = sw[ent & 3];
+            const int out = (w >> fvmr::kGsOutShift) & fvmr::kLocalMask;
+            const bool has_out = out != fvmr::kLocalNone;
+            int4 fr = ta.face[offF + (w & fvmr::kLocalMask)];
+            if (ent >> 2) fr = ta.face[offF + lfield(fr.y, 2)];
+            V4<G> eg;
+            V2<G> em;
+            tile_face_geom<G>(sN, fr, eg, em);
+            double Uo[4], ro[4];
+            ld(sU, has_out ? out : i, Uo);
+            prim(Uo, ro, ph.gamma);
+            double av, vv;
+            edge_alpha(ph, rc, ro, (ent >> 2) ? true : has_out, (double)eg.x, (double)eg.y, (double)eg.z, vol_c, &av,
+                       &vv);
+            alpha_c += av;
+            if (ph.equation == FVMR_EQ_NS) alpha_c += vv;
+        }
+        dtc = fmin(dtc, ph.CFL * vol_c / alpha_c);
+    }
+    const double wm = wave_min(dtc);
+    if ((tid & 63) == 0) sMin[tid >> 6] = wm;
+    __syncthreads();
+    if (tid == 0) {
+        double m = sMin[0];
+        for (int w = 1; w < kTileThreads / 64; w++) m = fmin(m, sMin[w]);
+        partial[blk] = m;
     }
 }
 
@@ -982,6 +1328,13 @@ struct fvmr_solver {
     double *partial = nullptr, *norm_out = nullptr;
     int *bad_list = nullptr;
     Scalars *sc = nullptr;
+    // tile kernels (fvmr_tiles.h)
+    bool tiles = false;
+    int ntiles = 0;
+    int *t_info = nullptr, *t_halo = nullptr, *t_lnodes = nullptr;
+    V2<double> *t_nxy = nullptr;
+    int4 *t_gslot = nullptr, *t_face = nullptr;
+    void *t_gvol = nullptr;
     bool rhs_valid = false;
     bool profiling = false;
     Prof prof[kNumProf];
@@ -1002,6 +1355,14 @@ Mesh<G> mesh_args(const fvmr_solver *h) {
     m.eint = h->eint;
     m.dtcode = h->dtcode;
     return m;
+}
+
+template <class G>
+TileArgs<G> tile_args(const fvmr_solver *h) {
+    TileArgs<G> t;
+    t.info = h->t_info; t.halo = h->t_halo; t.nxy = h->t_nxy; t.lnodes = h->t_lnodes;
+    t.gslot = h->t_gslot; t.gvol = (const G *)h->t_gvol; t.face = h->t_face;
+    return t;
 }
 
 int prof_begin(fvmr_solver *h, int k, hipEvent_t *out) {
@@ -1079,6 +1440,58 @@ int launch_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, double a
     return prof_end(h, 3, e0);
 }
 
+// Global dt over the tiles (dt_mode as launch_gradient).
+template <class C, class G>
+int launch_tile_dt(fvmr_solver *h, const S4 *U, int dt_mode) {
+    hipEvent_t e0{};
+    int rc = prof_begin(h, 1, &e0);
+    if (rc) return rc;
+    k_tile_dt<G><<<h->ntiles, kTileThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, U, h->partial);
+    HIP_OK(hipGetLastError());
+    if ((rc = prof_end(h, 1, e0))) return rc;
+    if ((rc = prof_begin(h, 2, &e0))) return rc;
+    k_dt_finish<<<1, 1024, 0, h->stream>>>(h->ntiles, h->partial, h->sc, dt_mode);
+    HIP_OK(hipGetLastError());
+    return prof_end(h, 2, e0);
+}
+
+template <class C, class G>
+int launch_tile_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, double a, double b, int first,
+                      S4 *rhs_out) {
+    hipEvent_t e0{};
+    int rc = prof_begin(h, 0, &e0);
+    if (rc) return rc;
+    if (h->ph.equation == FVMR_EQ_NS)
+        k_tile<C, G, true><<<h->ntiles, kTileThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, Uin, Un, Uout, rhs_out,
+                                                                     h->sc, h->bad_list, h->nc, a, b, first);
+    else
+        k_tile<C, G, false><<<h->ntiles, kTileThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, Uin, Un, Uout, rhs_out,
+                                                                      h->sc, h->bad_list, h->nc, a, b, first);
+    HIP_OK(hipGetLastError());
+    if ((rc = prof_end(h, 0, e0))) return rc;
+    if ((rc = prof_begin(h, 3, &e0))) return rc;
+    k_restrict<G><<<1, 1024, 0, h->stream>>>(mesh_args<G>(h), Uout, h->sc, h->bad_list, h->fix);
+    HIP_OK(hipGetLastError());
+    return prof_end(h, 3, e0);
+}
+
+// One step on the device with the tile kernels.
+template <class C, class G>
+int device_step_tiles(fvmr_solver *h, bool store_rhs) {
+    int rc;
+    const int a = h->cur, b = (a + 1) % 3, c = (a + 2) % 3;
+    S4 *Ua = h->Ubuf[a], *Ub = h->Ubuf[b], *Uc = h->Ubuf[c];
+    if ((rc = launch_tile_dt<C, G>(h, Ua, 1))) return rc;
+    if ((rc = launch_tile_stage<C, G>(h, Ua, Ua, Ub, 0.0, 1.0, 1, store_rhs ? h->rhs : nullptr))) return rc;
+    if (h->p.time_advance == FVMR_EVO_RK3) {
+        if ((rc = launch_tile_stage<C, G>(h, Ub, Ua, Uc, 0.75, 0.25, 0, nullptr))) return rc;
+        if ((rc = launch_tile_stage<C, G>(h, Uc, Ua, Ub, 1.0 / 3.0, 2.0 / 3.0, 0, nullptr))) return rc;
+    }
+    h->cur = b;
+    if (store_rhs) h->rhs_valid = true;
+    return FVMR_OK;
+}
+
 // One step on the device (no host synchronisation).  store_rhs: keep stage-1 rhs.
 template <class C, class G>
 int device_step(fvmr_solver *h, bool store_rhs) {
@@ -1101,6 +1514,7 @@ int device_step(fvmr_solver *h, bool store_rhs) {
 }
 
 int run_step(fvmr_solver *h, bool store_rhs) {
+    if (h->tiles) return h->mixed ? device_step_tiles<float, float>(h, store_rhs) : device_step_tiles<double, double>(h, store_rhs);
     if (h->mixed) return device_step<float, float>(h, store_rhs);
     return device_step<double, double>(h, store_rhs);
 }
@@ -1132,7 +1546,8 @@ int build_geometry(fvmr_solver *h, const fvmr_mesh_soa *m, const std::vector<int
 void free_all(fvmr_solver *h) {
     for (auto &b : h->Ubuf) if (b) (void)hipFree(b);
     void *ptrs[] = {h->grad, h->rhs, h->fix, h->cgeo, h->nxy, h->egeo, h->emid, h->cslot, h->cnode, h->dtcode,
-                    h->eint, h->partial, h->norm_out, h->bad_list, h->sc};
+                    h->eint, h->partial, h->norm_out, h->bad_list, h->sc, h->t_info, h->t_halo, h->t_lnodes,
+                    h->t_nxy, h->t_gslot, h->t_face, h->t_gvol};
     for (void *q : ptrs) if (q) (void)hipFree(q);
     for (int k = 0; k < kNumProf; k++)
         for (auto e : h->pending[k]) (void)hipEventDestroy(e);
@@ -1151,6 +1566,10 @@ int validate_params(const fvmr_params *p) {
         return set_error(FVMR_ERR_UNSUPPORTED, "invalid temporal_choice");
     if (p->precision != FVMR_PREC_FP64 && p->precision != FVMR_PREC_MIXED) return set_error(FVMR_ERR_INVALID_ARG, "invalid precision");
     if (!(p->gamma > 1.0) || !(p->CFL > 0.0)) return set_error(FVMR_ERR_INVALID_ARG, "invalid gamma or CFL");
+    if (p->renumber < FVMR_RENUMBER_NONE || p->renumber > FVMR_RENUMBER_HILBERT)
+        return set_error(FVMR_ERR_INVALID_ARG, "invalid renumber");
+    if (p->kernels < FVMR_KERNELS_AUTO || p->kernels > FVMR_KERNELS_TILE) return set_error(FVMR_ERR_INVALID_ARG, "invalid kernels");
+    if (p->tile_cells < 0 || p->tile_cells > kTileMaxG) return set_error(FVMR_ERR_INVALID_ARG, "tile_cells out of [0, 256]");
     return FVMR_OK;
 }
 
@@ -1214,7 +1633,9 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
     ph.gradient = p->gradient; ph.limiter = p->limiter; ph.mu_model = p->viscosity_model; ph.compat_inv = p->compat_inv2x2;
     // cell permutation
     h->perm.resize(nc);
-    if (p->renumber) {
+    if (p->renumber == FVMR_RENUMBER_HILBERT) {
+        h->perm = fvmr::hilbert_order(nc, m->element_xy[0], m->element_xy[1]);
+    } else if (p->renumber == FVMR_RENUMBER_RCM) {
         std::vector<int> off(nc + 1, 0), adj;
         adj.reserve(3 * (size_t)nc);
         for (int c = 0; c < nc; c++) {
@@ -1290,11 +1711,60 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
         }
         for (int i = 0; i < nc; i++) dtcode[i] |= cnt[i];
     }
+    // tile records
+    fvmr::TileMesh tm;
+    if (p->kernels != FVMR_KERNELS_CELL) {
+        const char *why = nullptr;
+        if (!fvmr::standard_geometry(m, &why)) {
+            if (p->kernels == FVMR_KERNELS_TILE) return bail(set_error(FVMR_ERR_UNSUPPORTED, std::string("tile kernels: ") + why));
+        } else {
+            std::vector<int> sx(3 * (size_t)nc), sy(3 * (size_t)nc), eL(nf), eR(nf), et(nf), ep(nf), en0(nf), en1(nf);
+            std::vector<double> elen(nf), vol(nc);
+            for (size_t k = 0; k < sx.size(); k++) { sx[k] = cslot[k].x; sy[k] = cslot[k].y; }
+            for (int e = 0; e < nf; e++) {
+                eL[e] = eint[e].x; eR[e] = eint[e].y; et[e] = eint[e].z; ep[e] = eint[e].w;
+                en0[e] = m->edge_nodes[0][eperm[e]]; en1[e] = m->edge_nodes[1][eperm[e]];
+                elen[e] = m->edge_length[eperm[e]];
+            }
+            for (int i = 0; i < nc; i++) vol[i] = m->element_volume[h->perm[i]];
+            fvmr::TileInput ti{nc, nf, sx.data(), sy.data(), cnode.data(), eL.data(), eR.data(), et.data(), ep.data(),
+                               en0.data(), en1.data(), elen.data(), m->node_xy[0], m->node_xy[1], vol.data(), dtcode.data()};
+            const fvmr::TileCaps caps{p->tile_cells ? p->tile_cells : kTileDefaultCells, kTileMaxG, kTileMaxL, kTileMaxN,
+                                      kTileMaxF, kTileMaxFF};
+            if ((rc = fvmr::build_tiles(ti, caps, tm))) return bail(rc);
+            h->tiles = true;
+            h->ntiles = tm.ntiles;
+            std::vector<V2<double>> txy(tm.nxy.size() / 2);
+            for (size_t k = 0; k < txy.size(); k++) txy[k] = V2<double>{tm.nxy[2 * k], tm.nxy[2 * k + 1]};
+            std::vector<int4> tgs(tm.gslot.size() / 4), tfc(tm.face.size() / 4);
+            for (size_t k = 0; k < tgs.size(); k++)
+                tgs[k] = make_int4(tm.gslot[4 * k], tm.gslot[4 * k + 1], tm.gslot[4 * k + 2], tm.gslot[4 * k + 3]);
+            for (size_t k = 0; k < tfc.size(); k++)
+                tfc[k] = make_int4(tm.face[4 * k], tm.face[4 * k + 1], tm.face[4 * k + 2], tm.face[4 * k + 3]);
+            if ((rc = upload(&h->t_info, tm.info, h->stream)) || (rc = upload(&h->t_halo, tm.halo, h->stream)) ||
+                (rc = upload(&h->t_lnodes, tm.lnodes, h->stream)) || (rc = upload(&h->t_nxy, txy, h->stream)) ||
+                (rc = upload(&h->t_gslot, tgs, h->stream)) || (rc = upload(&h->t_face, tfc, h->stream)))
+                return bail(rc);
+            std::vector<float> gv;
+            if (h->mixed) {
+                gv.assign(tm.gvol.begin(), tm.gvol.end());
+                float *d = nullptr;
+                if ((rc = upload(&d, gv, h->stream))) return bail(rc);
+                h->t_gvol = d;
+            } else {
+                double *d = nullptr;
+                if ((rc = upload(&d, tm.gvol, h->stream))) return bail(rc);
+                h->t_gvol = d;
+            }
+            // the host vectors of these uploads end with this block
+            if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(set_error(FVMR_ERR_HIP, "tile upload failed"));
+        }
+    }
     // device memory
     const size_t gsz = h->mixed ? sizeof(Grad<float>) : sizeof(Grad<double>);
     for (int k = 0; k < 3; k++)
         if (hipMalloc((void **)&h->Ubuf[k], (size_t)nc * sizeof(S4)) != hipSuccess) return bail(set_error(FVMR_ERR_HIP, "hipMalloc U"));
-    const int nparts = std::max(grid3_of(nc), 4 * grid_of(nc));
+    const int nparts = std::max(std::max(grid3_of(nc), 4 * grid_of(nc)), h->ntiles);
     if (hipMalloc(&h->grad, (size_t)nc * gsz) != hipSuccess ||
         hipMalloc((void **)&h->rhs, (size_t)nc * sizeof(S4)) != hipSuccess ||
         hipMalloc((void **)&h->fix, (size_t)nc * sizeof(S4)) != hipSuccess ||
@@ -1403,7 +1873,11 @@ int fvmr_stage_rhs(fvmr_handle *h, double *const rhs[4]) {
     z.t_rk = s.t;
     HIP_OK(hipMemcpyAsync(h->sc, &z, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
     const int a = h->cur, b = (a + 1) % 3;
-    if (h->mixed) {
+    if (h->tiles) {
+        rc = h->mixed ? launch_tile_stage<float, float>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs)
+                      : launch_tile_stage<double, double>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs);
+        if (rc) return rc;
+    } else if (h->mixed) {
         if ((rc = launch_gradient<float, float>(h, h->Ubuf[a], -1)) ||
             (rc = launch_stage<float, float>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs))) return rc;
     } else {
@@ -1445,7 +1919,12 @@ int fvmr_gradient(fvmr_handle *h, double *const Ux[4], double *const Uy[4]) {
 
 int fvmr_global_dt(fvmr_handle *h, double *dt) {
     if (!h || !dt) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
-    int rc = h->mixed ? launch_gradient<float, float>(h, h->Ubuf[h->cur], 0)
+    int rc;
+    if (h->tiles)
+        rc = h->mixed ? launch_tile_dt<float, float>(h, h->Ubuf[h->cur], 0)
+                      : launch_tile_dt<double, double>(h, h->Ubuf[h->cur], 0);
+    else
+        rc = h->mixed ? launch_gradient<float, float>(h, h->Ubuf[h->cur], 0)
                       : launch_gradient<double, double>(h, h->Ubuf[h->cur], 0);
     if (rc) return rc;
     Scalars s;

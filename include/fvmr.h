@@ -60,6 +60,20 @@ extern "C" {
 #define FVMR_BC_SYMMETRY        6021
 #define FVMR_BC_PERIODIC_0      6100 /* 6100..6109 */
 
+/* Device cell order (results always cross the ABI in the caller's order). */
+#define FVMR_RENUMBER_NONE      0    /* the caller's order */
+#define FVMR_RENUMBER_RCM       1    /* reverse Cuthill-McKee */
+#define FVMR_RENUMBER_HILBERT   2    /* Hilbert curve over the centroids (default) */
+
+/* Device kernel family.  TILE: one work-group per tile of cells, gradients kept in LDS,
+ * every face flux evaluated once; it needs the standard geometry (centroids, midpoints,
+ * normals recomputable from the nodes, float-valued lengths, as fvmr_mesh_* produce).
+ * CELL: one gradient and one stage kernel per RK stage over packed per-cell records.
+ * AUTO: TILE when the geometry allows it, else CELL.  Both give the same results. */
+#define FVMR_KERNELS_AUTO       0
+#define FVMR_KERNELS_CELL       1
+#define FVMR_KERNELS_TILE       2
+
 /* precision of the device path */
 #define FVMR_PREC_FP64        0     /* plain: every array and operation FP64 */
 #define FVMR_PREC_MIXED       1     /* FP32 flux/reconstruction/limiter, FP64
@@ -132,8 +146,10 @@ typedef struct fvmr_params {
     int    time_advance;     /* FVMR_EVO_* */
     int    viscosity_model;  /* FVMR_MU_* */
     int    precision;        /* FVMR_PREC_* */
-    int    renumber;         /* 1: reverse Cuthill-McKee cell order on the device */
+    int    renumber;         /* device cell order: FVMR_RENUMBER_* */
     int    compat_inv2x2;    /* 1: reproduce the reference inv_2x2 determinant bug (LSQ) */
+    int    kernels;          /* FVMR_KERNELS_*: device kernel family */
+    int    tile_cells;       /* cells per tile of the tile kernels (0: default 160, at most 256) */
     double inf_ruvp[4];      /* boundaryCondition.2D.inf    (rho,u,v,p) */
     double inlet_ruvp[4];    /* boundaryCondition.2D.inlet  */
     double outlet_ruvp[4];   /* boundaryCondition.2D.outlet */

@@ -53,3 +53,38 @@ def cylinder_case(golden_dir):
     _, _, elems, _ = read_pause(golden_dir / "pause_cylinder_allfar_50_20000.dat")
     U = np.ascontiguousarray(elems[:, 4:8].T)
     return m, p, U
+
+
+class SoaMesh:
+    """A caller-built fvmr_mesh_soa over numpy arrays (the layout soa_arrays returns), as a
+    host code that fills the reference's SoA itself would pass it to fvmr_create."""
+
+    def __init__(self, arrays):
+        import ctypes as C
+        self.arrays = {k: np.ascontiguousarray(v) for k, v in arrays.items()}
+        a = self.arrays
+        s = F.MeshSoA()
+        s.num_node = a["node_xy"].shape[1]
+        s.num_element = a["element_volume"].shape[0]
+        s.num_edge = a["edge_length"].shape[0]
+        s.num_boundary_set = a["boundary_type"].shape[0]
+
+        def dp(x):
+            return x.ctypes.data_as(C.POINTER(C.c_double))
+
+        def ip(x):
+            return x.ctypes.data_as(C.POINTER(C.c_int))
+
+        for k in range(2):
+            s.node_xy[k] = dp(a["node_xy"][k]); s.element_xy[k] = dp(a["element_xy"][k])
+            s.edge_nodes[k] = ip(a["edge_nodes"][k]); s.edge_xy[k] = dp(a["edge_xy"][k])
+            s.edge_normal[k] = dp(a["edge_normal"][k])
+        for k in range(4):
+            s.element_nodes[k] = ip(a["element_nodes"][k]); s.element_edges[k] = ip(a["element_edges"][k])
+            s.element_neighbors[k] = ip(a["element_neighbors"][k])
+        s.element_volume = dp(a["element_volume"]); s.edge_setID = ip(a["edge_setID"])
+        s.edge_periodicPair = ip(a["edge_periodicPair"]); s.edge_elementL = ip(a["edge_elementL"])
+        s.edge_elementR = ip(a["edge_elementR"]); s.edge_length = dp(a["edge_length"])
+        s.boundary_type = ip(a["boundary_type"])
+        self.soa = s
+        self.num_element = s.num_element

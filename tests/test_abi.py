@@ -29,8 +29,8 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_struct_sizes_match_header_layout():
-    # fvmr_params: 7 doubles, 10 ints, 3x4 doubles, 5 doubles
-    assert C.sizeof(F.Params) == 7 * 8 + 10 * 4 + 12 * 8 + 5 * 8
+    # fvmr_params: 7 doubles, 12 ints, 3x4 doubles, 5 doubles
+    assert C.sizeof(F.Params) == 7 * 8 + 12 * 4 + 12 * 8 + 5 * 8
     assert C.sizeof(F.StepInfo) == 6 * 8 + 4 * 4
     assert C.sizeof(F.MeshSoA) == 16 + 8 * (2 + 4 * 3 + 2 + 1 + 2 + 5 + 2 + 2 + 1)
 
@@ -40,6 +40,7 @@ def test_params_default_is_the_reference_globalpara():
     assert (p.gamma, p.R, p.Pr, p.T0, p.mu0) == (1.4, 287.06, 0.73, 288.16, 17.9e-6)
     assert p.flux_scheme == F.SOL_ROE and p.equation == F.EQ_EULER
     assert p.time_advance == F.EVO_EXPLICIT and p.Pr_viscous == 0.72 and p.dsr_shock_speed == 20.0
+    assert (p.renumber, p.kernels, p.tile_cells) == (F.RENUMBER_HILBERT, F.KERNELS_AUTO, 0)
 
 
 def test_ruvp_from_mach_closed_form():
