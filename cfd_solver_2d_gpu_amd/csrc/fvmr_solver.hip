@@ -611,22 +611,45 @@ __device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[4], const 
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
+            // bounds over the cell and its non-periodic neighbours (fmin/fmax: a NaN bound
+            // already makes the Green-Gauss or least-squares gradient NaN)
             C lo = Uc[k], hi = Uc[k];
 #pragma unroll
             for (int i = 0; i < 3; i++)
-                if (fl[i] & 2) { lo = tmin(lo, S.Uo(i, k)); hi = tmax(hi, S.Uo(i, k)); }
+                if (fl[i] & 2) { lo = fmin(lo, S.Uo(i, k)); hi = fmax(hi, S.Uo(i, k)); }
             C phi = C(1.0);
+            const C hrel = hi - Uc[k], lrel = lo - Uc[k];
+            if (ph.limiter == FVMR_LIM_BARTH && !isnan(hrel) && !isnan(lrel)) {
+                // min over vertices of min(1, hrel / (rel + eps)) (rel > 0) and of
+                // min(1, lrel / (rel - eps)) (rel < 0): division rounds monotonically, so each
+                // minimum is one division by the extreme denominator, needed only when the
+                // quotient can fall below 1
+                C dpos = C(0), dneg = C(0);
 #pragma unroll
-            for (int jn = 0; jn < 3; jn++) {
-                const C ndx = S.N(jn, 0), ndy = S.N(jn, 1);
-                C pn = C(1.0);
-                if (ph.limiter == FVMR_LIM_BARTH) {
-                    const C Un = Uc[k] + gx[k] * ndx + gy[k] * ndy;
+                for (int jn = 0; jn < 3; jn++) {
+                    const C Un = Uc[k] + gx[k] * S.N(jn, 0) + gy[k] * S.N(jn, 1);
+                    const C rel = Un - Uc[k];
+                    if (rel > C(0)) dpos = fmax(dpos, rel + C(fvmr::kEPS));
+                    else if (rel < C(0)) dneg = fmin(dneg, rel - C(fvmr::kEPS));
+                }
+                if (dpos > C(0) && hrel < dpos) phi = tmin(phi, hrel / dpos);
+                if (dneg < C(0) && lrel > dneg) phi = tmin(phi, lrel / dneg);
+            } else if (ph.limiter == FVMR_LIM_BARTH) {
+                // NaN bounds: the vertex loop as the reference writes it
+#pragma unroll
+                for (int jn = 0; jn < 3; jn++) {
+                    const C Un = Uc[k] + gx[k] * S.N(jn, 0) + gy[k] * S.N(jn, 1);
                     C rel = Un - Uc[k];
-                    const C hrel = hi - Uc[k], lrel = lo - Uc[k];
+                    C pn = C(1.0);
                     if (rel > C(0)) { rel += C(fvmr::kEPS); pn = tmin(C(1), hrel / rel); }
                     else if (rel < C(0)) { rel -= C(fvmr::kEPS); pn = tmin(C(1), lrel / rel); }
-                } else {
+                    phi = tmin(phi, pn);
+                }
+            } else {
+#pragma unroll
+                for (int jn = 0; jn < 3; jn++) {
+                    const C ndx = S.N(jn, 0), ndy = S.N(jn, 1);
+                    C pn = C(1.0);
                     const C d2 = gx[k] * ndx + gy[k] * ndy;
                     if (d2 != C(0)) {
                         const C d1 = (d2 > C(0)) ? (hi - Uc[k]) : (lo - Uc[k]);
@@ -634,8 +657,8 @@ __device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[4], const 
                         const C den = d1 * d1 + C(2.0) * d2 * d2 + d1 * d2 + eps2;
                         pn = num / (den * d2);
                     }
+                    phi = tmin(phi, pn);
                 }
-                phi = tmin(phi, pn);
             }
             gx[k] *= phi;
             gy[k] *= phi;
@@ -846,13 +869,13 @@ __global__ __launch_bounds__(kGroup3) FVMR_STAGE_ATTR void k_stage(Mesh<G> m, Ph
 // and the first ring in LDS, evaluates the flux of every face of the tile once, and sums
 // the fluxes per cell in slot order.  Gradients never go to HBM.
 // ---------------------------------------------------------------------------
-constexpr int kTileThreads = 256;
-constexpr int kTileMaxG = 256;    // cells with a gradient (tile + first ring)
-constexpr int kTileMaxL = 384;    // cells with a state (+ second ring)
-constexpr int kTileMaxN = 256;    // nodes
-constexpr int kTileMaxF = 384;    // faces
-constexpr int kTileMaxFF = 256;   // faces of tile cells
-constexpr int kTileDefaultCells = 160;
+constexpr int kTileThreads = 512;
+constexpr int kTileMaxG = 400;    // cells with a gradient (tile + first ring)
+constexpr int kTileMaxL = 480;    // cells with a state (+ second ring)
+constexpr int kTileMaxN = 320;    // nodes
+constexpr int kTileMaxF = 672;    // faces
+constexpr int kTileMaxFF = kTileThreads;   // faces of tile cells: one per thread
+constexpr int kTileDefaultCells = 300;
 static_assert(kTileMaxL <= fvmr::kLocalNone && kTileMaxN <= fvmr::kLocalNone && kTileMaxF <= fvmr::kLocalNone,
               "local ids are 10-bit fields");
 
@@ -863,7 +886,7 @@ struct TileArgs {
     const V2<double> *nxy;    // node coordinates per tile
     const int *lnodes;        // per local cell: 3 local node ids
     const int4 *gslot;        // per gradient cell: 3 slot words + dtcode
-    const G *gvol;            // per gradient cell: volume
+    const V4<G> *gcoef;       // per gradient cell: Green-Gauss weights of the 3 slots, volume
     const int4 *face;         // per face: nodes | BC code, L | R | pair face, float length, edge
 };
 
@@ -919,35 +942,46 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs<G> ta, Phys ph, 
                                                         S4 *__restrict__ rhs_out, Scalars *__restrict__ sc,
                                                         int *__restrict__ bad_list, int nc, double a, double b,
                                                         int first) {
+    static_assert(kTileMaxL <= kTileThreads && kTileMaxFF <= kTileThreads, "one local cell and one face per thread");
     __shared__ S4 sU[kTileMaxL];
+    __shared__ S4 sUn[kTileMaxG];   // U^n of the tile cells (stages 2, 3)
     __shared__ V2<double> sN[kTileMaxN];
     __shared__ V2<G> sC[kTileMaxL];
     __shared__ int4 sFace[kTileMaxF];
-    __shared__ Grad<G> sGr[kTileMaxG];
-    __shared__ V4<C> sFl[kTileMaxFF];
+    // the face fluxes replace the gradients once every flux is formed
+    static_assert(kTileMaxFF * sizeof(V4<C>) <= kTileMaxG * sizeof(Grad<G>), "flux buffer overlays the gradients");
+    __shared__ __align__(16) char sGrFl[kTileMaxG * sizeof(Grad<G>)];
+    Grad<G> *sGr = reinterpret_cast<Grad<G> *>(sGrFl);
+    V4<C> *sFl = reinterpret_cast<V4<C> *>(sGrFl);
     const int tid = threadIdx.x;
     const int *ti = ta.info + (size_t)xcd_block() * fvmr::kTileInfo;
     const int c0 = ti[fvmr::kTiC0], nT = ti[fvmr::kTiNT], nG = ti[fvmr::kTiNG], nL = ti[fvmr::kTiNL];
     const int nN = ti[fvmr::kTiNN], nF = ti[fvmr::kTiNF], nFF = ti[fvmr::kTiNFF];
     const int offH = ti[fvmr::kTiOffH], offN = ti[fvmr::kTiOffN], offF = ti[fvmr::kTiOffF];
     const int offL = ti[fvmr::kTiOffL], offG = ti[fvmr::kTiOffG];
+    // thread i owns local cell i (phases 2, 3, 5) and face i (phase 4); its records are
+    // fetched with the staging loads
+    const int i = tid;
+    const int nw = (i < nL) ? ta.lnodes[offL + i] : 0;
+    const int4 gs = (i < nG) ? ta.gslot[offG + i] : make_int4(0, 0, 0, 0);
+    const V4<G> gco = (i < nG) ? ta.gcoef[offG + i] : V4<G>{G(0.5), G(0.5), G(0.5), G(1)};
+    const G gvol = gco.w;
     // 1. stage nodes, states and face records
-    for (int i = tid; i < nN; i += kTileThreads) sN[i] = ta.nxy[offN + i];
-    for (int i = tid; i < nL; i += kTileThreads) sU[i] = Uin[i < nT ? c0 + i : ta.halo[offH + i - nT]];
-    for (int i = tid; i < nF; i += kTileThreads) sFace[i] = ta.face[offF + i];
+    for (int k = tid; k < nN; k += kTileThreads) sN[k] = ta.nxy[offN + k];
+    if (i < nL) sU[i] = Uin[i < nT ? c0 + i : ta.halo[offH + i - nT]];
+    if (!first && i < nT) sUn[i] = Un[c0 + i];
+    for (int k = tid; k < nF; k += kTileThreads) sFace[k] = ta.face[offF + k];
     __syncthreads();
     // 2. centroids
-    for (int i = tid; i < nL; i += kTileThreads) {
-        const V2<double> ct = tile_centroid(sN, ta.lnodes[offL + i]);
+    if (i < nL) {
+        const V2<double> ct = tile_centroid(sN, nw);
         sC[i] = V2<G>{(G)ct.x, (G)ct.y};
     }
     __syncthreads();
     // 3. limited gradients of the tile and the first ring
-    for (int i = tid; i < nG; i += kTileThreads) {
-        const int4 gs = ta.gslot[offG + i];
+    if (i < nG) {
         const int sw[3] = {gs.x, gs.y, gs.z};
-        const C vol = (C)ta.gvol[offG + i];
-        const int nw = ta.lnodes[offL + i];
+        const C vol = (C)gvol;
         double Ud[4];
         ld(sU, i, Ud);
         C Uc[4];
@@ -980,24 +1014,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs<G> ta, Phys ph, 
             sl.n[j][0] = (C)(G)nd.x - xc;
             sl.n[j][1] = (C)(G)nd.y - yc;
             if (ph.gradient == FVMR_GRA_GREENGAUSS) {
+                // weight g_C of the slot precomputed by the host (the device formula in FP64)
                 const C len = (C)eg.z;
-                C gC = C(0.5);
-                if (has_out) {
-                    const C fx = (C)em.x, fy = (C)em.y;
-                    C Fx = (C)sC[out].x, Fy = (C)sC[out].y;
-                    const int pl = lfield(fr.y, 2);
-                    if (periodic && pl != fvmr::kLocalNone) {
-                        const V2<G> pm = tile_face_mid<G>(sN, sFace[pl]);
-                        Fx += fx - (C)pm.x;
-                        Fy += fy - (C)pm.y;
-                    }
-                    const C ax = fx - Fx, ay = fy - Fy, bx = xc - Fx, by = yc - Fy;
-                    const C dFf = sqrt(ax * ax + ay * ay);
-                    C dFC = sqrt(bx * bx + by * by);
-                    if (dFC < C(fvmr::kEPS)) dFC = C(fvmr::kEPS);
-                    gC = dFf / dFC;
-                }
-                sl.a[j][0] = gC;
+                sl.a[j][0] = (C)(j == 0 ? gco.x : (j == 1 ? gco.y : gco.z));
                 sl.a[j][1] = (C)eg.x * len * sgn;
                 sl.a[j][2] = (C)eg.y * len * sgn;
             } else {
@@ -1023,10 +1042,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs<G> ta, Phys ph, 
         sGr[i] = g;
     }
     __syncthreads();
-    // 4. one canonical flux per face of the tile
+    // 4. one canonical flux per face of the tile (nFF <= threads), written over the gradients
     const double t_rk = sc->t_rk;
     const TileCells<G> cells{sU, sC, sGr};
-    for (int f = tid; f < nFF; f += kTileThreads) {
+    C F[4] = {C(0), C(0), C(0), C(0)};
+    const int f = tid;
+    if (f < nFF) {
         const int4 fr = sFace[f];
         V4<G> eg;
         V2<G> em;
@@ -1035,16 +1056,15 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs<G> ta, Phys ph, 
         const int bType = tile_btype(fr);
         const bool has_pair = pl != fvmr::kLocalNone;
         const V2<G> pm = (bType != -1 && has_pair) ? tile_face_mid<G>(sN, sFace[pl]) : em;
-        C F[4];
         face_flux<C, G, NS>(cells, ph, t_rk, L, R == fvmr::kLocalNone ? -1 : R, bType, has_pair, eg, em, pm, F);
-        sFl[f] = V4<C>{F[0], F[1], F[2], F[3]};
     }
+    __syncthreads();
+    if (f < nFF) sFl[f] = V4<C>{F[0], F[1], F[2], F[3]};
     __syncthreads();
     // 5. cell-owned sum in slot order, /(-V), RK update (as k_stage)
     const double dt = sc->dt;
-    for (int i = tid; i < nT; i += kTileThreads) {
+    if (i < nT) {
         const int c = c0 + i;
-        const int4 gs = ta.gslot[offG + i];
         const int sw[3] = {gs.x, gs.y, gs.z};
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -1056,10 +1076,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs<G> ta, Phys ph, 
             acc[2] += s * (double)fv.z;
             acc[3] += s * (double)fv.w;
         }
-        const double w = -1.0 * (double)ta.gvol[offG + i];
+        const double w = -1.0 * (double)gvol;
         double Ui[4], Ub[4] = {0, 0, 0, 0}, r[4], Unew[4];
         ld(sU, i, Ui);
-        if (!first) ld(Un, c, Ub);
+        if (!first) ld(sUn, i, Ub);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             r[k] = acc[k] / w;
@@ -1080,50 +1100,80 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs<G> ta, Phys ph, 
     }
 }
 
-// Block minima of CFL V / alphaC over the tiles (the dt part of k_gradient<DT>).
+// Block minima of CFL V / alphaC over the tiles (the dt part of k_gradient<DT>): the
+// spectral radius of every face of the tile once (inviscid part, and the viscous part up
+// to the division by the cell volume), then per cell the sum in the reference edge order.
 template <class G>
 __global__ __launch_bounds__(kTileThreads) void k_tile_dt(TileArgs<G> ta, Phys ph, const S4 *__restrict__ U,
                                                            double *__restrict__ partial) {
     __shared__ S4 sU[kTileMaxG];
     __shared__ V2<double> sN[kTileMaxN];
+    __shared__ int4 sFace[kTileMaxF];
+    __shared__ double sAi[kTileMaxFF], sAv[kTileMaxFF];
     __shared__ double sMin[kTileThreads / 64];
+    static_assert(kTileMaxG <= kTileThreads, "one cell per thread");
     const int tid = threadIdx.x;
     const int blk = xcd_block();
     const int *ti = ta.info + (size_t)blk * fvmr::kTileInfo;
     const int c0 = ti[fvmr::kTiC0], nT = ti[fvmr::kTiNT], nG = ti[fvmr::kTiNG];
-    const int nN = ti[fvmr::kTiNN];
+    const int nN = ti[fvmr::kTiNN], nF = ti[fvmr::kTiNF], nFF = ti[fvmr::kTiNFF];
     const int offH = ti[fvmr::kTiOffH], offN = ti[fvmr::kTiOffN], offF = ti[fvmr::kTiOffF], offG = ti[fvmr::kTiOffG];
+    const int4 gs = (tid < nT) ? ta.gslot[offG + tid] : make_int4(0, 0, 0, 0);
+    const G gvol = (tid < nT) ? ta.gcoef[offG + tid].w : G(1);
     for (int i = tid; i < nN; i += kTileThreads) sN[i] = ta.nxy[offN + i];
-    for (int i = tid; i < nG; i += kTileThreads) sU[i] = U[i < nT ? c0 + i : ta.halo[offH + i - nT]];
+    if (tid < nG) sU[tid] = U[tid < nT ? c0 + tid : ta.halo[offH + tid - nT]];
+    for (int i = tid; i < nF; i += kTileThreads) sFace[i] = ta.face[offF + i];
+    __syncthreads();
+    // faces of the tile cells: L and R (when present) are tile or first-ring cells
+    for (int f = tid; f < nFF; f += kTileThreads) {
+        const int4 fr = sFace[f];
+        const int L = lfield(fr.y, 0), R = lfield(fr.y, 1);
+        V4<G> eg;
+        V2<G> em;
+        tile_face_geom<G>(sN, fr, eg, em);
+        double uL[4], rL[4], uR[4], rR[4];
+        ld(sU, L, uL);
+        prim(uL, rL, ph.gamma);
+        const bool has_R = R != fvmr::kLocalNone;
+        ld(sU, has_R ? R : L, uR);
+        prim(uR, rR, ph.gamma);
+        double av, vv;
+        edge_alpha(ph, rL, rR, has_R, (double)eg.x, (double)eg.y, (double)eg.z, 1.0, &av, &vv);
+        sAi[f] = av;
+        sAv[f] = vv;
+    }
     __syncthreads();
     double dtc = 1e10;
-    for (int i = tid; i < nT; i += kTileThreads) {
-        const int4 gs = ta.gslot[offG + i];
+    if (tid < nT) {
+        const int i = tid;
         const int sw[3] = {gs.x, gs.y, gs.z};
         const int code = gs.w;
-        const double vol_c = (double)ta.gvol[offG + i];
-        double Uc[4], rc[4];
-        ld(sU, i, Uc);
-        prim(Uc, rc, ph.gamma);
+        const double vol_c = (double)gvol;
         // alphaC in the reference edge order: count (3 bits) + 6 x (slot 2 bits, pair 1 bit)
         const int cnt = code & 7;
         double alpha_c = 0.0;
         for (int q = 0; q < cnt; q++) {
             const int ent = (code >> (3 + 3 * q)) & 7;
             const int w = sw[ent & 3];
-            const int out = (w >> fvmr::kGsOutShift) & fvmr::kLocalMask;
-            const bool has_out = out != fvmr::kLocalNone;
-            int4 fr = ta.face[offF + (w & fvmr::kLocalMask)];
-            if (ent >> 2) fr = ta.face[offF + lfield(fr.y, 2)];
-            V4<G> eg;
-            V2<G> em;
-            tile_face_geom<G>(sN, fr, eg, em);
-            double Uo[4], ro[4];
-            ld(sU, has_out ? out : i, Uo);
-            prim(Uo, ro, ph.gamma);
+            const int f = w & fvmr::kLocalMask;
             double av, vv;
-            edge_alpha(ph, rc, ro, (ent >> 2) ? true : has_out, (double)eg.x, (double)eg.y, (double)eg.z, vol_c, &av,
-                       &vv);
+            if (!(ent >> 2)) {
+                av = sAi[f];
+                vv = sAv[f] / vol_c;
+            } else {
+                // the periodic pair of face f, seen from this cell and the cell across
+                const int out = (w >> fvmr::kGsOutShift) & fvmr::kLocalMask;
+                const int4 fr = sFace[lfield(sFace[f].y, 2)];
+                V4<G> eg;
+                V2<G> em;
+                tile_face_geom<G>(sN, fr, eg, em);
+                double Uc[4], rc[4], Uo[4], ro[4];
+                ld(sU, i, Uc);
+                prim(Uc, rc, ph.gamma);
+                ld(sU, out, Uo);
+                prim(Uo, ro, ph.gamma);
+                edge_alpha(ph, rc, ro, true, (double)eg.x, (double)eg.y, (double)eg.z, vol_c, &av, &vv);
+            }
             alpha_c += av;
             if (ph.equation == FVMR_EQ_NS) alpha_c += vv;
         }
@@ -1334,7 +1384,7 @@ struct fvmr_solver {
     int *t_info = nullptr, *t_halo = nullptr, *t_lnodes = nullptr;
     V2<double> *t_nxy = nullptr;
     int4 *t_gslot = nullptr, *t_face = nullptr;
-    void *t_gvol = nullptr;
+    void *t_gcoef = nullptr;
     bool rhs_valid = false;
     bool profiling = false;
     Prof prof[kNumProf];
@@ -1361,7 +1411,7 @@ template <class G>
 TileArgs<G> tile_args(const fvmr_solver *h) {
     TileArgs<G> t;
     t.info = h->t_info; t.halo = h->t_halo; t.nxy = h->t_nxy; t.lnodes = h->t_lnodes;
-    t.gslot = h->t_gslot; t.gvol = (const G *)h->t_gvol; t.face = h->t_face;
+    t.gslot = h->t_gslot; t.gcoef = (const V4<G> *)h->t_gcoef; t.face = h->t_face;
     return t;
 }
 
@@ -1547,7 +1597,7 @@ void free_all(fvmr_solver *h) {
     for (auto &b : h->Ubuf) if (b) (void)hipFree(b);
     void *ptrs[] = {h->grad, h->rhs, h->fix, h->cgeo, h->nxy, h->egeo, h->emid, h->cslot, h->cnode, h->dtcode,
                     h->eint, h->partial, h->norm_out, h->bad_list, h->sc, h->t_info, h->t_halo, h->t_lnodes,
-                    h->t_nxy, h->t_gslot, h->t_face, h->t_gvol};
+                    h->t_nxy, h->t_gslot, h->t_face, h->t_gcoef};
     for (void *q : ptrs) if (q) (void)hipFree(q);
     for (int k = 0; k < kNumProf; k++)
         for (auto e : h->pending[k]) (void)hipEventDestroy(e);
@@ -1727,8 +1777,12 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
                 elen[e] = m->edge_length[eperm[e]];
             }
             for (int i = 0; i < nc; i++) vol[i] = m->element_volume[h->perm[i]];
+            std::vector<double> ccx(nc), ccy(nc), emx(nf), emy(nf);
+            for (int i = 0; i < nc; i++) { ccx[i] = m->element_xy[0][h->perm[i]]; ccy[i] = m->element_xy[1][h->perm[i]]; }
+            for (int e = 0; e < nf; e++) { emx[e] = m->edge_xy[0][eperm[e]]; emy[e] = m->edge_xy[1][eperm[e]]; }
             fvmr::TileInput ti{nc, nf, sx.data(), sy.data(), cnode.data(), eL.data(), eR.data(), et.data(), ep.data(),
-                               en0.data(), en1.data(), elen.data(), m->node_xy[0], m->node_xy[1], vol.data(), dtcode.data()};
+                               en0.data(), en1.data(), elen.data(), m->node_xy[0], m->node_xy[1], vol.data(), dtcode.data(),
+                               ccx.data(), ccy.data(), emx.data(), emy.data()};
             const fvmr::TileCaps caps{p->tile_cells ? p->tile_cells : kTileDefaultCells, kTileMaxG, kTileMaxL, kTileMaxN,
                                       kTileMaxF, kTileMaxFF};
             if ((rc = fvmr::build_tiles(ti, caps, tm))) return bail(rc);
@@ -1745,16 +1799,21 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
                 (rc = upload(&h->t_lnodes, tm.lnodes, h->stream)) || (rc = upload(&h->t_nxy, txy, h->stream)) ||
                 (rc = upload(&h->t_gslot, tgs, h->stream)) || (rc = upload(&h->t_face, tfc, h->stream)))
                 return bail(rc);
-            std::vector<float> gv;
+            std::vector<V4<float>> gcf;
+            std::vector<V4<double>> gcd(tm.gcoef.size() / 4);
+            for (size_t k = 0; k < gcd.size(); k++)
+                gcd[k] = V4<double>{tm.gcoef[4 * k], tm.gcoef[4 * k + 1], tm.gcoef[4 * k + 2], tm.gcoef[4 * k + 3]};
             if (h->mixed) {
-                gv.assign(tm.gvol.begin(), tm.gvol.end());
-                float *d = nullptr;
-                if ((rc = upload(&d, gv, h->stream))) return bail(rc);
-                h->t_gvol = d;
+                gcf.resize(gcd.size());
+                for (size_t k = 0; k < gcd.size(); k++)
+                    gcf[k] = V4<float>{(float)gcd[k].x, (float)gcd[k].y, (float)gcd[k].z, (float)gcd[k].w};
+                V4<float> *d = nullptr;
+                if ((rc = upload(&d, gcf, h->stream))) return bail(rc);
+                h->t_gcoef = d;
             } else {
-                double *d = nullptr;
-                if ((rc = upload(&d, tm.gvol, h->stream))) return bail(rc);
-                h->t_gvol = d;
+                V4<double> *d = nullptr;
+                if ((rc = upload(&d, gcd, h->stream))) return bail(rc);
+                h->t_gcoef = d;
             }
             // the host vectors of these uploads end with this block
             if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(set_error(FVMR_ERR_HIP, "tile upload failed"));

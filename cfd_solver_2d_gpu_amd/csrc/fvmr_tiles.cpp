@@ -41,6 +41,26 @@ uint64_t hilbert_d(uint32_t x, uint32_t y, int order) {
 
 bool same_bits(double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; }
 
+// Green-Gauss weight g_C = |r_F - r_f| / max(|r_F - r_C|, EPSILON) of cell c across edge e to
+// cell o, the neighbour's centroid shifted by the periodic translation on periodic faces
+// (GradientGreenGaussKernel, FVMR/src/space/gradient/GradientGPU.cu:223-425); 0.5 without a
+// cell across.
+double gg_weight(const TileInput &in, int c, int e, int o, bool periodic) {
+    if (o < 0) return 0.5;
+    const double fx = in.mx[e], fy = in.my[e];
+    double Fx = in.cx[o], Fy = in.cy[o];
+    const int pe = in.epair[e];
+    if (periodic && pe >= 0) {
+        Fx += fx - in.mx[pe];
+        Fy += fy - in.my[pe];
+    }
+    const double ax = fx - Fx, ay = fy - Fy, bx = in.cx[c] - Fx, by = in.cy[c] - Fy;
+    const double dFf = std::sqrt(ax * ax + ay * ay);
+    double dFC = std::sqrt(bx * bx + by * by);
+    if (dFC < kEPS) dFC = kEPS;
+    return dFf / dFC;
+}
+
 }  // namespace
 
 std::vector<int> hilbert_order(int nc, const double *cx, const double *cy) {
@@ -174,7 +194,7 @@ int build_tiles(const TileInput &in, const TileCaps &caps, TileMesh &out) {
         const int nL = (int)cells.size(), nF = (int)faces.size(), nN = (int)nodes.size();
         const int info[kTileInfo] = {c0, s, nG, nL, nN, nF, nFF,
                                      (int)out.halo.size(), (int)(out.nxy.size() / 2), (int)(out.face.size() / 4),
-                                     (int)out.lnodes.size(), (int)out.gvol.size()};
+                                     (int)out.lnodes.size(), (int)(out.gcoef.size() / 4)};
         out.info.insert(out.info.end(), info, info + kTileInfo);
         for (int i = s; i < nL; i++) out.halo.push_back(cells[i]);
         for (int v : nodes) { out.nxy.push_back(in.node_x[v]); out.nxy.push_back(in.node_y[v]); }
@@ -186,6 +206,7 @@ int build_tiles(const TileInput &in, const TileCaps &caps, TileMesh &out) {
         }
         for (int i = 0; i < nG; i++) {
             const int c = cells[i];
+            double gw[3];
             for (int j = 0; j < 3; j++) {
                 const int x = in.cslot_x[(size_t)j * nc + c], o = in.cslot_y[(size_t)j * nc + c];
                 const int e = x & ((1 << 28) - 1);
@@ -193,9 +214,10 @@ int build_tiles(const TileInput &in, const TileCaps &caps, TileMesh &out) {
                 if (x & (1 << 30)) w |= kGsIsR;
                 if (x & (1 << 28)) w |= kGsPeriodic;
                 out.gslot.push_back(w);
+                gw[j] = gg_weight(in, c, e, o, (x & (1 << 28)) != 0);
             }
             out.gslot.push_back(i < s ? in.dtcode[c] : 0);
-            out.gvol.push_back(in.vol[c]);
+            out.gcoef.insert(out.gcoef.end(), {gw[0], gw[1], gw[2], in.vol[c]});
         }
         for (int k = 0; k < nF; k++) {
             const int e = faces[k];

@@ -46,7 +46,7 @@ struct TileMesh {
     std::vector<double> nxy;      // node coordinates, 2 per local node
     std::vector<int> lnodes;      // per local cell: 3 local node ids (10 bits each)
     std::vector<int> gslot;       // per gradient cell: 3 slot words + dtcode (tile cells)
-    std::vector<double> gvol;     // per gradient cell: volume
+    std::vector<double> gcoef;    // per gradient cell: Green-Gauss weight of slots 0-2, volume
     std::vector<int> face;        // 4 ints per face
     int max_nL = 0, max_nG = 0, max_nF = 0, max_nFF = 0, max_nN = 0;
     int64_t sum_nL = 0, sum_nG = 0, sum_nF = 0, sum_nN = 0;
@@ -64,6 +64,8 @@ struct TileInput {
     const double *node_x, *node_y;
     const double *vol;    // cell volume
     const int *dtcode;    // per cell
+    const double *cx, *cy;  // cell centroids
+    const double *mx, *my;  // edge midpoints
 };
 
 // Cells sorted along a Hilbert curve over their centroids (ties by index).
