@@ -913,6 +913,16 @@ __device__ __forceinline__ void tile_face_geom(const V2<double> *sN, int4 fr, V4
     eg = V4<G>{(G)(dy / len), (G)(-dx / len), (G)len, G(0)};
     em = V2<G>{(G)((a.x + b.x) / 2.0), (G)((a.y + b.y) / 2.0)};
 }
+// Face geometry with the normal of face f already formed in sNrm.
+template <class G>
+__device__ __forceinline__ void tile_face_geom_n(const V2<double> *sN, const V2<G> *sNrm, int f, int4 fr, V4<G> &eg,
+                                                 V2<G> &em) {
+#pragma clang fp contract(off)
+    const V2<double> a = sN[lfield(fr.x, 0)], b = sN[lfield(fr.x, 1)];
+    const V2<G> n = sNrm[f];
+    eg = V4<G>{n.x, n.y, (G)(double)__int_as_float(fr.z), G(0)};
+    em = V2<G>{(G)((a.x + b.x) / 2.0), (G)((a.y + b.y) / 2.0)};
+}
 template <class G>
 __device__ __forceinline__ V2<G> tile_face_mid(const V2<double> *sN, int4 fr) {
 #pragma clang fp contract(off)
@@ -944,10 +954,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs<G> ta, Phys ph, 
                                                         int first) {
     static_assert(kTileMaxL <= kTileThreads && kTileMaxFF <= kTileThreads, "one local cell and one face per thread");
     __shared__ S4 sU[kTileMaxL];
-    __shared__ S4 sUn[kTileMaxG];   // U^n of the tile cells (stages 2, 3)
     __shared__ V2<double> sN[kTileMaxN];
     __shared__ V2<G> sC[kTileMaxL];
     __shared__ int4 sFace[kTileMaxF];
+    __shared__ V2<G> sNrm[kTileMaxF];   // face normals
     // the face fluxes replace the gradients once every flux is formed
     static_assert(kTileMaxFF * sizeof(V4<C>) <= kTileMaxG * sizeof(Grad<G>), "flux buffer overlays the gradients");
     __shared__ __align__(16) char sGrFl[kTileMaxG * sizeof(Grad<G>)];
@@ -969,13 +979,19 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs<G> ta, Phys ph, 
     // 1. stage nodes, states and face records
     for (int k = tid; k < nN; k += kTileThreads) sN[k] = ta.nxy[offN + k];
     if (i < nL) sU[i] = Uin[i < nT ? c0 + i : ta.halo[offH + i - nT]];
-    if (!first && i < nT) sUn[i] = Un[c0 + i];
+    const S4 un = (!first && i < nT) ? Un[c0 + i] : S4{0.0, 0.0, 0.0, 0.0};
     for (int k = tid; k < nF; k += kTileThreads) sFace[k] = ta.face[offF + k];
     __syncthreads();
-    // 2. centroids
+    // 2. centroids and face normals
     if (i < nL) {
         const V2<double> ct = tile_centroid(sN, nw);
         sC[i] = V2<G>{(G)ct.x, (G)ct.y};
+    }
+    for (int k = tid; k < nF; k += kTileThreads) {
+        V4<G> eg;
+        V2<G> em;
+        tile_face_geom<G>(sN, sFace[k], eg, em);
+        sNrm[k] = V2<G>{eg.x, eg.y};
     }
     __syncthreads();
     // 3. limited gradients of the tile and the first ring
@@ -1005,7 +1021,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs<G> ta, Phys ph, 
             const int4 fr = sFace[f];
             V4<G> eg;
             V2<G> em;
-            tile_face_geom<G>(sN, fr, eg, em);
+            tile_face_geom_n<G>(sN, sNrm, f, fr, eg, em);
             double Uo[4];
             ld(sU, has_out ? out : i, Uo);
 #pragma unroll
@@ -1051,7 +1067,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs<G> ta, Phys ph, 
         const int4 fr = sFace[f];
         V4<G> eg;
         V2<G> em;
-        tile_face_geom<G>(sN, fr, eg, em);
+        tile_face_geom_n<G>(sN, sNrm, f, fr, eg, em);
         const int L = lfield(fr.y, 0), R = lfield(fr.y, 1), pl = lfield(fr.y, 2);
         const int bType = tile_btype(fr);
         const bool has_pair = pl != fvmr::kLocalNone;
@@ -1079,7 +1095,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs<G> ta, Phys ph, 
         const double w = -1.0 * (double)gvol;
         double Ui[4], Ub[4] = {0, 0, 0, 0}, r[4], Unew[4];
         ld(sU, i, Ui);
-        if (!first) ld(sUn, i, Ub);
+        if (!first) { Ub[0] = un.x; Ub[1] = un.y; Ub[2] = un.z; Ub[3] = un.w; }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             r[k] = acc[k] / w;
@@ -1103,12 +1119,13 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs<G> ta, Phys ph, 
 // Block minima of CFL V / alphaC over the tiles (the dt part of k_gradient<DT>): the
 // spectral radius of every face of the tile once (inviscid part, and the viscous part up
 // to the division by the cell volume), then per cell the sum in the reference edge order.
-template <class G>
+template <class G, bool NS>
 __global__ __launch_bounds__(kTileThreads) void k_tile_dt(TileArgs<G> ta, Phys ph, const S4 *__restrict__ U,
                                                            double *__restrict__ partial) {
+    if (!NS) ph.equation = FVMR_EQ_EULER;   // lets the compiler drop the viscous terms
     __shared__ S4 sU[kTileMaxG];
     __shared__ V2<double> sN[kTileMaxN];
-    __shared__ int4 sFace[kTileMaxF];
+    __shared__ int4 sFace[kTileMaxFF];   // faces of the tile cells
     __shared__ double sAi[kTileMaxFF], sAv[kTileMaxFF];
     __shared__ double sMin[kTileThreads / 64];
     static_assert(kTileMaxG <= kTileThreads, "one cell per thread");
@@ -1116,13 +1133,13 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_dt(TileArgs<G> ta, Phys p
     const int blk = xcd_block();
     const int *ti = ta.info + (size_t)blk * fvmr::kTileInfo;
     const int c0 = ti[fvmr::kTiC0], nT = ti[fvmr::kTiNT], nG = ti[fvmr::kTiNG];
-    const int nN = ti[fvmr::kTiNN], nF = ti[fvmr::kTiNF], nFF = ti[fvmr::kTiNFF];
+    const int nN = ti[fvmr::kTiNN], nFF = ti[fvmr::kTiNFF];
     const int offH = ti[fvmr::kTiOffH], offN = ti[fvmr::kTiOffN], offF = ti[fvmr::kTiOffF], offG = ti[fvmr::kTiOffG];
     const int4 gs = (tid < nT) ? ta.gslot[offG + tid] : make_int4(0, 0, 0, 0);
     const G gvol = (tid < nT) ? ta.gcoef[offG + tid].w : G(1);
     for (int i = tid; i < nN; i += kTileThreads) sN[i] = ta.nxy[offN + i];
     if (tid < nG) sU[tid] = U[tid < nT ? c0 + tid : ta.halo[offH + tid - nT]];
-    for (int i = tid; i < nF; i += kTileThreads) sFace[i] = ta.face[offF + i];
+    if (tid < nFF) sFace[tid] = ta.face[offF + tid];
     __syncthreads();
     // faces of the tile cells: L and R (when present) are tile or first-ring cells
     for (int f = tid; f < nFF; f += kTileThreads) {
@@ -1163,7 +1180,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_dt(TileArgs<G> ta, Phys p
             } else {
                 // the periodic pair of face f, seen from this cell and the cell across
                 const int out = (w >> fvmr::kGsOutShift) & fvmr::kLocalMask;
-                const int4 fr = sFace[lfield(sFace[f].y, 2)];
+                const int4 fr = ta.face[offF + lfield(sFace[f].y, 2)];
                 V4<G> eg;
                 V2<G> em;
                 tile_face_geom<G>(sN, fr, eg, em);
@@ -1496,7 +1513,10 @@ int launch_tile_dt(fvmr_solver *h, const S4 *U, int dt_mode) {
     hipEvent_t e0{};
     int rc = prof_begin(h, 1, &e0);
     if (rc) return rc;
-    k_tile_dt<G><<<h->ntiles, kTileThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, U, h->partial);
+    if (h->ph.equation == FVMR_EQ_NS)
+        k_tile_dt<G, true><<<h->ntiles, kTileThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, U, h->partial);
+    else
+        k_tile_dt<G, false><<<h->ntiles, kTileThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, U, h->partial);
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 1, e0))) return rc;
     if ((rc = prof_begin(h, 2, &e0))) return rc;
