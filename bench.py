@@ -6,9 +6,15 @@ split into 1,048,576 triangles, interior nodes jittered by +-10 % of the local s
 limiter, linear reconstruction, SSP-RK3, CFL 0.5, FP64.  A step is one SSP-RK3 step
 (3 stages); value = cells x 3 x steps / wall time of the timed steps.
 
+Per step the tile kernels run k_tile_dt + k_dt_finish (global dt) and 3 x (k_tile +
+k_restrict); k_tile is one whole RK stage (gradients, fluxes, update).  Its roofline counts
+the algorithmic bytes of one RK stage, B_stage = n_c(12 s_U + 16 s_G + 3 s_X + 24) +
+n_f(8 + 5 s_X) + n_v 2 s_X, against its average launch time from HIP events; the measured
+HBM traffic per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated) is reported beside.
+
 Also reported: the same run in FP32-mixed precision (speed-up and relative L2 density
-error against FP64 after the same steps), the LLF flux, the dominant kernel's achieved
-HBM bandwidth from HIP events, and the CPU oracle timed on a bounded sample.
+error against FP64 after the same steps), the LLF flux, the cell-kernel family on the same
+workload, and the CPU oracle timed on a bounded sample.
 """
 import argparse
 import json
@@ -25,35 +31,47 @@ HBM_PEAK_GBS = 8000.0
 NI, NJ = 1024, 512
 
 
-def make_case(scheme, precision):
+def make_case(scheme, precision, kernels=None):
     import cfd_solver_2d_gpu_amd as F
     m = F.Mesh.generate(F.GEN_WEDGE_CHANNEL, NI, NJ, a=3.0, b=1.0, c=0.5, d=15.0, jitter=0.1, seed=2024)
     ruvp = [1.4, 2.0, 0.0, 1.0]
     p = F.default_params(inf_ruvp=ruvp, inlet_ruvp=ruvp, outlet_ruvp=ruvp, CFL=0.5, flux_scheme=scheme,
                          gradient=F.GRA_GREENGAUSS, limiter=F.LIM_BARTH, reconstruct=F.REC_LINEAR,
                          time_advance=F.EVO_RK3, precision=precision)
+    if kernels is not None:
+        p.kernels = kernels
     U = F.field_init(m, p, F.INIT_UNIFORM)
     return m, p, U
 
 
-def stage_bytes(nc, nf, s_u, s_g, s_x, first):
-    """Compulsory HBM bytes of one launch of the stage kernel (flux + gather + RK update):
-    U_stage 4 s_U, Ux/Uy 8 s_G, centroid + volume 3 s_X, cell->face int32 12, U_out 4 s_U,
-    U^n 4 s_U (stages 2 and 3), per face L/R int32 8 + normal/length/midpoint 5 s_X."""
-    per_cell = 4 * s_u + 8 * s_g + 3 * s_x + 12 + 4 * s_u + (0 if first else 4 * s_u)
-    return nc * per_cell + nf * (8 + 5 * s_x)
+def stage_bytes(nc, nf, nv, s_u, s_g, s_x):
+    """Algorithmic bytes of one RK stage (gradient + limiter + fluxes + update): U_stage, U^n,
+    U_new (4 s_U each), Ux/Uy written and read (16 s_G), centroid + volume (3 s_X), cell->node
+    and cell->face int32 (24); per face L/R int32 (8) + normal, length, midpoint (5 s_X); per
+    node xy (2 s_X)."""
+    return nc * (12 * s_u + 16 * s_g + 3 * s_x + 24) + nf * (8 + 5 * s_x) + nv * 2 * s_x
+
+
+def dt_bytes(nc, nf, s_u, s_x):
+    """Algorithmic bytes of the global time step: state and volume per cell, per face L/R
+    int32 + normal and length."""
+    return nc * (4 * s_u + s_x) + nf * (8 + 3 * s_x)
 
 
 def path_bytes_per_step(nc, nf, nv, s_u, s_g, s_x):
     """Algorithmic bytes of one SSP-RK3 step: 3 B_stage + B_dt."""
-    b_stage = nc * (12 * s_u + 16 * s_g + 3 * s_x + 24) + nf * (8 + 5 * s_x) + nv * 2 * s_x
-    b_dt = nc * (4 * s_u + s_x) + nf * (8 + 3 * s_x)
-    return 3 * b_stage + b_dt
+    return 3 * stage_bytes(nc, nf, nv, s_u, s_g, s_x) + dt_bytes(nc, nf, s_u, s_x)
 
 
-def run_gpu(scheme, precision, steps, warmup, profile):
+KERNEL_SLOTS = {  # profile slot -> kernel name, per kernel family
+    "tile": {0: "k_tile", 1: "k_tile_dt", 2: "k_dt_finish", 3: "k_restrict"},
+    "cell": {0: "k_stage", 1: "k_gradient", 2: "k_dt_finish", 3: "k_restrict"},
+}
+
+
+def run_gpu(scheme, precision, steps, warmup, profile, kernels=None):
     import cfd_solver_2d_gpu_amd as F
-    m, p, U = make_case(scheme, precision)
+    m, p, U = make_case(scheme, precision, kernels)
     g = F.GPUSolver2(m, p)
     g.set_state(U)
     t = 0.0
@@ -72,8 +90,8 @@ def run_gpu(scheme, precision, steps, warmup, profile):
         g.profile_reset()
         g.advance(steps, t, 1e9)
         g.synchronize()
-        for name, k in (("stage", g.PROF_STAGE), ("gradient", g.PROF_GRADIENT), ("dt", g.PROF_DT),
-                        ("restrict", g.PROF_RESTRICT)):
+        names = KERNEL_SLOTS["cell" if kernels == F.KERNELS_CELL else "tile"]
+        for k, name in names.items():
             ms, n = g.profile_query(k)
             kern[name] = (ms, n)
         g.profile(False)
@@ -82,9 +100,9 @@ def run_gpu(scheme, precision, steps, warmup, profile):
 
 
 def traffic_from_profiles():
-    """HBM bytes per stage-kernel launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+    """HBM bytes per k_tile launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
     scaled by the copy-kernel calibration; committed in profiles/ (None when absent)."""
-    path = os.path.join(ROOT, "profiles", "stage_kernel_pmc_wedge_1M.txt")
+    path = os.path.join(ROOT, "profiles", "k_tile_pmc_wedge_1M_fp64.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -139,13 +157,13 @@ def main():
     value = nc * 3 * args.steps / wall
     ms_step = wall / args.steps * 1e3
 
-    st_ms, st_n = kern["stage"]
-    b_launch = (stage_bytes(nc, nf, 8, 8, 8, True) + 2 * stage_bytes(nc, nf, 8, 8, 8, False)) / 3.0
+    st_ms, st_n = kern["k_tile"]
+    b_launch = stage_bytes(nc, nf, nv, 8, 8, 8)
     achieved = b_launch / (st_ms * 1e-3) / 1e9
     traffic = traffic_from_profiles()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_stage<double,double> (flux + gather + RK update)",
+                "kernel": "k_tile<double,double> (one RK stage: gradients, limiter, fluxes, update)",
                 "bytes_per_launch": int(b_launch), "avg_launch_ms": round(st_ms, 5), "launches": st_n}
     path_b = path_bytes_per_step(nc, nf, nv, 8, 8, 8)
     out = {
@@ -169,6 +187,11 @@ def main():
                         "kernels_ms": {k: round(v[0], 5) for k, v in kmx.items()}}
         _, wall_llf, _, _ = run_gpu(F.SOL_LLF, F.PREC_FP64, args.steps, args.warmup, profile=False)
         out["llf_fp64_value"] = nc * 3 * args.steps / wall_llf
+        _, wall_cell, Ucell, kcell = run_gpu(F.SOL_ROE, F.PREC_FP64, args.steps, args.warmup, profile=True,
+                                             kernels=F.KERNELS_CELL)
+        out["cell_kernels"] = {"value": nc * 3 * args.steps / wall_cell,
+                               "rel_l2_density_vs_tile": float(np.linalg.norm(Ucell[0] - U64[0]) / np.linalg.norm(U64[0])),
+                               "kernels_ms": {k: round(v[0], 5) for k, v in kcell.items()}}
     if not args.no_cpu:
         rate, cw = cpu_baseline(args.cpu_steps)
         out["cpu_baseline"] = {"value": rate, "unit": "cell-RK-stage updates/s", "cores": 1, "kind": "port",

@@ -879,6 +879,14 @@ constexpr int kTileDefaultCells = 300;
 static_assert(kTileMaxL <= fvmr::kLocalNone && kTileMaxN <= fvmr::kLocalNone && kTileMaxF <= fvmr::kLocalNone,
               "local ids are 10-bit fields");
 
+// Per cell, for the time-step kernel: alphaC code, the 3 slot faces (local ids in the
+// cell's tile), volume.
+struct DtRec {
+    int code;
+    int faces;
+    double vol;
+};
+
 template <class G>
 struct TileArgs {
     const int *info;          // fvmr::kTileInfo ints per tile
@@ -888,6 +896,7 @@ struct TileArgs {
     const int4 *gslot;        // per gradient cell: 3 slot words + dtcode
     const V4<G> *gcoef;       // per gradient cell: Green-Gauss weights of the 3 slots, volume
     const int4 *face;         // per face: nodes | BC code, L | R | pair face, float length, edge
+    const DtRec *dtrec;       // per device cell
 };
 
 __device__ __forceinline__ int lfield(int w, int k) { return (w >> (fvmr::kLocalBits * k)) & fvmr::kLocalMask; }
@@ -1135,8 +1144,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_dt(TileArgs<G> ta, Phys p
     const int c0 = ti[fvmr::kTiC0], nT = ti[fvmr::kTiNT], nG = ti[fvmr::kTiNG];
     const int nN = ti[fvmr::kTiNN], nFF = ti[fvmr::kTiNFF];
     const int offH = ti[fvmr::kTiOffH], offN = ti[fvmr::kTiOffN], offF = ti[fvmr::kTiOffF], offG = ti[fvmr::kTiOffG];
-    const int4 gs = (tid < nT) ? ta.gslot[offG + tid] : make_int4(0, 0, 0, 0);
-    const G gvol = (tid < nT) ? ta.gcoef[offG + tid].w : G(1);
+    const DtRec dr = (tid < nT) ? ta.dtrec[c0 + tid] : DtRec{0, 0, 1.0};
     for (int i = tid; i < nN; i += kTileThreads) sN[i] = ta.nxy[offN + i];
     if (tid < nG) sU[tid] = U[tid < nT ? c0 + tid : ta.halo[offH + tid - nT]];
     if (tid < nFF) sFace[tid] = ta.face[offF + tid];
@@ -1163,24 +1171,23 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_dt(TileArgs<G> ta, Phys p
     double dtc = 1e10;
     if (tid < nT) {
         const int i = tid;
-        const int sw[3] = {gs.x, gs.y, gs.z};
-        const int code = gs.w;
-        const double vol_c = (double)gvol;
+        const int code = dr.code;
+        const double vol_c = dr.vol;
         // alphaC in the reference edge order: count (3 bits) + 6 x (slot 2 bits, pair 1 bit)
         const int cnt = code & 7;
         double alpha_c = 0.0;
         for (int q = 0; q < cnt; q++) {
             const int ent = (code >> (3 + 3 * q)) & 7;
-            const int w = sw[ent & 3];
-            const int f = w & fvmr::kLocalMask;
+            const int f = lfield(dr.faces, ent & 3);
             double av, vv;
             if (!(ent >> 2)) {
                 av = sAi[f];
                 vv = sAv[f] / vol_c;
             } else {
-                // the periodic pair of face f, seen from this cell and the cell across
-                const int out = (w >> fvmr::kGsOutShift) & fvmr::kLocalMask;
-                const int4 fr = ta.face[offF + lfield(sFace[f].y, 2)];
+                // the periodic pair of face f, seen from this cell (the face's L) and its R
+                const int4 ff = sFace[f];
+                const int out = lfield(ff.y, 1);
+                const int4 fr = ta.face[offF + lfield(ff.y, 2)];
                 V4<G> eg;
                 V2<G> em;
                 tile_face_geom<G>(sN, fr, eg, em);
@@ -1402,6 +1409,7 @@ struct fvmr_solver {
     V2<double> *t_nxy = nullptr;
     int4 *t_gslot = nullptr, *t_face = nullptr;
     void *t_gcoef = nullptr;
+    DtRec *t_dtrec = nullptr;
     bool rhs_valid = false;
     bool profiling = false;
     Prof prof[kNumProf];
@@ -1428,7 +1436,7 @@ template <class G>
 TileArgs<G> tile_args(const fvmr_solver *h) {
     TileArgs<G> t;
     t.info = h->t_info; t.halo = h->t_halo; t.nxy = h->t_nxy; t.lnodes = h->t_lnodes;
-    t.gslot = h->t_gslot; t.gcoef = (const V4<G> *)h->t_gcoef; t.face = h->t_face;
+    t.gslot = h->t_gslot; t.gcoef = (const V4<G> *)h->t_gcoef; t.face = h->t_face; t.dtrec = h->t_dtrec;
     return t;
 }
 
@@ -1617,7 +1625,7 @@ void free_all(fvmr_solver *h) {
     for (auto &b : h->Ubuf) if (b) (void)hipFree(b);
     void *ptrs[] = {h->grad, h->rhs, h->fix, h->cgeo, h->nxy, h->egeo, h->emid, h->cslot, h->cnode, h->dtcode,
                     h->eint, h->partial, h->norm_out, h->bad_list, h->sc, h->t_info, h->t_halo, h->t_lnodes,
-                    h->t_nxy, h->t_gslot, h->t_face, h->t_gcoef};
+                    h->t_nxy, h->t_gslot, h->t_face, h->t_gcoef, h->t_dtrec};
     for (void *q : ptrs) if (q) (void)hipFree(q);
     for (int k = 0; k < kNumProf; k++)
         for (auto e : h->pending[k]) (void)hipEventDestroy(e);
@@ -1835,6 +1843,10 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
                 if ((rc = upload(&d, gcd, h->stream))) return bail(rc);
                 h->t_gcoef = d;
             }
+            std::vector<DtRec> dtr(nc);
+            for (int i = 0; i < nc; i++)
+                dtr[i] = DtRec{dtcode[i], tm.dtface[i], h->mixed ? (double)(float)vol[i] : vol[i]};
+            if ((rc = upload(&h->t_dtrec, dtr, h->stream))) return bail(rc);
             // the host vectors of these uploads end with this block
             if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(set_error(FVMR_ERR_HIP, "tile upload failed"));
         }

@@ -128,6 +128,7 @@ int build_tiles(const TileInput &in, const TileCaps &caps, TileMesh &out) {
             return set_error(FVMR_ERR_UNSUPPORTED, "boundary type outside the tile kernels' code range");
     }
     out = TileMesh();
+    out.dtface.assign(nc, 0);
     // stamps: lid valid when lst == current tile
     std::vector<int> lst(nc, -1), lid(nc, 0), fst(nf, -1), fid(nf, 0);
     int nv = 0;
@@ -217,6 +218,11 @@ int build_tiles(const TileInput &in, const TileCaps &caps, TileMesh &out) {
                 gw[j] = gg_weight(in, c, e, o, (x & (1 << 28)) != 0);
             }
             out.gslot.push_back(i < s ? in.dtcode[c] : 0);
+            if (i < s) {
+                int w = 0;
+                for (int j = 0; j < 3; j++) w |= fid[in.cslot_x[(size_t)j * nc + c] & ((1 << 28) - 1)] << (kLocalBits * j);
+                out.dtface[c] = w;
+            }
             out.gcoef.insert(out.gcoef.end(), {gw[0], gw[1], gw[2], in.vol[c]});
         }
         for (int k = 0; k < nF; k++) {

@@ -48,6 +48,7 @@ struct TileMesh {
     std::vector<int> gslot;       // per gradient cell: 3 slot words + dtcode (tile cells)
     std::vector<double> gcoef;    // per gradient cell: Green-Gauss weight of slots 0-2, volume
     std::vector<int> face;        // 4 ints per face
+    std::vector<int> dtface;      // per device cell: its 3 slot faces, local ids in its tile
     int max_nL = 0, max_nG = 0, max_nF = 0, max_nFF = 0, max_nN = 0;
     int64_t sum_nL = 0, sum_nG = 0, sum_nF = 0, sum_nN = 0;
 };
