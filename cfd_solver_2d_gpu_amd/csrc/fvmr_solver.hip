@@ -1,27 +1,29 @@
 // Device path of the FVMR explicit residual-and-update step for MI355X (gfx950).
 //
-// One step (CIteration::iteration_device, FVMR/src/time/CIteration.cpp:89-161):
-//   k_gradient<DT>  per cell, 3 wave64 x 64 cells: wave j gathers the neighbour, geometry
-//                   and vertex of face slot j; wave 0 forms the Green-Gauss or least-squares
-//                   gradient and the Barth/Venkatakrishnan limiter.  In the first stage of a
-//                   step it also sums the spectral radius alphaC in the reference edge order
-//                   and writes the block minimum of CFL V / alphaC.
-//   k_dt_finish     one work-group: global minimum, clamp to T - t, t advanced on the device.
-//   k_stage         per cell, 3 wave64 x 64 cells: wave j evaluates the canonical flux of
-//                   face slot j (reconstruction, BC ghost state, Roe/LLF, viscous part); wave 0
-//                   sums the signed fluxes in slot order (cell-owned gather, no atomics),
-//                   divides by -V and applies the RK update.
-//   k_restrict      one work-group: cells flagged by k_stage are repaired from their
-//                   neighbours (Jacobi reads), then written.
+// One step (CIteration::iteration_device, FVMR/src/time/CIteration.cpp:89-161) runs one of two
+// kernel families over the same device numbering (cells in Hilbert, RCM or caller order):
 //
-// Device layout: cells and edges in device order (identity or reverse Cuthill-McKee);
-// gathered data is packed per cell / per edge so that a neighbour costs 1-2 cache lines:
-//   state     {rho, rho u, rho v, rho E}       32 B per cell, three rotating buffers
-//   gradient  {Ux[4], Uy[4]}                   8 G per cell
-//   cell geo  {cx, cy, volume, 0}              4 G per cell
-//   edge geo  {nx, ny, length, 0}, {mx, my}    6 G per edge
-//   edge int  {L, R, BC type (-1 interior), periodic pair (-1)}
-// with G = double (plain FP64) or float (mixed precision).
+// Tile family (default; the mesh is cut into tiles of consecutive cells, fvmr_tiles.cpp):
+//   k_tile_dt       one work-group per tile: spectral radius of every face of the tile once,
+//                   alphaC per cell in the reference edge order, block minimum of CFL V / alphaC.
+//   k_dt_finish     one work-group: global minimum, clamp to T - t, t advanced on the device.
+//   k_tile          one work-group per tile and RK stage: states of the tile and two halo
+//                   rings, nodes and face records staged in LDS; centroids and normals
+//                   recomputed from the nodes; limited gradients of the tile and the first
+//                   ring in LDS; the canonical flux of every face of the tile once; per cell
+//                   the signed sum in slot order, /(-V), RK update.
+//   k_restrict      one work-group: flagged cells repaired from their neighbours (Jacobi).
+//
+// Cell family (any geometry; packed per-cell / per-edge records, a neighbour costs 1-2 lines):
+//   k_gradient<DT>  3 wave64 x 64 cells: wave j gathers face slot j; wave 0 forms the
+//                   gradient and limiter (and, in stage 1, the dt block minimum).
+//   k_stage         3 wave64 x 64 cells: wave j evaluates the canonical flux of face slot j;
+//                   wave 0 sums in slot order, divides by -V, applies the RK update.
+//   k_dt_finish, k_restrict as above.
+//
+// Records: state {rho, rho u, rho v, rho E} (32 B, three rotating buffers); gradient
+// {Ux[4], Uy[4]}; cell geo {cx, cy, V, 0}; edge geo {nx, ny, L, 0}, {mx, my}; edge int
+// {L, R, BC type, periodic pair}; G = double (plain FP64) or float (mixed precision).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1647,7 +1649,7 @@ int validate_params(const fvmr_params *p) {
     if (p->renumber < FVMR_RENUMBER_NONE || p->renumber > FVMR_RENUMBER_HILBERT)
         return set_error(FVMR_ERR_INVALID_ARG, "invalid renumber");
     if (p->kernels < FVMR_KERNELS_AUTO || p->kernels > FVMR_KERNELS_TILE) return set_error(FVMR_ERR_INVALID_ARG, "invalid kernels");
-    if (p->tile_cells < 0 || p->tile_cells > kTileMaxG) return set_error(FVMR_ERR_INVALID_ARG, "tile_cells out of [0, 256]");
+    if (p->tile_cells < 0 || p->tile_cells > kTileMaxG) return set_error(FVMR_ERR_INVALID_ARG, "tile_cells out of [0, 400]");
     return FVMR_OK;
 }
 
