@@ -262,6 +262,33 @@ class GPUSolver2:
         check(lib().fvmr_permutation(self._h, _iptr(p)), "fvmr_permutation")
         return p
 
+    def node_field(self):
+        """FieldWriter::update_nodeField on the device: (ruvp 4 x num_node, Cp, Ma) of the
+        current state (FVMR/src/output/FieldWriter.cpp:371-452)."""
+        nv = self.mesh.soa.num_node
+        R = np.zeros((4, nv))
+        cp = np.zeros(nv)
+        ma = np.zeros(nv)
+        R, rows = _rows(R)
+        check(lib().fvmr_node_field(self._h, rows, _dptr(cp), _dptr(ma)), "fvmr_node_field")
+        return R, cp, ma
+
+    def surface_force(self, S_ref=1.0, sets=None):
+        """ReferenceData::calculate_edgeSet_force (FieldWriter.cpp:174-205) on the device: per
+        boundary set (default: the mesh's sets, in set order) the pressure force coefficients
+        (Cx, Cy) = the reference's (CD, CL)."""
+        if sets is None:
+            sets = [s[2] for s in self.mesh.boundary_sets()]
+        off = np.zeros(len(sets) + 1, dtype=np.int32)
+        off[1:] = np.cumsum([len(e) for e in sets])
+        edges = np.ascontiguousarray(np.concatenate([np.asarray(e, dtype=np.int32) for e in sets])
+                                     if len(sets) else np.zeros(1, np.int32))
+        cx = np.zeros(max(1, len(sets)))
+        cy = np.zeros(max(1, len(sets)))
+        check(lib().fvmr_surface_force(self._h, len(sets), _iptr(off), _iptr(edges), float(S_ref), _dptr(cx),
+                                       _dptr(cy)), "fvmr_surface_force")
+        return cx[:len(sets)], cy[:len(sets)]
+
     def profile(self, on=True):
         check(lib().fvmr_profile_enable(self._h, 1 if on else 0), "fvmr_profile_enable")
 

@@ -102,6 +102,8 @@ SIGNATURES = {
     "fvmr_global_dt": (C.c_int, [C.c_void_p, _dp]),
     "fvmr_residual": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "fvmr_permutation": (C.c_int, [C.c_void_p, _ip]),
+    "fvmr_node_field": (C.c_int, [C.c_void_p, _dp * 4, _dp, _dp]),
+    "fvmr_surface_force": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, C.c_double, _dp, _dp]),
     "fvmr_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "fvmr_profile_query": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip]),
     "fvmr_profile_reset": (C.c_int, [C.c_void_p]),

@@ -1215,6 +1215,58 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_dt(TileArgs<G> ta, Phys p
     }
 }
 
+// update_nodeField_ruvp + update_nodeField_other_variables (FieldWriter.cpp:371-452): one
+// thread per node, the cells of the node in the caller's element order, U2ruvp as the
+// reference writes it (PhysicalKernelGPU.h:11-16), no contraction.  out: ruvp (4 nv), Cp, Ma.
+__global__ __launch_bounds__(kBlock) void k_node_field(int nv, const int *__restrict__ off,
+                                                        const int *__restrict__ cells, const S4 *__restrict__ U,
+                                                        double gamma, double p_inf, double p_dyn,
+                                                        double *__restrict__ out) {
+#pragma clang fp contract(off)
+    const int v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= nv) return;
+    double sum[4] = {0.0, 0.0, 0.0, 0.0};
+    const int k0 = off[v], k1 = off[v + 1];
+    for (int k = k0; k < k1; k++) {
+        const S4 u = U[cells[k]];
+        double r[4];
+        r[0] = u.x;
+        r[1] = u.y / u.x;
+        r[2] = u.z / u.x;
+        r[3] = r[0] * (gamma - 1) * (u.w / u.x - (r[1] * r[1] + r[2] * r[2]) * 0.5);
+        for (int q = 0; q < 4; q++) sum[q] += r[q];
+    }
+    const int n = k1 - k0;
+    if (n > 0)
+        for (int q = 0; q < 4; q++) sum[q] /= n;
+    for (int q = 0; q < 4; q++) out[(size_t)q * nv + v] = sum[q];
+    const double rho = sum[0], u = sum[1], w = sum[2], p = sum[3];
+    out[(size_t)4 * nv + v] = (p - p_inf) / p_dyn;
+    const double V2 = u * u + w * w;
+    const double a2 = gamma * p / rho;
+    out[(size_t)5 * nv + v] = sqrt(V2 / a2);
+}
+
+// ReferenceData::calculate_edgeSet_force (FieldWriter.cpp:174-205): one thread per set, its
+// edges in set order.  edge e: nodes en[2e], en[2e+1], geometry eg[3e..3e+2] = nx, ny, L.
+__global__ void k_surface_force(int nsets, const int *__restrict__ off, const int *__restrict__ en,
+                                const double *__restrict__ eg, const double *__restrict__ Cp, double S_ref,
+                                double *__restrict__ out) {
+#pragma clang fp contract(off)
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsets) return;
+    const double one_on_Sref = 1 / S_ref;
+    double cd = 0.0, cl = 0.0;
+    for (int k = off[s]; k < off[s + 1]; k++) {
+        const double cp = 0.5 * (Cp[en[2 * k]] + Cp[en[2 * k + 1]]);
+        const double dC = cp * one_on_Sref * eg[3 * k + 2];
+        cd += dC * eg[3 * k];
+        cl += dC * eg[3 * k + 1];
+    }
+    out[2 * s] = cd;
+    out[2 * s + 1] = cl;
+}
+
 // modifyElementFieldU2d_device_kernel (RestrictGPU.cu:6-84) with Jacobi reads: one work-group,
 // phase 1 computes all repairs from the unrepaired field, phase 2 writes them.
 template <class G>
@@ -1412,6 +1464,12 @@ struct fvmr_solver {
     int4 *t_gslot = nullptr, *t_face = nullptr;
     void *t_gcoef = nullptr;
     DtRec *t_dtrec = nullptr;
+    // output quantities: node -> cells (caller's element order), node buffers, boundary edges
+    int *nc_off = nullptr, *nc_cells = nullptr;
+    double *node_buf = nullptr;               // 6 x nv: ruvp, Cp, Ma
+    std::vector<int> bedge_ref;               // caller ids of boundary-set edges, ascending
+    std::vector<int> bedge_nodes;             // 2 per boundary edge
+    std::vector<double> bedge_geo;            // nx, ny, L per boundary edge
     bool rhs_valid = false;
     bool profiling = false;
     Prof prof[kNumProf];
@@ -1627,7 +1685,8 @@ void free_all(fvmr_solver *h) {
     for (auto &b : h->Ubuf) if (b) (void)hipFree(b);
     void *ptrs[] = {h->grad, h->rhs, h->fix, h->cgeo, h->nxy, h->egeo, h->emid, h->cslot, h->cnode, h->dtcode,
                     h->eint, h->partial, h->norm_out, h->bad_list, h->sc, h->t_info, h->t_halo, h->t_lnodes,
-                    h->t_nxy, h->t_gslot, h->t_face, h->t_gcoef, h->t_dtrec};
+                    h->t_nxy, h->t_gslot, h->t_face, h->t_gcoef, h->t_dtrec,
+                    h->nc_off, h->nc_cells, h->node_buf};
     for (void *q : ptrs) if (q) (void)hipFree(q);
     for (int k = 0; k < kNumProf; k++)
         for (auto e : h->pending[k]) (void)hipEventDestroy(e);
@@ -1790,6 +1849,28 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
             }
         }
         for (int i = 0; i < nc; i++) dtcode[i] |= cnt[i];
+    }
+    // node -> cells in the caller's element order; boundary-set edges for the surface force
+    {
+        const int nv = h->nv;
+        std::vector<int> off(nv + 1, 0), cells(3 * (size_t)nc);
+        for (int r = 0; r < nc; r++)
+            for (int j = 0; j < 3; j++) off[m->element_nodes[j][r] + 1]++;
+        for (int v = 0; v < nv; v++) off[v + 1] += off[v];
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (int r = 0; r < nc; r++)
+            for (int j = 0; j < 3; j++) cells[fill[m->element_nodes[j][r]]++] = h->iperm[r];
+        if ((rc = upload(&h->nc_off, off, h->stream)) || (rc = upload(&h->nc_cells, cells, h->stream))) return bail(rc);
+        if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(set_error(FVMR_ERR_HIP, "node table upload failed"));
+        for (int e = 0; e < nf; e++) {
+            if (m->edge_setID[e] == -1) continue;
+            h->bedge_ref.push_back(e);
+            h->bedge_nodes.push_back(m->edge_nodes[0][e]);
+            h->bedge_nodes.push_back(m->edge_nodes[1][e]);
+            h->bedge_geo.push_back(m->edge_normal[0][e]);
+            h->bedge_geo.push_back(m->edge_normal[1][e]);
+            h->bedge_geo.push_back(m->edge_length[e]);
+        }
     }
     // tile records
     fvmr::TileMesh tm;
@@ -2046,6 +2127,83 @@ int fvmr_residual(fvmr_handle *h, int norm, double out[4]) {
 int fvmr_permutation(fvmr_handle *h, int *perm) {
     if (!h || !perm) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
     std::copy(h->perm.begin(), h->perm.end(), perm);
+    return FVMR_OK;
+}
+
+// Node fields of the current state into h->node_buf (6 x nv on the device).
+static int node_field_device(fvmr_handle *h) {
+    const int nv = h->nv;
+    if (!h->node_buf && hipMalloc((void **)&h->node_buf, 6 * (size_t)nv * sizeof(double)) != hipSuccess)
+        return set_error(FVMR_ERR_HIP, "hipMalloc node fields");
+    const double *inf = h->p.inf_ruvp;
+    const double V2 = inf[1] * inf[1] + inf[2] * inf[2];
+    k_node_field<<<grid_of(nv), kBlock, 0, h->stream>>>(nv, h->nc_off, h->nc_cells, h->Ubuf[h->cur], h->p.gamma,
+                                                        inf[3], 0.5 * inf[0] * V2, h->node_buf);
+    HIP_OK(hipGetLastError());
+    return FVMR_OK;
+}
+
+int fvmr_node_field(fvmr_handle *h, double *const ruvp[4], double *Cp, double *Ma) {
+    if (!h) return set_error(FVMR_ERR_INVALID_ARG, "null handle");
+    int rc = node_field_device(h);
+    if (rc) return rc;
+    const size_t nv = h->nv;
+    std::vector<double> buf(6 * nv);
+    HIP_OK(hipMemcpyAsync(buf.data(), h->node_buf, buf.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    if (ruvp)
+        for (int k = 0; k < 4; k++)
+            if (ruvp[k]) std::copy(buf.begin() + k * nv, buf.begin() + (k + 1) * nv, ruvp[k]);
+    if (Cp) std::copy(buf.begin() + 4 * nv, buf.begin() + 5 * nv, Cp);
+    if (Ma) std::copy(buf.begin() + 5 * nv, buf.begin() + 6 * nv, Ma);
+    return FVMR_OK;
+}
+
+int fvmr_surface_force(fvmr_handle *h, int nsets, const int *set_offsets, const int *set_edges, double S_ref,
+                       double *Cx, double *Cy) {
+    if (!h || nsets < 0 || (nsets > 0 && (!set_offsets || !Cx || !Cy))) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
+    if (nsets == 0) return FVMR_OK;
+    if (set_offsets[0] != 0) return set_error(FVMR_ERR_INVALID_ARG, "set_offsets[0] must be 0");
+    for (int s = 0; s < nsets; s++)
+        if (set_offsets[s + 1] < set_offsets[s]) return set_error(FVMR_ERR_INVALID_ARG, "set_offsets must not decrease");
+    const int ne = set_offsets[nsets];
+    if (ne > 0 && !set_edges) return set_error(FVMR_ERR_INVALID_ARG, "null set_edges");
+    std::vector<int> en(2 * (size_t)std::max(ne, 1));
+    std::vector<double> eg(3 * (size_t)std::max(ne, 1));
+    for (int k = 0; k < ne; k++) {
+        const auto it = std::lower_bound(h->bedge_ref.begin(), h->bedge_ref.end(), set_edges[k]);
+        if (it == h->bedge_ref.end() || *it != set_edges[k])
+            return set_error(FVMR_ERR_INVALID_ARG, "surface force: edge " + std::to_string(set_edges[k]) + " is not in a boundary set");
+        const size_t b = it - h->bedge_ref.begin();
+        en[2 * k] = h->bedge_nodes[2 * b];
+        en[2 * k + 1] = h->bedge_nodes[2 * b + 1];
+        for (int q = 0; q < 3; q++) eg[3 * k + q] = h->bedge_geo[3 * b + q];
+    }
+    int rc = node_field_device(h);
+    if (rc) return rc;
+    std::vector<int> off(set_offsets, set_offsets + nsets + 1);
+    int *d_off = nullptr, *d_en = nullptr;
+    double *d_eg = nullptr, *d_out = nullptr;
+    auto release = [&]() {
+        if (d_off) (void)hipFree(d_off);
+        if (d_en) (void)hipFree(d_en);
+        if (d_eg) (void)hipFree(d_eg);
+        if (d_out) (void)hipFree(d_out);
+    };
+    std::vector<double> out(2 * (size_t)nsets);
+    if ((rc = upload(&d_off, off, h->stream)) || (rc = upload(&d_en, en, h->stream)) || (rc = upload(&d_eg, eg, h->stream)) ||
+        hipMalloc((void **)&d_out, out.size() * sizeof(double)) != hipSuccess) {
+        release();
+        return rc ? rc : set_error(FVMR_ERR_HIP, "hipMalloc surface force");
+    }
+    k_surface_force<<<(nsets + 63) / 64, 64, 0, h->stream>>>(nsets, d_off, d_en, d_eg, h->node_buf + 4 * (size_t)h->nv,
+                                                           S_ref, d_out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out.data(), d_out, out.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    release();
+    if (e != hipSuccess) return set_error(FVMR_ERR_HIP, std::string("surface force: ") + hipGetErrorString(e));
+    for (int s = 0; s < nsets; s++) { Cx[s] = out[2 * s]; Cy[s] = out[2 * s + 1]; }
     return FVMR_OK;
 }
 

@@ -257,6 +257,25 @@ int fvmr_residual(fvmr_handle *h, int norm, double out[4]);
 int fvmr_permutation(fvmr_handle *h, int *perm);
 
 /* ---------------------------------------------------------------------- */
+/* output quantities (FieldWriter, FVMR/src/output/FieldWriter.cpp)         */
+/* ---------------------------------------------------------------------- */
+/* Node fields of the current state, formed on the device: node ruvp = mean of the primitives
+ * of the cells listing the node, summed in the caller's element order (update_nodeField_ruvp,
+ * FieldWriter.cpp:371-418); Cp = (p - p_inf) / (0.5 rho_inf |V_inf|^2) with the handle's
+ * inf_ruvp and Ma = |V| / sqrt(gamma p / rho) (update_nodeField_other_variables, :420-452).
+ * num_node long, caller's node order; ruvp, Cp or Ma may be NULL.  (The reference forms them
+ * from the primitives refreshed at the start of the last step, one step behind its U; call
+ * this before fvmr_step to reproduce that.) */
+int fvmr_node_field(fvmr_handle *h, double *const ruvp[4], double *Cp, double *Ma);
+/* Pressure force coefficients per boundary set from the node Cp of the current state
+ * (ReferenceData::calculate_edgeSet_force, FieldWriter.cpp:174-205): set s lists the caller's
+ * edge indices set_edges[set_offsets[s] .. set_offsets[s+1]-1] in set order (the reference's
+ * BoundaryV2::edgeSets); Cx[s] = sum of 0.5 (Cp_0 + Cp_1) (1/S_ref) L nx (the reference's CD),
+ * Cy[s] likewise with ny (CL).  Edges must belong to a boundary set. */
+int fvmr_surface_force(fvmr_handle *h, int nsets, const int *set_offsets, const int *set_edges, double S_ref,
+                       double *Cx, double *Cy);
+
+/* ---------------------------------------------------------------------- */
 /* measurement                                                             */
 /* ---------------------------------------------------------------------- */
 /* 1: record HIP events around every kernel launch on the handle's stream. */

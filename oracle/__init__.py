@@ -48,6 +48,8 @@ _SIG = {
     "orc_u2ruvp": (None, [_dp, C.c_double, _dp]),
     "orc_ruvp2u": (None, [_dp, C.c_double, _dp]),
     "orc_inv2x2": (None, [_dp, C.c_int]),
+    "orc_node_field": (None, [C.POINTER(MeshSoA), _dp * 4, C.c_double, _dp, _dp * 4, _dp, _dp]),
+    "orc_surface_force": (None, [C.POINTER(MeshSoA), _dp, C.c_int, _ip, _ip, C.c_double, _dp, _dp]),
 }
 _lib = None
 
@@ -270,3 +272,32 @@ def inv2x2(m, compat=False):
     m, pm = vec(np.array(m, dtype=np.float64).reshape(4))
     lib().orc_inv2x2(pm, 1 if compat else 0)
     return m
+
+
+def node_field(soa, U, gamma, ruvp_inf):
+    """Node ruvp (4 x num_node), Cp and Ma as the reference's FieldWriter forms them."""
+    nv = soa.num_node
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    R = np.zeros((4, nv))
+    cp = np.zeros(nv)
+    ma = np.zeros(nv)
+    inf, pinf = vec(np.asarray(ruvp_inf, dtype=np.float64))
+    lib().orc_node_field(C.byref(soa), (_dp * 4)(*[U[k].ctypes.data_as(_dp) for k in range(4)]), gamma, pinf,
+                         (_dp * 4)(*[R[k].ctypes.data_as(_dp) for k in range(4)]), cp.ctypes.data_as(_dp),
+                         ma.ctypes.data_as(_dp))
+    return R, cp, ma
+
+
+def surface_force(soa, Cp, sets, S_ref):
+    """Per boundary set (a list of edge-index lists, set order) the pressure force
+    coefficients (CD, CL) of FieldWriter::ReferenceData::calculate_edgeSet_force."""
+    off = np.zeros(len(sets) + 1, dtype=np.int32)
+    off[1:] = np.cumsum([len(e) for e in sets])
+    edges = np.ascontiguousarray(np.concatenate([np.asarray(e, dtype=np.int32) for e in sets])
+                                 if sets else np.zeros(0, np.int32))
+    Cp = np.ascontiguousarray(Cp, dtype=np.float64)
+    cd = np.zeros(len(sets))
+    cl = np.zeros(len(sets))
+    lib().orc_surface_force(C.byref(soa), Cp.ctypes.data_as(_dp), len(sets), off.ctypes.data_as(_ip),
+                            edges.ctypes.data_as(_ip), S_ref, cd.ctypes.data_as(_dp), cl.ctypes.data_as(_dp))
+    return cd, cl

@@ -1199,6 +1199,64 @@ void orc_llf(const double *UL, const double *UR, double nx, double ny, double le
 void orc_farfield(const double *UL, const double *ruvp_inf, double nx, double ny, double gamma, double *UR) { ghost_farfield(UL, ruvp_inf, UR, nx, ny, gamma); }
 void orc_slip(const double *UL, double nx, double ny, double *UR) { ghost_slip(UL, UR, nx, ny); }
 void orc_u2ruvp(const double *U, double gamma, double *r) { U2ruvp(U, r, gamma); }
+
+/* FieldWriter::update_nodeField_ruvp (FVMR/src/output/FieldWriter.cpp:371-418): node ruvp =
+ * mean of the ruvp of the cells listing the node, accumulated in element order, slot order;
+ * then update_nodeField_other_variables (:420-452): Cp = (p - p_inf) / (0.5 rho_inf V_inf^2)
+ * (ReferenceData::update_farfield, :544-553) and Ma = sqrt(V^2 / (gamma p / rho)).
+ * Cp or Ma may be NULL. */
+void orc_node_field(const fvmr_mesh_soa *m, const double *const *U, double gamma, const double *ruvp_inf,
+                    double *const *node_ruvp, double *Cp, double *Ma) {
+    const int nv = m->num_node, ne = m->num_element;
+    std::vector<int> cnt(nv, 0);
+    for (int k = 0; k < 4; k++) std::fill(node_ruvp[k], node_ruvp[k] + nv, 0.0);
+    for (int ie = 0; ie < ne; ie++) {
+        double Ue[4] = {U[0][ie], U[1][ie], U[2][ie], U[3][ie]}, r[4];
+        U2ruvp(Ue, r, gamma);
+        for (int j = 0; j < 4; j++) {
+            const int v = m->element_nodes[j][ie];
+            if (v < 0 || v >= nv) continue;
+            cnt[v]++;
+            for (int k = 0; k < 4; k++) node_ruvp[k][v] += r[k];
+        }
+    }
+    for (int v = 0; v < nv; v++) {
+        if (cnt[v] == 0) continue;
+        for (int k = 0; k < 4; k++) node_ruvp[k][v] /= cnt[v];
+    }
+    const double V2f = ruvp_inf[1] * ruvp_inf[1] + ruvp_inf[2] * ruvp_inf[2];
+    const double p_inf = ruvp_inf[3], p_dyn = 0.5 * ruvp_inf[0] * V2f;
+    for (int v = 0; v < nv; v++) {
+        const double rho = node_ruvp[0][v], u = node_ruvp[1][v], w = node_ruvp[2][v], p = node_ruvp[3][v];
+        if (Cp) Cp[v] = (p - p_inf) / p_dyn;
+        if (Ma) {
+            const double V2 = u * u + w * w;
+            const double a2 = gamma * p / rho;
+            Ma[v] = std::sqrt(V2 / a2);
+        }
+    }
+}
+
+/* FieldWriter::ReferenceData::calculate_edgeSet_force (FVMR/src/output/FieldWriter.cpp:174-205):
+ * per boundary set, over its edges in set order, Cp_e = 0.5 (Cp_0 + Cp_1),
+ * dC = Cp_e (1/S_ref) L; CD += dC nx; CL += dC ny.  Set s lists edges off[s] .. off[s+1]-1. */
+void orc_surface_force(const fvmr_mesh_soa *m, const double *Cp, int nsets, const int *off, const int *edges,
+                       double S_ref, double *CD, double *CL) {
+    const double one_on_Sref = 1 / S_ref;
+    for (int s = 0; s < nsets; s++) {
+        CD[s] = 0.0;
+        CL[s] = 0.0;
+        for (int k = off[s]; k < off[s + 1]; k++) {
+            const int e = edges[k];
+            const int n0 = m->edge_nodes[0][e], n1 = m->edge_nodes[1][e];
+            const double nx = m->edge_normal[0][e], ny = m->edge_normal[1][e], dS = m->edge_length[e];
+            const double cp = 0.5 * (Cp[n0] + Cp[n1]);
+            const double dC = cp * one_on_Sref * dS;
+            CD[s] += dC * nx;
+            CL[s] += dC * ny;
+        }
+    }
+}
 void orc_ruvp2u(const double *r, double gamma, double *U) { ruvp2U(r, U, gamma); }
 void orc_inv2x2(double *m, int compat) { inv2x2(m, compat != 0); }
 

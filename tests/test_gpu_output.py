@@ -1,0 +1,57 @@
+"""Output quantities formed on the device (node fields, surface force) against the oracle,
+through the C ABI.  The node field kernel uses the reference's division forms without
+multiply-add contraction, so values agree to the last bit or an ulp (sqrt of Ma)."""
+import numpy as np
+import pytest
+
+import cfd_solver_2d_gpu_amd as F
+import oracle as O
+from helpers import cylinder_case
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def maxrel(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def test_node_field_and_surface_force_cylinder(golden_dir):
+    announce("node fields and surface force on the cylinder continue-file state")
+    m, p, U = cylinder_case(golden_dir)
+    g = F.GPUSolver2(m, p)
+    g.set_state(U)
+    R, cp, ma = g.node_field()
+    Ro, cpo, mao = O.node_field(m.soa, U, p.gamma, list(p.inf_ruvp))
+    assert np.array_equal(R, Ro), maxrel(R, Ro)
+    assert np.array_equal(cp, cpo), maxrel(cp, cpo)
+    assert maxrel(ma, mao) < 1e-15
+    sets = [s[2] for s in m.boundary_sets()]
+    cx, cy = g.surface_force(1.0)
+    cxo, cyo = O.surface_force(m.soa, cpo, sets, 1.0)
+    assert np.array_equal(cx, cxo) and np.array_equal(cy, cyo), (cx, cxo, cy, cyo)
+    # after steps the node field follows the device state
+    g.advance(2, 0.0, 1e9)
+    U2 = g.get_state()
+    R2, _, _ = g.node_field()
+    R2o, _, _ = O.node_field(m.soa, U2, p.gamma, list(p.inf_ruvp))
+    assert np.array_equal(R2, R2o)
+    g.close()
+
+
+def test_surface_force_rejects_interior_edges():
+    announce("surface force: an interior edge is refused with a message")
+    m = F.Mesh.generate(F.GEN_CYLINDER_OGRID, 32, 16, a=0.5, b=6.0, c=1.1)
+    p = F.default_params(inf_ruvp=[1.0, 0.5, 0.0, 1.0])
+    g = F.GPUSolver2(m, p)
+    g.set_state(F.field_init(m, p, F.INIT_UNIFORM))
+    interior = int(np.nonzero(m.arrays()["edge_setID"] == -1)[0][0])
+    with pytest.raises(F.FvmrError) as e:
+        g.surface_force(1.0, sets=[[interior]])
+    assert "not in a boundary set" in str(e.value)
+    cx, cy = g.surface_force(1.0)
+    assert np.all(np.abs(cx) < 1e-12) and np.all(np.abs(cy) < 1e-12)   # uniform pressure
+    g.close()
