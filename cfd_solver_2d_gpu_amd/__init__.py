@@ -65,6 +65,27 @@ class Mesh:
         return cls(h.value)
 
     @classmethod
+    def read_continue(cls, path, equation=EQ_EULER):
+        """ContinueFileReader (FVMR/src/input/ContinueFileReader.cpp:19-158,457-494): returns
+        (mesh, U 4 x num_element, t, istep, CFL) of a pause_*.dat continue file."""
+        h = C.c_void_p()
+        t, cfl = C.c_double(), C.c_double()
+        istep = C.c_int()
+        check(lib().fvmr_mesh_read_continue(str(path).encode(), equation, C.byref(h), C.byref(t), C.byref(istep),
+                                            C.byref(cfl)), "fvmr_mesh_read_continue")
+        m = cls(h.value)
+        U = np.zeros((4, m.num_element))
+        U, rows = _rows(U)
+        check(lib().fvmr_mesh_continue_state(m._h, rows), "fvmr_mesh_continue_state")
+        return m, U, t.value, istep.value, cfl.value
+
+    def write_continue(self, path, U, t, istep, CFL):
+        """FieldWriter::writeContinueFile_1 layout with 17 significant digits."""
+        U, rows = _rows(U)
+        check(lib().fvmr_write_continue(self._h, str(path).encode(), float(t), int(istep), float(CFL), rows),
+              "fvmr_write_continue")
+
+    @classmethod
     def generate(cls, family, ni, nj, a=0.0, b=0.0, c=0.0, d=0.0, jitter=0.0, seed=0, naca_digits=12,
                  equation=EQ_EULER, isothermal_wall=False):
         g = GenParams(ni=ni, nj=nj, a=a, b=b, c=c, d=d, jitter=jitter, seed=seed, naca_digits=naca_digits,

@@ -85,6 +85,9 @@ SIGNATURES = {
                                        C.POINTER(C.c_char_p), _ip, _ip, C.c_int, C.POINTER(C.c_void_p)]),
     "fvmr_mesh_generate": (C.c_int, [C.c_int, C.POINTER(GenParams), C.c_int, C.POINTER(C.c_void_p)]),
     "fvmr_mesh_write_su2": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "fvmr_mesh_read_continue": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p), _dp, _ip, _dp]),
+    "fvmr_mesh_continue_state": (C.c_int, [C.c_void_p, _dp * 4]),
+    "fvmr_write_continue": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_int, C.c_double, _dp * 4]),
     "fvmr_mesh_soa_view": (C.c_int, [C.c_void_p, C.POINTER(MeshSoA)]),
     "fvmr_mesh_boundary_set": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), _ip,
                                          C.POINTER(_ip), _ip]),

@@ -133,6 +133,7 @@ struct fvmr_mesh {
     std::vector<double> glen, gx, gy, gnx, gny;
     std::vector<int> btype;
     fvmr_mesh_soa soa{};
+    std::vector<double> cont_U[4];  // element states of a continue file (fvmr_mesh_read_continue)
 };
 
 namespace fvmr {
@@ -558,6 +559,132 @@ int fvmr_ruvp_from_mach(double Ma, double AoA, double T0, double p0, double gamm
     ruvp[2] = c * Ma * std::sin(AoA / 180 * PI);
     ruvp[3] = p;
     return FVMR_OK;
+}
+
+// ContinueFileReader::readContinueFile_1 (FVMR/src/input/ContinueFileReader.cpp:19-158) and
+// process (:457-494): nodes (ID, x, y), elements (ID, node IDs, U), boundary sets (ID, name,
+// then edge IDs into the first-appearance edge list).  The edges are those of FVM_2D::iniEdges
+// over the element lists, so the mesh is built once without sets to number the edges, and the
+// set edge IDs become node pairs for the final build.
+int fvmr_mesh_read_continue(const char *path, int equation, fvmr_mesh **out, double *t, int *istep, double *CFL) {
+    if (!path || !out) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    FILE *f = fopen(path, "rb");
+    if (!f) return set_error(FVMR_ERR_IO, std::string("cannot open file ") + path);
+    fvmr_mesh *m = new fvmr_mesh();
+    fvmr::RawLists &r = m->raw;
+    struct SetIn { int id; std::string name; std::vector<int> edges; };
+    std::vector<SetIn> sets;
+    double t0 = 0.0, cfl0 = 0.0;
+    int step0 = 0;
+    int state = -1, rc = FVMR_OK;
+    std::vector<char> line(1 << 16);
+    const char *w[16]; int wl[16];
+    while (fgets(line.data(), (int)line.size(), f)) {
+        const int n = fvmr::split_words(line.data(), w, wl, 16);
+        if (n == 0) continue;
+        if (fvmr::word_is(w[0], wl[0], "t,")) { state = 0; continue; }
+        if (fvmr::word_is(w[0], wl[0], "nodes:")) { state = 1; continue; }
+        if (fvmr::word_is(w[0], wl[0], "edges:")) { state = 2; continue; }
+        if (fvmr::word_is(w[0], wl[0], "elements:")) { state = 3; continue; }
+        if (fvmr::word_is(w[0], wl[0], "boundaries:")) { state = 4; continue; }
+        if (state == 0) {
+            if (n < 2) { rc = set_error(FVMR_ERR_MESH, "continue file: bad header line"); break; }
+            t0 = fvmr::word_d(w[0], wl[0]);
+            step0 = (int)fvmr::word_d(w[1], wl[1]);
+            if (n >= 3) cfl0 = fvmr::word_d(w[2], wl[2]);
+        } else if (state == 1) {
+            if (n < 3) { rc = set_error(FVMR_ERR_MESH, "continue file: bad node line"); break; }
+            const int id = (int)fvmr::word_d(w[0], wl[0]);
+            r.id1.push_back(id);
+            r.max_id = std::max(r.max_id, id);
+            r.x.push_back(fvmr::word_d(w[1], wl[1]));
+            r.y.push_back(fvmr::word_d(w[2], wl[2]));
+        } else if (state == 3) {
+            if (n < 8) { rc = set_error(FVMR_ERR_MESH, "continue file: bad element line"); break; }
+            const int a = (int)fvmr::word_d(w[1], wl[1]), b = (int)fvmr::word_d(w[2], wl[2]), c = (int)fvmr::word_d(w[3], wl[3]);
+            r.tri.push_back({a, b, c});
+            r.etype.push_back(5);
+            r.econn.push_back({a - 1, b - 1, c - 1, -1});
+            for (int k = 0; k < 4; k++) m->cont_U[k].push_back(fvmr::word_d(w[4 + k], wl[4 + k]));
+        } else if (state == 4) {
+            // a set header "ID name" (second word not a number) or a line of edge IDs
+            if (n >= 2 && !isdigit((unsigned char)w[1][0])) {
+                sets.push_back({(int)fvmr::word_d(w[0], wl[0]), std::string(w[1], wl[1]), {}});
+            } else {
+                if (sets.empty()) { rc = set_error(FVMR_ERR_MESH, "continue file: edge IDs before a set"); break; }
+                for (int k = 0; k < n; k++) sets.back().edges.push_back((int)fvmr::word_d(w[k], wl[k]));
+            }
+        }
+    }
+    fclose(f);
+    if (!rc && (r.x.empty() || r.tri.empty())) rc = set_error(FVMR_ERR_MESH, "continue file without nodes or elements");
+    if (!rc) {
+        // number the edges (first appearance) and turn the set edge IDs into node-ID pairs
+        fvmr_mesh tmp;
+        tmp.raw = r;
+        tmp.raw.markers.clear();
+        rc = fvmr::build(tmp, equation);
+        for (size_t s = 0; !rc && s < sets.size(); s++) {
+            fvmr::Marker mk;
+            mk.name = sets[s].name;
+            for (int id : sets[s].edges) {
+                const int e = id - 1;
+                if (e < 0 || e >= (int)tmp.gn0.size()) { rc = set_error(FVMR_ERR_MESH, "continue file: edge ID out of range"); break; }
+                mk.e.push_back({r.id1[tmp.gn0[e]], r.id1[tmp.gn1[e]]});
+            }
+            r.markers.push_back(mk);
+        }
+    }
+    if (!rc) rc = fvmr::build(*m, equation);
+    if (rc) { delete m; return rc; }
+    if (t) *t = t0;
+    if (istep) *istep = step0;
+    if (CFL) *CFL = cfl0;
+    *out = m;
+    return FVMR_OK;
+}
+
+int fvmr_mesh_continue_state(const fvmr_mesh *m, double *const U[4]) {
+    if (!m || !U) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
+    if (m->cont_U[0].size() != m->evol.size()) return set_error(FVMR_ERR_INVALID_ARG, "the mesh was not read from a continue file");
+    for (int k = 0; k < 4; k++) {
+        if (!U[k]) return set_error(FVMR_ERR_INVALID_ARG, "null state component");
+        std::copy(m->cont_U[k].begin(), m->cont_U[k].end(), U[k]);
+    }
+    return FVMR_OK;
+}
+
+// FieldWriter::writeContinueFile_1 (FVMR/src/output/FieldWriter.cpp:264-326) in the same layout,
+// numbers with 17 significant digits so that a restart reproduces the state bit for bit.
+int fvmr_write_continue(const fvmr_mesh *m, const char *path, double t, int istep, double CFL, const double *const U[4]) {
+    if (!m || !path || !U) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
+    for (int k = 0; k < 4; k++)
+        if (!U[k]) return set_error(FVMR_ERR_INVALID_ARG, "null state component");
+    FILE *f = fopen(path, "w");
+    if (!f) return set_error(FVMR_ERR_IO, std::string("cannot open file ") + path);
+    const fvmr::RawLists &r = m->raw;
+    fprintf(f, "t, istep, CFL\n%.17g %d %.17g\nnodes: ID, x, y\n", t, istep, CFL);
+    for (size_t i = 0; i < r.x.size(); i++) fprintf(f, "%d %.17g %.17g\n", r.id1[i], r.x[i], r.y[i]);
+    fprintf(f, "elements: ID, nodes, U\n");
+    const size_t ne = m->evol.size();
+    for (size_t ie = 0; ie < ne; ie++)
+        fprintf(f, "%zu %d %d %d %.17g %.17g %.17g %.17g\n", ie + 1, m->elem_ids3[3 * ie], m->elem_ids3[3 * ie + 1],
+                m->elem_ids3[3 * ie + 2], U[0][ie], U[1][ie], U[2][ie], U[3][ie]);
+    fprintf(f, "boundaries: ID, name; edgeIDs\n");
+    for (const BoundarySet &bs : m->sets) {
+        fprintf(f, "%d %s\n", bs.ID, bs.name.c_str());
+        bool at_line_end = false;
+        for (size_t k = 0; k < bs.edges.size(); k++) {
+            fprintf(f, "%d", bs.edges[k] + 1);
+            at_line_end = (k % 10 == 9);
+            fputc(at_line_end ? '\n' : ' ', f);
+        }
+        if (!at_line_end) fputc('\n', f);
+    }
+    const bool ok = !ferror(f);
+    fclose(f);
+    return ok ? FVMR_OK : set_error(FVMR_ERR_IO, std::string("write failed: ") + path);
 }
 
 int fvmr_mesh_read_su2(const char *path, int equation, double scale, fvmr_mesh **out) {

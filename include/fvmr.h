@@ -215,6 +215,16 @@ int fvmr_mesh_from_lists(int num_node, const double *x, const double *y, const i
 int fvmr_mesh_generate(int family, const fvmr_gen_params *g, int equation, fvmr_mesh **out);
 /* Write a mesh produced by fvmr_mesh_generate/fvmr_mesh_from_lists as SU2 text. */
 int fvmr_mesh_write_su2(const fvmr_mesh *m, const char *path);
+/* ContinueFileReader::readContinueFile_1 + process (FVMR/src/input/ContinueFileReader.cpp:19-158,
+ * 457-494): the mesh of a continue file (nodes, element node lists, boundary sets given by edge
+ * ID) and its header; t, istep, CFL may be NULL.  fvmr_mesh_continue_state copies the element
+ * states it holds (caller's element order = file order). */
+int fvmr_mesh_read_continue(const char *path, int equation, fvmr_mesh **out, double *t, int *istep, double *CFL);
+int fvmr_mesh_continue_state(const fvmr_mesh *m, double *const U[4]);
+/* FieldWriter::writeContinueFile_1 (FVMR/src/output/FieldWriter.cpp:264-326): the reference's
+ * layout (header, nodes, elements with node IDs and U, boundary sets with edge IDs ten per
+ * line); numbers with 17 significant digits, so a restart reproduces the state bit for bit. */
+int fvmr_write_continue(const fvmr_mesh *m, const char *path, double t, int istep, double CFL, const double *const U[4]);
 int fvmr_mesh_soa_view(const fvmr_mesh *m, fvmr_mesh_soa *out);
 /* BoundaryV2 edge sets (FVMR/src/gpu/datatype/BoundaryV2.h:13-23): name, type and edge list of set s (0-based). */
 int fvmr_mesh_boundary_set(const fvmr_mesh *m, int s, const char **name, int *type,

@@ -55,3 +55,39 @@ def test_surface_force_rejects_interior_edges():
     cx, cy = g.surface_force(1.0)
     assert np.all(np.abs(cx) < 1e-12) and np.all(np.abs(cy) < 1e-12)   # uniform pressure
     g.close()
+
+
+def test_checkpoint_and_restart_through_a_continue_file(golden_dir, tmp_path):
+    announce("restart: reference continue file -> 2 steps -> written continue file -> 2 more steps")
+    m, U, t, istep, cfl = F.Mesh.read_continue(golden_dir / "pause_cylinder_allfar_50_20000.dat", equation=F.EQ_NS)
+    inf = F.ruvp_from_mach(0.5, 0.0)
+    u_ref = float(np.hypot(inf[1], inf[2]))
+    p = F.default_params(equation=F.EQ_NS, inf_ruvp=inf, inlet_ruvp=inf, outlet_ruvp=inf, CFL=cfl,
+                         Pr=0.73, mu0=inf[0] * u_ref * 1.0 / 6.5e6)
+    g = F.GPUSolver2(m, p)
+    g.set_state(U)
+    o = O.OracleSolver(m, p)
+    o.set_state(U)
+    tg = to = t
+    for _ in range(2):
+        tg, _ = g.iteration_device(tg, 1e9)
+        to, _ = o.step(to, 1e9)
+    assert abs(tg / to - 1) < 1e-12
+    Ug = g.get_state()
+    assert np.linalg.norm(Ug - o.get_state()) / np.linalg.norm(o.get_state()) < 1e-12
+    path = tmp_path / "pause_restart[20002].dat"
+    m.write_continue(path, Ug, tg, istep + 2, cfl)
+    for _ in range(2):
+        tg, _ = g.iteration_device(tg, 1e9)
+    straight = g.get_state()
+    g.close()
+    m2, U2, t2, i2, _ = F.Mesh.read_continue(path, equation=F.EQ_NS)
+    assert i2 == istep + 2 and np.array_equal(U2, Ug)
+    g2 = F.GPUSolver2(m2, p)
+    g2.set_state(U2)
+    t3 = t2
+    for _ in range(2):
+        t3, _ = g2.iteration_device(t3, 1e9)
+    assert t3 == tg
+    assert np.array_equal(g2.get_state(), straight)
+    g2.close()
