@@ -134,6 +134,15 @@ __device__ __forceinline__ int xcd_block() {
 template <class T> __device__ __forceinline__ T tmin(T a, T b) { return (a < b) ? a : b; }
 template <class T> __device__ __forceinline__ T tmax(T a, T b) { return (a > b) ? a : b; }
 
+// Division, reciprocal and square root of the face and gradient arithmetic: IEEE in FP64;
+// the hardware's approximate forms (about an ulp) in the FP32 of mixed precision.
+template <class T> __device__ __forceinline__ T fdiv(T a, T b) { return a / b; }
+template <> __device__ __forceinline__ float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <class T> __device__ __forceinline__ T frcp(T a) { return T(1) / a; }
+template <> __device__ __forceinline__ float frcp<float>(float a) { return __builtin_amdgcn_rcpf(a); }
+template <class T> __device__ __forceinline__ T fsqrt(T a) { return sqrt(a); }
+template <> __device__ __forceinline__ float fsqrt<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
+
 __device__ __forceinline__ void ld(const S4 *p, int i, double u[4]) {
     const S4 v = p[i];
     u[0] = v.x; u[1] = v.y; u[2] = v.z; u[3] = v.w;
@@ -150,7 +159,7 @@ __device__ __forceinline__ void ruvp2u(const T r[4], T U[4], T gamma) {
     U[0] = r[0];
     U[1] = r[0] * r[1];
     U[2] = r[0] * r[2];
-    U[3] = r[3] / (gamma - 1) + T(0.5) * r[0] * (r[1] * r[1] + r[2] * r[2]);
+    U[3] = fdiv(r[3], gamma - 1) + T(0.5) * r[0] * (r[1] * r[1] + r[2] * r[2]);
 }
 // outOfRange_device (RestrictGPU.h:11-19)
 template <class T>
@@ -181,7 +190,7 @@ struct FaceState {
 
 template <class C>
 __device__ __forceinline__ void set_prim(FaceState<C> &s, C gamma) {
-    s.ir = C(1) / s.U[0];
+    s.ir = frcp(s.U[0]);
     s.r = s.U[0];
     s.u = s.U[1] * s.ir;
     s.v = s.U[2] * s.ir;
@@ -254,15 +263,15 @@ __device__ __forceinline__ void ghost_farfield(const FaceState<C> &L, const doub
     const C ga1 = gamma - 1;
     const C rho_f = (C)ruvp_inf[0], u_f = (C)ruvp_inf[1], v_f = (C)ruvp_inf[2], p_f = (C)ruvp_inf[3];
     const C rho_e = L.r, u_e = L.u, v_e = L.v, p_e = L.p;
-    const C a2_f = gamma * p_f / rho_f;
-    const C af = sqrt(a2_f);
-    const C ae = sqrt(gamma * p_e / rho_e);
+    const C a2_f = fdiv(gamma * p_f, rho_f);
+    const C af = fsqrt(a2_f);
+    const C ae = fsqrt(fdiv(gamma * p_e, rho_e));
     const C qnf = u_f * nx + v_f * ny;
     const C qne = u_e * nx + v_e * ny;
-    const C rf = qnf - C(2.0) * af / ga1;
-    const C re = qne + C(2.0) * ae / ga1;
+    const C rf = qnf - fdiv(C(2.0) * af, ga1);
+    const C re = qne + fdiv(C(2.0) * ae, ga1);
     const C qn = C(0.5) * (re + rf);
-    const C as = ga1 * (re - rf) / C(4.0);
+    const C as = fdiv(ga1 * (re - rf), C(4.0));
     C r0[4];
     if (qn <= -as) {
         r0[0] = rho_f; r0[1] = u_f; r0[2] = v_f; r0[3] = p_f;
@@ -272,11 +281,11 @@ __device__ __forceinline__ void ghost_farfield(const FaceState<C> &L, const doub
         const bool in = qn < 0;
         const C s_p = in ? p_f : p_e, s_r = in ? rho_f : rho_e;
         const C s_u = in ? u_f : u_e, s_v = in ? v_f : v_e, s_qn = in ? qnf : qne;
-        const C son = s_p / pow(s_r, gamma);
+        const C son = fdiv(s_p, pow(s_r, gamma));
         const C qtx = s_u - s_qn * nx, qty = s_v - s_qn * ny;
-        r0[0] = pow((as * as / son / gamma), C(1.0) / ga1);
+        r0[0] = pow(fdiv(fdiv(as * as, son), gamma), frcp(ga1));
         r0[1] = qtx + qn * nx; r0[2] = qty + qn * ny;
-        r0[3] = as * as * r0[0] / gamma;
+        r0[3] = fdiv(as * as * r0[0], gamma);
     } else {
         r0[0] = rho_e; r0[1] = u_e; r0[2] = v_e; r0[3] = p_e;
     }
@@ -287,10 +296,10 @@ __device__ __forceinline__ void ghost_farfield(const FaceState<C> &L, const doub
 // New: no-slip isothermal wall ghost (u_R = -u_L, p_R = p_L, T_R = 2 T_w - T_L, floored).
 template <class C>
 __device__ __forceinline__ void ghost_isothermal(const FaceState<C> &L, FaceState<C> &R, C Tw, C Rg, C gamma) {
-    const C TL = L.p / (L.r * Rg);
+    const C TL = fdiv(L.p, L.r * Rg);
     C TR = C(2.0) * Tw - TL;
     if (TR < C(0.5) * Tw) TR = C(0.5) * Tw;
-    const C g[4] = {L.p / (Rg * TR), -L.u, -L.v, L.p};
+    const C g[4] = {fdiv(L.p, Rg * TR), -L.u, -L.v, L.p};
     ruvp2u(g, R.U, gamma);
     set_prim(R, gamma);
 }
@@ -307,7 +316,7 @@ __device__ __forceinline__ void flux_llf(const FaceState<C> &L, const FaceState<
     FnR[0] = R.r * unR; FnR[1] = R.r * unR * R.u + nx * R.p; FnR[2] = R.r * unR * R.v + ny * R.p;
     const C hr = R.p * g_ga1 * R.ir + C(0.5) * (R.u * R.u + R.v * R.v);
     FnR[3] = R.r * unR * hr;
-    const C lam = tmax(fabs(unL) + sqrt(gamma * L.p * L.ir), fabs(unR) + sqrt(gamma * R.p * R.ir));
+    const C lam = tmax(fabs(unL) + fsqrt(gamma * L.p * L.ir), fabs(unR) + fsqrt(gamma * R.p * R.ir));
 #pragma unroll
     for (int i = 0; i < 4; i++) flux[i] = C(0.5) * (FnL[i] + FnR[i] - lam * (R.U[i] - L.U[i])) * length;
 }
@@ -330,16 +339,16 @@ __device__ __forceinline__ void flux_roe(const FaceState<C> &L, const FaceState<
     F[1] = C(0.5) * (rul * uL + rur * uR + nx * psum) * area;
     F[2] = C(0.5) * (rul * vL + rur * vR + ny * psum) * area;
     F[3] = C(0.5) * (rul * HL + rur * HR) * area;
-    const C rrorl = sqrt(rR * L.ir);
-    const C inv_p1 = C(1.0) / (rrorl + C(1.0));
+    const C rrorl = fsqrt(rR * L.ir);
+    const C inv_p1 = frcp(rrorl + C(1.0));
     const C rm = rL * rrorl;
     const C um = (uL + uR * rrorl) * inv_p1;
     const C vm = (vL + vR * rrorl) * inv_p1;
     const C Hm = (HL + HR * rrorl) * inv_p1;
     const C Vm2 = um * um + vm * vm;
     const C am2 = ga1 * (Hm - C(0.5) * Vm2);
-    const C am = sqrt(am2);
-    const C inv_am = C(1.0) / am;
+    const C am = fsqrt(am2);
+    const C inv_am = frcp(am);
     const C unm = um * nx + vm * ny;
     const C eig1 = fabs(unm);
     const C eig2 = fabs(unm + am);
@@ -353,7 +362,7 @@ __device__ __forceinline__ void flux_roe(const FaceState<C> &L, const FaceState<
     const C astar = (eig2 + eig3) * C(0.5);
     const C Mstar = (eig2 - eig3) * C(0.5) * inv_am;
     const C dun = nx * (uR - uL) + ny * (vR - vL);
-    const C dUroe = Mstar * dun + (astar - eig1) * (pR - pL) / rm * (inv_am * inv_am);
+    const C dUroe = Mstar * dun + fdiv((astar - eig1) * (pR - pL), rm) * (inv_am * inv_am);
     const C dProe = Mstar * (pR - pL) + (astar - eig1) * rm * dun;
     F[0] -= C(0.5) * (eig1 * dW0 + dUroe * rm) * area;
     F[1] -= C(0.5) * (eig1 * dW1 + dUroe * rm * um + dProe * nx) * area;
@@ -372,7 +381,7 @@ template <class C, class G, class Cells>
 __device__ __forceinline__ void flux_viscous(const Cells &cells, const Phys &ph, int L, int R, C nx, C ny, C area,
                                              C F[4]) {
     const C ga1 = (C)ph.gamma - C(1.0);
-    const C Cv = (C)ph.R / ga1;
+    const C Cv = fdiv((C)ph.R, ga1);
     const C Cp = Cv * (C)ph.gamma;
     const bool hasR = cells.has(R);
     C gx[4], gy[4], Um[4];
@@ -399,7 +408,7 @@ __device__ __forceinline__ void flux_viscous(const Cells &cells, const Phys &ph,
             }
         }
     }
-    const C one_on_rho = C(1.0) / Um[0];
+    const C one_on_rho = frcp(Um[0]);
     const C u = Um[1] * one_on_rho, v = Um[2] * one_on_rho, E = Um[3] * one_on_rho;
     const C dudx = one_on_rho * (gx[1] - u * gx[0]);
     const C dvdx = one_on_rho * (gx[2] - v * gx[0]);
@@ -407,7 +416,7 @@ __device__ __forceinline__ void flux_viscous(const Cells &cells, const Phys &ph,
     const C dvdy = one_on_rho * (gy[2] - v * gy[0]);
     const C dEdx = one_on_rho * (gx[3] - E * gx[0]);
     const C dEdy = one_on_rho * (gy[3] - E * gy[0]);
-    const C one_on_Cv = C(1.0) / Cv;
+    const C one_on_Cv = frcp(Cv);
     const C V2 = u * u + v * v;
     const C T = one_on_Cv * (E - C(0.5) * V2);
     const C dTdx = (dEdx - u * dudx - v * dvdx) * one_on_Cv;
@@ -418,7 +427,7 @@ __device__ __forceinline__ void flux_viscous(const Cells &cells, const Phys &ph,
     const C txx = mu * (NMD * div + C(2.0) * dudx);
     const C tyy = mu * (NMD * div + C(2.0) * dvdy);
     const C txy = mu * (dudy + dvdx);
-    const C akmu = Cp * (mu / (C)ph.Pr_visc);
+    const C akmu = Cp * fdiv(mu, (C)ph.Pr_visc);
     const C Phix = akmu * dTdx + u * txx + v * txy;
     const C Phiy = akmu * dTdy + u * txy + v * tyy;
     F[0] = C(0.0);
@@ -554,7 +563,7 @@ template <class C, class Slots>
 __device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[4], const Slots &S, const int fl[3], C vol,
                                            C gx[4], C gy[4]) {
     if (ph.gradient == FVMR_GRA_GREENGAUSS) {
-        const C inv_vol = C(1) / vol;
+        const C inv_vol = frcp(vol);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             C s0 = C(0), s1 = C(0);
@@ -589,7 +598,7 @@ __device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[4], const 
             if (det >= C(0) && det < eps) det = eps;
             else if (det < C(0) && det > -eps) det = -eps;
         }
-        const C idet = C(1.0) / det;
+        const C idet = frcp(det);
         const C i00 = M[3] * idet, i01 = -M[1] * idet, i10 = -M[2] * idet, i11 = M[0] * idet;
         const bool any = (fl[0] | fl[1] | fl[2]) & 2;
 #pragma unroll
@@ -608,7 +617,7 @@ __device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[4], const 
     if (ph.limiter != FVMR_LIM_NONE) {
         C eps2 = C(0);
         if (ph.limiter == FVMR_LIM_VENKATAKRISHNAN) {
-            const C Kh = (C)ph.venkat_K * sqrt(vol);
+            const C Kh = (C)ph.venkat_K * fsqrt(vol);
             eps2 = Kh * Kh * Kh;
         }
 #pragma unroll
@@ -634,8 +643,8 @@ __device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[4], const 
                     if (rel > C(0)) dpos = fmax(dpos, rel + C(fvmr::kEPS));
                     else if (rel < C(0)) dneg = fmin(dneg, rel - C(fvmr::kEPS));
                 }
-                if (dpos > C(0) && hrel < dpos) phi = tmin(phi, hrel / dpos);
-                if (dneg < C(0) && lrel > dneg) phi = tmin(phi, lrel / dneg);
+                if (dpos > C(0) && hrel < dpos) phi = tmin(phi, fdiv(hrel, dpos));
+                if (dneg < C(0) && lrel > dneg) phi = tmin(phi, fdiv(lrel, dneg));
             } else if (ph.limiter == FVMR_LIM_BARTH) {
                 // NaN bounds: the vertex loop as the reference writes it
 #pragma unroll
@@ -643,8 +652,8 @@ __device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[4], const 
                     const C Un = Uc[k] + gx[k] * S.N(jn, 0) + gy[k] * S.N(jn, 1);
                     C rel = Un - Uc[k];
                     C pn = C(1.0);
-                    if (rel > C(0)) { rel += C(fvmr::kEPS); pn = tmin(C(1), hrel / rel); }
-                    else if (rel < C(0)) { rel -= C(fvmr::kEPS); pn = tmin(C(1), lrel / rel); }
+                    if (rel > C(0)) { rel += C(fvmr::kEPS); pn = tmin(C(1), fdiv(hrel, rel)); }
+                    else if (rel < C(0)) { rel -= C(fvmr::kEPS); pn = tmin(C(1), fdiv(lrel, rel)); }
                     phi = tmin(phi, pn);
                 }
             } else {
@@ -657,7 +666,7 @@ __device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[4], const 
                         const C d1 = (d2 > C(0)) ? (hi - Uc[k]) : (lo - Uc[k]);
                         const C num = (d1 * d1 + eps2) * d2 + C(2.0) * d2 * d2 * d1;
                         const C den = d1 * d1 + C(2.0) * d2 * d2 + d1 * d2 + eps2;
-                        pn = num / (den * d2);
+                        pn = fdiv(num, den * d2);
                     }
                     phi = tmin(phi, pn);
                 }
@@ -718,10 +727,10 @@ __global__ __launch_bounds__(kGroup3) FVMR_GRAD_ATTR void k_gradient(Mesh<G> m, 
                     Fy += fy - (C)pm.y;
                 }
                 const C ax = fx - Fx, ay = fy - Fy, bx = xc - Fx, by = yc - Fy;
-                const C dFf = sqrt(ax * ax + ay * ay);
-                C dFC = sqrt(bx * bx + by * by);
+                const C dFf = fsqrt(ax * ax + ay * ay);
+                C dFC = fsqrt(bx * bx + by * by);
                 if (dFC < C(fvmr::kEPS)) dFC = C(fvmr::kEPS);
-                gC = dFf / dFC;
+                gC = fdiv(dFf, dFC);
             }
             sA[j][0][lane] = gC;
             sA[j][1][lane] = (C)eg.x * len * sgn;
@@ -871,15 +880,32 @@ __global__ __launch_bounds__(kGroup3) FVMR_STAGE_ATTR void k_stage(Mesh<G> m, Ph
 // and the first ring in LDS, evaluates the flux of every face of the tile once, and sums
 // the fluxes per cell in slot order.  Gradients never go to HBM.
 // ---------------------------------------------------------------------------
-constexpr int kTileThreads = 512;
-constexpr int kTileMaxG = 400;    // cells with a gradient (tile + first ring)
-constexpr int kTileMaxL = 480;    // cells with a state (+ second ring)
-constexpr int kTileMaxN = 320;    // nodes
-constexpr int kTileMaxF = 672;    // faces
-constexpr int kTileMaxFF = kTileThreads;   // faces of tile cells: one per thread
-constexpr int kTileDefaultCells = 300;
-static_assert(kTileMaxL <= fvmr::kLocalNone && kTileMaxN <= fvmr::kLocalNone && kTileMaxF <= fvmr::kLocalNone,
-              "local ids are 10-bit fields");
+// Tile shape per arithmetic precision, sized so that the LDS of a work-group lets as many
+// work-groups share a CU as the registers allow (FP64: 116 VGPRs -> 4 waves per SIMD ->
+// 4 work-groups of 4 waves with <= 40 KB LDS each; FP32: 75 VGPRs -> 6 waves per SIMD ->
+// 3 work-groups of 8 waves with <= 53 KB each).
+template <class C> struct TileCfg;
+template <> struct TileCfg<double> {
+    static constexpr int kThreads = 256;
+    static constexpr int kMaxG = 200;    // cells with a gradient (tile + first ring)
+    static constexpr int kMaxL = 256;    // cells with a state (+ second ring)
+    static constexpr int kMaxN = 176;    // nodes
+    static constexpr int kMaxF = 352;    // faces
+    static constexpr int kMaxFF = kThreads;   // faces of tile cells: one per thread
+    static constexpr int kCells = 150;   // default tile size
+};
+template <> struct TileCfg<float> {
+    static constexpr int kThreads = 512;
+    static constexpr int kMaxG = 400;
+    static constexpr int kMaxL = 480;
+    static constexpr int kMaxN = 320;
+    static constexpr int kMaxF = 672;
+    static constexpr int kMaxFF = kThreads;
+    static constexpr int kCells = 300;
+};
+constexpr int kTileMaxCells = 400;   // largest tile_cells accepted (the FP32 shape's kMaxG)
+static_assert(TileCfg<float>::kMaxL <= fvmr::kLocalNone && TileCfg<float>::kMaxN <= fvmr::kLocalNone &&
+              TileCfg<float>::kMaxF <= fvmr::kLocalNone, "local ids are 10-bit fields");
 
 // Per cell, for the time-step kernel: alphaC code, the 3 slot faces (local ids in the
 // cell's tile), volume.
@@ -958,11 +984,14 @@ struct TileCells {
 
 // One RK stage over the tiles (the work of k_gradient + k_stage, see there).
 template <class C, class G, bool NS>
-__global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs<G> ta, Phys ph, const S4 *__restrict__ Uin,
+__global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, Phys ph, const S4 *__restrict__ Uin,
                                                         const S4 *__restrict__ Un, S4 *__restrict__ Uout,
                                                         S4 *__restrict__ rhs_out, Scalars *__restrict__ sc,
                                                         int *__restrict__ bad_list, int nc, double a, double b,
                                                         int first) {
+    using Cfg = TileCfg<C>;
+    constexpr int kTileThreads = Cfg::kThreads, kTileMaxG = Cfg::kMaxG, kTileMaxL = Cfg::kMaxL;
+    constexpr int kTileMaxN = Cfg::kMaxN, kTileMaxF = Cfg::kMaxF, kTileMaxFF = Cfg::kMaxFF;
     static_assert(kTileMaxL <= kTileThreads && kTileMaxFF <= kTileThreads, "one local cell and one face per thread");
     __shared__ S4 sU[kTileMaxL];
     __shared__ V2<double> sN[kTileMaxN];
@@ -1131,9 +1160,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(TileArgs<G> ta, Phys ph, 
 // spectral radius of every face of the tile once (inviscid part, and the viscous part up
 // to the division by the cell volume), then per cell the sum in the reference edge order.
 template <class G, bool NS>
-__global__ __launch_bounds__(kTileThreads) void k_tile_dt(TileArgs<G> ta, Phys ph, const S4 *__restrict__ U,
+__global__ __launch_bounds__(TileCfg<G>::kThreads) void k_tile_dt(TileArgs<G> ta, Phys ph, const S4 *__restrict__ U,
                                                            double *__restrict__ partial) {
     if (!NS) ph.equation = FVMR_EQ_EULER;   // lets the compiler drop the viscous terms
+    using Cfg = TileCfg<G>;
+    constexpr int kTileThreads = Cfg::kThreads, kTileMaxG = Cfg::kMaxG, kTileMaxN = Cfg::kMaxN;
+    constexpr int kTileMaxFF = Cfg::kMaxFF;
     __shared__ S4 sU[kTileMaxG];
     __shared__ V2<double> sN[kTileMaxN];
     __shared__ int4 sFace[kTileMaxFF];   // faces of the tile cells
@@ -1582,9 +1614,9 @@ int launch_tile_dt(fvmr_solver *h, const S4 *U, int dt_mode) {
     int rc = prof_begin(h, 1, &e0);
     if (rc) return rc;
     if (h->ph.equation == FVMR_EQ_NS)
-        k_tile_dt<G, true><<<h->ntiles, kTileThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, U, h->partial);
+        k_tile_dt<G, true><<<h->ntiles, TileCfg<G>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, U, h->partial);
     else
-        k_tile_dt<G, false><<<h->ntiles, kTileThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, U, h->partial);
+        k_tile_dt<G, false><<<h->ntiles, TileCfg<G>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, U, h->partial);
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 1, e0))) return rc;
     if ((rc = prof_begin(h, 2, &e0))) return rc;
@@ -1600,10 +1632,10 @@ int launch_tile_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, dou
     int rc = prof_begin(h, 0, &e0);
     if (rc) return rc;
     if (h->ph.equation == FVMR_EQ_NS)
-        k_tile<C, G, true><<<h->ntiles, kTileThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, Uin, Un, Uout, rhs_out,
+        k_tile<C, G, true><<<h->ntiles, TileCfg<C>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, Uin, Un, Uout, rhs_out,
                                                                      h->sc, h->bad_list, h->nc, a, b, first);
     else
-        k_tile<C, G, false><<<h->ntiles, kTileThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, Uin, Un, Uout, rhs_out,
+        k_tile<C, G, false><<<h->ntiles, TileCfg<C>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, Uin, Un, Uout, rhs_out,
                                                                       h->sc, h->bad_list, h->nc, a, b, first);
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 0, e0))) return rc;
@@ -1708,7 +1740,7 @@ int validate_params(const fvmr_params *p) {
     if (p->renumber < FVMR_RENUMBER_NONE || p->renumber > FVMR_RENUMBER_HILBERT)
         return set_error(FVMR_ERR_INVALID_ARG, "invalid renumber");
     if (p->kernels < FVMR_KERNELS_AUTO || p->kernels > FVMR_KERNELS_TILE) return set_error(FVMR_ERR_INVALID_ARG, "invalid kernels");
-    if (p->tile_cells < 0 || p->tile_cells > kTileMaxG) return set_error(FVMR_ERR_INVALID_ARG, "tile_cells out of [0, 400]");
+    if (p->tile_cells < 0 || p->tile_cells > kTileMaxCells) return set_error(FVMR_ERR_INVALID_ARG, "tile_cells out of [0, 400]");
     return FVMR_OK;
 }
 
@@ -1894,8 +1926,12 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
             fvmr::TileInput ti{nc, nf, sx.data(), sy.data(), cnode.data(), eL.data(), eR.data(), et.data(), ep.data(),
                                en0.data(), en1.data(), elen.data(), m->node_xy[0], m->node_xy[1], vol.data(), dtcode.data(),
                                ccx.data(), ccy.data(), emx.data(), emy.data()};
-            const fvmr::TileCaps caps{p->tile_cells ? p->tile_cells : kTileDefaultCells, kTileMaxG, kTileMaxL, kTileMaxN,
-                                      kTileMaxF, kTileMaxFF};
+            auto caps_of = [&](auto cfg) {
+                using Cfg = decltype(cfg);
+                return fvmr::TileCaps{std::min(p->tile_cells ? p->tile_cells : Cfg::kCells, Cfg::kMaxG), Cfg::kMaxG,
+                                      Cfg::kMaxL, Cfg::kMaxN, Cfg::kMaxF, Cfg::kMaxFF};
+            };
+            const fvmr::TileCaps caps = h->mixed ? caps_of(TileCfg<float>{}) : caps_of(TileCfg<double>{});
             if ((rc = fvmr::build_tiles(ti, caps, tm))) return bail(rc);
             h->tiles = true;
             h->ntiles = tm.ntiles;

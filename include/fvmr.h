@@ -149,7 +149,8 @@ typedef struct fvmr_params {
     int    renumber;         /* device cell order: FVMR_RENUMBER_* */
     int    compat_inv2x2;    /* 1: reproduce the reference inv_2x2 determinant bug (LSQ) */
     int    kernels;          /* FVMR_KERNELS_*: device kernel family */
-    int    tile_cells;       /* cells per tile of the tile kernels (0: default 300, at most 400) */
+    int    tile_cells;       /* cells per tile of the tile kernels (0: default, 150 in FP64 and 300 in
+                                mixed precision; at most 400, capped by the kernel's LDS shape) */
     double inf_ruvp[4];      /* boundaryCondition.2D.inf    (rho,u,v,p) */
     double inlet_ruvp[4];    /* boundaryCondition.2D.inlet  */
     double outlet_ruvp[4];   /* boundaryCondition.2D.outlet */
