@@ -111,6 +111,104 @@ def traffic_from_profiles():
     return None
 
 
+def var_errors(U, Uref):
+    """Relative L2 and Linf errors of rho, rho u, rho v, rho E of U against Uref."""
+    names = ("rho", "rhou", "rhov", "rhoE")
+    out = {}
+    for k, n in enumerate(names):
+        d = U[k] - Uref[k]
+        out[n] = {"l2": float(np.linalg.norm(d) / np.linalg.norm(Uref[k])),
+                  "linf": float(np.max(np.abs(d)) / np.max(np.abs(Uref[k])))}
+    return out
+
+
+def vortex_analytic(m, t, gamma=1.4, chi=5.0, L=10.0):
+    """Isentropic vortex of FieldInitializer::setInitialIsentropicVortex (FieldInitializer.cpp:
+    139-177, the reference's pi and T_inf = p_inf) advected by (1, 1) for time t on the
+    periodic [0, L]^2, nearest periodic image; conserved variables at the centroids."""
+    pi = 3.1415926535897
+    a = m.arrays()
+    x, y = a["element_xy"]
+    xc = (L / 2 + t) % L
+    dx = (x - xc + L / 2) % L - L / 2
+    dy = (y - xc + L / 2) % L - L / 2
+    r2 = dx * dx + dy * dy
+    cd = np.exp(0.5 * (1.0 - r2))
+    u = 1.0 + chi / (2 * pi) * cd * (-dy)
+    v = 1.0 + chi / (2 * pi) * cd * dx
+    T = 1.0 - (gamma - 1) * chi * chi / (8 * gamma * pi * pi) * np.exp(1.0 - r2)
+    rho = T ** (1.0 / (gamma - 1))
+    p = rho * T
+    return np.stack([rho, rho * u, rho * v, p / (gamma - 1) + 0.5 * rho * (u * u + v * v)])
+
+
+def config_c1(with_oracle):
+    """Periodic vortex 64x64x2 = 8192 cells, Roe, least squares + Barth, SSP-RK3, CFL 0.5, 100
+    steps: FP64 field against the oracle and the analytic vortex; FP32-mixed against FP64."""
+    import cfd_solver_2d_gpu_amd as F
+    m = F.Mesh.generate(F.GEN_PERIODIC_SQUARE, 64, 64, a=0.0, b=10.0)
+    res = {"cells": m.num_element}
+    outs = {}
+    for prec in (F.PREC_FP64, F.PREC_MIXED):
+        p = F.default_params(inf_ruvp=[1, 1, 1, 1], CFL=0.5, gradient=F.GRA_LEASTSQUARE, limiter=F.LIM_BARTH,
+                             flux_scheme=F.SOL_ROE, time_advance=F.EVO_RK3, precision=prec)
+        U0 = F.field_init(m, p, F.INIT_ISENTROPIC_VORTEX, (5.0, 5.0, 5.0))
+        g = F.GPUSolver2(m, p)
+        g.set_state(U0)
+        t, _ = g.advance(100, 0.0, 1e9)
+        outs[prec] = (t, g.get_state())
+        g.close()
+    t, U = outs[F.PREC_FP64]
+    res["t_end"] = t
+    res["fp64_vs_analytic"] = var_errors(U, vortex_analytic(m, t))
+    res["mixed_vs_fp64"] = var_errors(outs[F.PREC_MIXED][1], U)
+    if with_oracle:
+        import oracle as O
+        p = F.default_params(inf_ruvp=[1, 1, 1, 1], CFL=0.5, gradient=F.GRA_LEASTSQUARE, limiter=F.LIM_BARTH,
+                             flux_scheme=F.SOL_ROE, time_advance=F.EVO_RK3)
+        o = O.OracleSolver(m, p)
+        o.set_state(F.field_init(m, p, F.INIT_ISENTROPIC_VORTEX, (5.0, 5.0, 5.0)))
+        to = 0.0
+        for _ in range(100):
+            to, _ = o.step(to, 1e9)
+        res["fp64_vs_oracle"] = var_errors(U, o.get_state())
+        res["t_rel_diff_vs_oracle"] = abs(t / to - 1)
+    return res
+
+
+def config_c3(steps):
+    """Laminar NS cylinder: O-grid 512 x 512 (524,288 cells), r from 0.5 to 30, growth 1.02,
+    M = 0.2, Re = 100 (constant viscosity), isothermal wall + farfield, SSP-RK3, Green-Gauss and
+    least squares; FP64 and FP32-mixed throughput and mixed-vs-FP64 errors after `steps`."""
+    import cfd_solver_2d_gpu_amd as F
+    m = F.Mesh.generate(F.GEN_CYLINDER_OGRID, 512, 512, a=0.5, b=30.0, c=1.02, equation=F.EQ_NS,
+                        isothermal_wall=True)
+    inf = F.ruvp_from_mach(0.2, 0.0)
+    u_inf = float(np.hypot(inf[1], inf[2]))
+    T_inf = inf[3] / (inf[0] * 287.06)
+    res = {"cells": m.num_element, "steps": steps}
+    for name, grad in (("green_gauss", F.GRA_GREENGAUSS), ("least_squares", F.GRA_LEASTSQUARE)):
+        outs = {}
+        for prec in (F.PREC_FP64, F.PREC_MIXED):
+            p = F.default_params(equation=F.EQ_NS, viscosity_model=F.MU_CONSTANT, mu0=inf[0] * u_inf * 1.0 / 100.0,
+                                 inf_ruvp=inf, inlet_ruvp=inf, outlet_ruvp=inf, CFL=0.5, gradient=grad,
+                                 time_advance=F.EVO_RK3, precision=prec, wall_temperature=T_inf)
+            g = F.GPUSolver2(m, p)
+            g.set_state(F.field_init(m, p, F.INIT_UNIFORM))
+            g.advance(2, 0.0, 1e9)
+            g.synchronize()
+            t0 = time.perf_counter()
+            g.advance(steps, 0.0, 1e9)
+            g.synchronize()
+            wall = time.perf_counter() - t0
+            outs[prec] = (m.num_element * 3 * steps / wall, g.get_state())
+            g.close()
+        res[name] = {"fp64_value": outs[F.PREC_FP64][0], "mixed_value": outs[F.PREC_MIXED][0],
+                     "mixed_speedup": outs[F.PREC_MIXED][0] / outs[F.PREC_FP64][0],
+                     "mixed_vs_fp64": var_errors(outs[F.PREC_MIXED][1], outs[F.PREC_FP64][1])}
+    return res
+
+
 def cpu_baseline(steps):
     """The CPU oracle (single thread, FP64) on the same mesh for a bounded number of steps."""
     import cfd_solver_2d_gpu_amd as F
@@ -184,6 +282,7 @@ def main():
         err = float(np.linalg.norm(Umx[0] - U64[0]) / np.linalg.norm(U64[0]))
         out["mixed"] = {"value": nc * 3 * args.steps / wall_mx, "speedup_vs_fp64": wall / wall_mx,
                         "rel_l2_density_vs_fp64": err, "steps": args.steps,
+                        "errors_vs_fp64": var_errors(Umx, U64),
                         "kernels_ms": {k: round(v[0], 5) for k, v in kmx.items()}}
         _, wall_llf, _, _ = run_gpu(F.SOL_LLF, F.PREC_FP64, args.steps, args.warmup, profile=False)
         out["llf_fp64_value"] = nc * 3 * args.steps / wall_llf
@@ -192,6 +291,9 @@ def main():
         out["cell_kernels"] = {"value": nc * 3 * args.steps / wall_cell,
                                "rel_l2_density_vs_tile": float(np.linalg.norm(Ucell[0] - U64[0]) / np.linalg.norm(U64[0])),
                                "kernels_ms": {k: round(v[0], 5) for k, v in kcell.items()}}
+    if not args.no_extra:
+        out["c1_vortex"] = config_c1(with_oracle=not args.no_cpu)
+        out["c3_ns_cylinder"] = config_c3(2000)
     if not args.no_cpu:
         rate, cw = cpu_baseline(args.cpu_steps)
         out["cpu_baseline"] = {"value": rate, "unit": "cell-RK-stage updates/s", "cores": 1, "kind": "port",
