@@ -16,7 +16,7 @@ from . import _abi
 from ._abi import *  # noqa: F401,F403  (codes)
 from ._abi import FvmrError, GenParams, MeshSoA, Params, StepInfo, check, lib
 
-__all__ = ["Mesh", "GPUSolver2", "field_init", "default_params", "ruvp_from_mach", "FvmrError",
+__all__ = ["Mesh", "GPUSolver2", "BatchSolver", "field_init", "default_params", "ruvp_from_mach", "FvmrError",
            "Params", "StepInfo", "MeshSoA", "lib"]
 
 
@@ -324,3 +324,67 @@ class GPUSolver2:
 
     def synchronize(self):
         check(lib().fvmr_synchronize(self._h), "fvmr_synchronize")
+
+
+class BatchSolver:
+    """Many independent cases on one device handle (fvmr_create_batch): the cases share the
+    scheme settings; boundary states, CFL, dt and time are per case."""
+
+    def __init__(self, meshes, params):
+        if len(meshes) != len(params) or not meshes:
+            raise ValueError("one params per mesh")
+        self.meshes = list(meshes)   # keep the SoA storage alive
+        n = len(meshes)
+        soas = (C.POINTER(MeshSoA) * n)(*[C.pointer(m.soa) for m in meshes])
+        self._params = (Params * n)(*params)
+        self._h = C.c_void_p()
+        check(lib().fvmr_create_batch(n, soas, self._params, C.byref(self._h)), "fvmr_create_batch")
+        self.n = n
+
+    def close(self):
+        if self._h and self._h.value:
+            check(lib().fvmr_destroy(self._h), "fvmr_destroy")
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def cases(self):
+        nc = C.c_int()
+        b = np.zeros(self.n + 1, dtype=np.int32)
+        check(lib().fvmr_batch_cases(self._h, C.byref(nc), _iptr(b)), "fvmr_batch_cases")
+        return b
+
+    def set_state(self, c, U):
+        U, rows = _rows(U)
+        check(lib().fvmr_batch_set_state(self._h, c, rows), "fvmr_batch_set_state")
+
+    def get_state(self, c):
+        U = np.zeros((4, self.meshes[c].num_element))
+        U, rows = _rows(U)
+        check(lib().fvmr_batch_get_state(self._h, c, rows), "fvmr_batch_get_state")
+        return U
+
+    def advance(self, nsteps, t, T):
+        """nsteps steps of every case; t, T: per-case arrays; returns the new t."""
+        t = np.ascontiguousarray(t, dtype=np.float64).copy()
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        check(lib().fvmr_batch_advance(self._h, nsteps, _dptr(t), _dptr(T)), "fvmr_batch_advance")
+        return t
+
+    def synchronize(self):
+        check(lib().fvmr_synchronize(self._h), "fvmr_synchronize")
+
+    def profile(self, on=True):
+        check(lib().fvmr_profile_enable(self._h, 1 if on else 0), "fvmr_profile_enable")
+
+    def profile_reset(self):
+        check(lib().fvmr_profile_reset(self._h), "fvmr_profile_reset")
+
+    def profile_query(self, kernel):
+        ms, n = C.c_double(), C.c_int()
+        check(lib().fvmr_profile_query(self._h, kernel, C.byref(ms), C.byref(n)), "fvmr_profile_query")
+        return ms.value, n.value

@@ -106,6 +106,11 @@ SIGNATURES = {
     "fvmr_residual": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "fvmr_permutation": (C.c_int, [C.c_void_p, _ip]),
     "fvmr_node_field": (C.c_int, [C.c_void_p, _dp * 4, _dp, _dp]),
+    "fvmr_create_batch": (C.c_int, [C.c_int, C.POINTER(C.POINTER(MeshSoA)), C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "fvmr_batch_cases": (C.c_int, [C.c_void_p, _ip, _ip]),
+    "fvmr_batch_set_state": (C.c_int, [C.c_void_p, C.c_int, _dp * 4]),
+    "fvmr_batch_get_state": (C.c_int, [C.c_void_p, C.c_int, _dp * 4]),
+    "fvmr_batch_advance": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "fvmr_surface_force": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, C.c_double, _dp, _dp]),
     "fvmr_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "fvmr_profile_query": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip]),

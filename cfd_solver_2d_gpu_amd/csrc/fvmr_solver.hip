@@ -105,11 +105,17 @@ struct Mesh {
     const int *dtcode;     // per cell: alphaC contributions in reference edge order
 };
 
+// Values a case of a batch may set on its own: boundary states and CFL.
+struct CaseBC {
+    double inf[4], inlet[4], outlet[4];
+    double CFL, pad[3];
+};
+
 struct Phys {
-    double gamma, R, Pr, Pr_visc, mu0, sutherland_C1, CFL, wall_T, venkat_K;
+    double gamma, R, Pr, Pr_visc, mu0, sutherland_C1, wall_T, venkat_K;
     double dsr_x, dsr_y, dsr_speed;
     double g_ga1, inv_g;           // gamma/(gamma-1), 1/gamma
-    double inf[4], inlet[4], outlet[4];
+    CaseBC bc;                     // the handle's (first case's) boundary states and CFL
     int equation, scheme, reconstruct, gradient, limiter, mu_model, compat_inv;
 };
 
@@ -440,8 +446,9 @@ __device__ __forceinline__ void flux_viscous(const Cells &cells, const Phys &ph,
 // the canonical flux of a face, oriented L -> R.  bType: BC code of the face's set (-1
 // interior); (pmx, pmy): midpoint of the periodic pair face, where R is reconstructed.
 template <class C, class G, bool NS, class Cells>
-__device__ __forceinline__ void face_flux(const Cells &cells, const Phys &ph, double t_rk, int Lc, int Rc, int bType,
-                                          bool has_pair, const V4<G> eg, const V2<G> em, const V2<G> pm, C F[4]) {
+__device__ __forceinline__ void face_flux(const Cells &cells, const Phys &ph, const CaseBC &bc, double t_rk, int Lc,
+                                          int Rc, int bType, bool has_pair, const V4<G> eg, const V2<G> em,
+                                          const V2<G> pm, C F[4]) {
     const C nx = (C)eg.x, ny = (C)eg.y, len = (C)eg.z;
     const C xe = (C)em.x, ye = (C)em.y;
     const C gamma = (C)ph.gamma;
@@ -470,12 +477,12 @@ __device__ __forceinline__ void face_flux(const Cells &cells, const Phys &ph, do
     } else if (bType == FVMR_BC_WALL_ISOTHERMAL) {
         ghost_isothermal(L, R, (C)ph.wall_T, (C)ph.R, gamma);
     } else {
-        const double *ruvp = ph.inf;
-        if (bType == FVMR_BC_INLET) ruvp = ph.inlet;
-        else if (bType == FVMR_BC_OUTLET) ruvp = ph.outlet;
+        const double *ruvp = bc.inf;
+        if (bType == FVMR_BC_INLET) ruvp = bc.inlet;
+        else if (bType == FVMR_BC_OUTLET) ruvp = bc.outlet;
         else if (bType == FVMR_BC_DSR) {
             const double right = ph.dsr_x + ((double)em.y + ph.dsr_speed * t_rk) / sqrt(3.0);
-            ruvp = ((double)em.x < right) ? ph.inlet : ph.outlet;
+            ruvp = ((double)em.x < right) ? bc.inlet : bc.outlet;
         }
         ghost_farfield(L, ruvp, R, nx, ny, gamma);
     }
@@ -508,7 +515,7 @@ __device__ __forceinline__ void edge_flux(const Mesh<G> &m, const Phys &ph, cons
     V2<G> pm = em;
     if (bType != -1 && has_pair) pm = m.emid[pe];
     const GlobalCells<G> cells{m, U, grad};
-    face_flux<C, G, NS>(cells, ph, t_rk, Lc, Rc, bType, has_pair, eg, em, pm, F);
+    face_flux<C, G, NS>(cells, ph, ph.bc, t_rk, Lc, Rc, bType, has_pair, eg, em, pm, F);
 }
 
 // ---------------------------------------------------------------------------
@@ -805,7 +812,7 @@ __global__ __launch_bounds__(kGroup3) FVMR_GRAD_ATTR void k_gradient(Mesh<G> m, 
                 alpha_c += sAl[slot][2 * pr][lane];
                 if (ph.equation == FVMR_EQ_NS) alpha_c += sAl[slot][2 * pr + 1][lane];
             }
-            dtc = ph.CFL * (double)cg.z / alpha_c;
+            dtc = ph.bc.CFL * (double)cg.z / alpha_c;
         }
     }
     if (DT) {
@@ -925,6 +932,7 @@ struct TileArgs {
     const V4<G> *gcoef;       // per gradient cell: Green-Gauss weights of the 3 slots, volume
     const int4 *face;         // per face: nodes | BC code, L | R | pair face, float length, edge
     const DtRec *dtrec;       // per device cell
+    const CaseBC *cbc;        // per case of the batch (one case: the handle's)
 };
 
 __device__ __forceinline__ int lfield(int w, int k) { return (w >> (fvmr::kLocalBits * k)) & fvmr::kLocalMask; }
@@ -1009,6 +1017,7 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
     const int nN = ti[fvmr::kTiNN], nF = ti[fvmr::kTiNF], nFF = ti[fvmr::kTiNFF];
     const int offH = ti[fvmr::kTiOffH], offN = ti[fvmr::kTiOffN], offF = ti[fvmr::kTiOffF];
     const int offL = ti[fvmr::kTiOffL], offG = ti[fvmr::kTiOffG];
+    const int cs = ti[fvmr::kTiCase];   // case of the batch; sc[cs] holds its dt and times
     // thread i owns local cell i (phases 2, 3, 5) and face i (phase 4); its records are
     // fetched with the staging loads
     const int i = tid;
@@ -1099,7 +1108,8 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
     }
     __syncthreads();
     // 4. one canonical flux per face of the tile (nFF <= threads), written over the gradients
-    const double t_rk = sc->t_rk;
+    const double t_rk = sc[cs].t_rk;
+    const CaseBC &bc = ta.cbc[cs];
     const TileCells<G> cells{sU, sC, sGr};
     C F[4] = {C(0), C(0), C(0), C(0)};
     const int f = tid;
@@ -1112,13 +1122,13 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
         const int bType = tile_btype(fr);
         const bool has_pair = pl != fvmr::kLocalNone;
         const V2<G> pm = (bType != -1 && has_pair) ? tile_face_mid<G>(sN, sFace[pl]) : em;
-        face_flux<C, G, NS>(cells, ph, t_rk, L, R == fvmr::kLocalNone ? -1 : R, bType, has_pair, eg, em, pm, F);
+        face_flux<C, G, NS>(cells, ph, bc, t_rk, L, R == fvmr::kLocalNone ? -1 : R, bType, has_pair, eg, em, pm, F);
     }
     __syncthreads();
     if (f < nFF) sFl[f] = V4<C>{F[0], F[1], F[2], F[3]};
     __syncthreads();
     // 5. cell-owned sum in slot order, /(-V), RK update (as k_stage)
-    const double dt = sc->dt;
+    const double dt = sc[cs].dt;
     if (i < nT) {
         const int c = c0 + i;
         const int sw[3] = {gs.x, gs.y, gs.z};
@@ -1235,7 +1245,7 @@ __global__ __launch_bounds__(TileCfg<G>::kThreads) void k_tile_dt(TileArgs<G> ta
             alpha_c += av;
             if (ph.equation == FVMR_EQ_NS) alpha_c += vv;
         }
-        dtc = fmin(dtc, ph.CFL * vol_c / alpha_c);
+        dtc = fmin(dtc, ta.cbc[ti[fvmr::kTiCase]].CFL * vol_c / alpha_c);
     }
     const double wm = wave_min(dtc);
     if ((tid & 63) == 0) sMin[tid >> 6] = wm;
@@ -1358,6 +1368,31 @@ __global__ __launch_bounds__(1024) void k_dt_finish(int nparts, const double *__
             sc->n_restricted = 0;
         }
         sc->dt = d;
+    }
+}
+
+// k_dt_finish per case of a batch: block c takes the minimum over the partials of its tiles
+// tile_begin[c] .. tile_begin[c+1]-1 and clamps / advances the times of sc[c].
+__global__ __launch_bounds__(256) void k_dt_finish_cases(const int *__restrict__ tile_begin,
+                                                         const double *__restrict__ partial,
+                                                         Scalars *__restrict__ sc, int advance) {
+    __shared__ double s[4];
+    const int c = blockIdx.x;
+    double b = 1e10;
+    for (int i = tile_begin[c] + (int)threadIdx.x; i < tile_begin[c + 1]; i += 256) b = fmin(b, partial[i]);
+    b = wave_min(b);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double d = fmin(fmin(s[0], s[1]), fmin(s[2], s[3]));
+        if (advance) {
+            const double t = sc[c].t;
+            if (t + d > sc[c].T) d = sc[c].T - t;
+            sc[c].t_rk = t;
+            sc[c].t = t + d;
+            if (c == 0) sc[0].n_restricted = 0;
+        }
+        sc[c].dt = d;
     }
 }
 
@@ -1488,6 +1523,12 @@ struct fvmr_solver {
     double *partial = nullptr, *norm_out = nullptr;
     int *bad_list = nullptr;
     Scalars *sc = nullptr;
+    // cases of a batch (one for fvmr_create): cells case_cell_begin[c] .. [c+1]-1 in the caller's
+    // numbering; per case a Scalars entry (sc[c]) and boundary states / CFL (cbc[c])
+    int ncases = 1;
+    std::vector<int> case_cell_begin;
+    CaseBC *cbc = nullptr;
+    int *tile_case_begin = nullptr;       // tiles of case c: [c] .. [c+1]-1
     // tile kernels (fvmr_tiles.h)
     bool tiles = false;
     int ntiles = 0;
@@ -1529,6 +1570,7 @@ TileArgs<G> tile_args(const fvmr_solver *h) {
     TileArgs<G> t;
     t.info = h->t_info; t.halo = h->t_halo; t.nxy = h->t_nxy; t.lnodes = h->t_lnodes;
     t.gslot = h->t_gslot; t.gcoef = (const V4<G> *)h->t_gcoef; t.face = h->t_face; t.dtrec = h->t_dtrec;
+    t.cbc = h->cbc;
     return t;
 }
 
@@ -1620,7 +1662,7 @@ int launch_tile_dt(fvmr_solver *h, const S4 *U, int dt_mode) {
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 1, e0))) return rc;
     if ((rc = prof_begin(h, 2, &e0))) return rc;
-    k_dt_finish<<<1, 1024, 0, h->stream>>>(h->ntiles, h->partial, h->sc, dt_mode);
+    k_dt_finish_cases<<<h->ncases, 256, 0, h->stream>>>(h->tile_case_begin, h->partial, h->sc, dt_mode);
     HIP_OK(hipGetLastError());
     return prof_end(h, 2, e0);
 }
@@ -1718,7 +1760,7 @@ void free_all(fvmr_solver *h) {
     void *ptrs[] = {h->grad, h->rhs, h->fix, h->cgeo, h->nxy, h->egeo, h->emid, h->cslot, h->cnode, h->dtcode,
                     h->eint, h->partial, h->norm_out, h->bad_list, h->sc, h->t_info, h->t_halo, h->t_lnodes,
                     h->t_nxy, h->t_gslot, h->t_face, h->t_gcoef, h->t_dtrec,
-                    h->nc_off, h->nc_cells, h->node_buf};
+                    h->nc_off, h->nc_cells, h->node_buf, h->cbc, h->tile_case_begin};
     for (void *q : ptrs) if (q) (void)hipFree(q);
     for (int k = 0; k < kNumProf; k++)
         for (auto e : h->pending[k]) (void)hipEventDestroy(e);
@@ -1774,7 +1816,13 @@ int validate_mesh(const fvmr_mesh_soa *m) {
 
 extern "C" {
 
-int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out) {
+}  // extern "C"
+
+// fvmr_create for one case or a batch: m is the caller's SoA (for a batch, the cases' SoAs
+// one after the other), case_begin[c] .. case_begin[c+1]-1 the cells of case c, case_params[c]
+// its parameters (NULL: p for every case).
+static int create_impl(const fvmr_mesh_soa *m, const fvmr_params *p, int ncases, const int *case_begin,
+                       const fvmr_params *case_params, fvmr_handle **out) {
     if (!m || !p || !out) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     int rc = validate_params(p);
@@ -1795,32 +1843,40 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
     Phys &ph = h->ph;
     ph.gamma = p->gamma; ph.R = p->R; ph.Pr = p->Pr; ph.Pr_visc = p->Pr_viscous; ph.mu0 = p->mu0;
     ph.sutherland_C1 = p->mu0 / std::pow(p->T0, 1.5) * (p->T0 + 110.4);  // ViscousFluxGPU.h:26-28
-    ph.CFL = p->CFL; ph.wall_T = p->wall_temperature; ph.venkat_K = p->venkat_K;
+    ph.bc.CFL = p->CFL; ph.wall_T = p->wall_temperature; ph.venkat_K = p->venkat_K;
     ph.g_ga1 = p->gamma / (p->gamma - 1.0); ph.inv_g = 1.0 / p->gamma;
     ph.dsr_x = p->dsr_shock_x; ph.dsr_y = p->dsr_shock_y; ph.dsr_speed = p->dsr_shock_speed;
-    for (int k = 0; k < 4; k++) { ph.inf[k] = p->inf_ruvp[k]; ph.inlet[k] = p->inlet_ruvp[k]; ph.outlet[k] = p->outlet_ruvp[k]; }
+    for (int k = 0; k < 4; k++) { ph.bc.inf[k] = p->inf_ruvp[k]; ph.bc.inlet[k] = p->inlet_ruvp[k]; ph.bc.outlet[k] = p->outlet_ruvp[k]; }
+    h->ncases = ncases;
+    h->case_cell_begin.assign(case_begin, case_begin + ncases + 1);
     ph.equation = p->equation; ph.scheme = p->flux_scheme;
     ph.reconstruct = (p->reconstruct == FVMR_REC_LINEAR) ? FVMR_REC_LINEAR : FVMR_REC_CONSTANT;
     ph.gradient = p->gradient; ph.limiter = p->limiter; ph.mu_model = p->viscosity_model; ph.compat_inv = p->compat_inv2x2;
-    // cell permutation
+    // cell permutation, within each case (case c keeps cells case_begin[c] .. [c+1]-1)
     h->perm.resize(nc);
-    if (p->renumber == FVMR_RENUMBER_HILBERT) {
-        h->perm = fvmr::hilbert_order(nc, m->element_xy[0], m->element_xy[1]);
-    } else if (p->renumber == FVMR_RENUMBER_RCM) {
-        std::vector<int> off(nc + 1, 0), adj;
-        adj.reserve(3 * (size_t)nc);
-        for (int c = 0; c < nc; c++) {
-            for (int j = 0; j < 3; j++) {
-                const int e = m->element_edges[j][c];
-                const int L = m->edge_elementL[e], R = m->edge_elementR[e];
-                const int o = (c != R) ? R : L;
-                if (o >= 0 && o != c) adj.push_back(o);
+    for (int cs = 0; cs < ncases; cs++) {
+        const int b = case_begin[cs], n = case_begin[cs + 1] - b;
+        std::vector<int> local;
+        if (p->renumber == FVMR_RENUMBER_HILBERT) {
+            local = fvmr::hilbert_order(n, m->element_xy[0] + b, m->element_xy[1] + b);
+        } else if (p->renumber == FVMR_RENUMBER_RCM) {
+            std::vector<int> off(n + 1, 0), adj;
+            adj.reserve(3 * (size_t)n);
+            for (int c = 0; c < n; c++) {
+                for (int j = 0; j < 3; j++) {
+                    const int e = m->element_edges[j][b + c];
+                    const int L = m->edge_elementL[e], R = m->edge_elementR[e];
+                    const int o = (b + c != R) ? R : L;
+                    if (o >= 0 && o != b + c) adj.push_back(o - b);
+                }
+                off[c + 1] = (int)adj.size();
             }
-            off[c + 1] = (int)adj.size();
+            local = rcm_order(n, off, adj);
+        } else {
+            local.resize(n);
+            std::iota(local.begin(), local.end(), 0);
         }
-        h->perm = rcm_order(nc, off, adj);
-    } else {
-        std::iota(h->perm.begin(), h->perm.end(), 0);
+        for (int i = 0; i < n; i++) h->perm[b + i] = b + local[i];
     }
     h->iperm.assign(nc, -1);
     for (int i = 0; i < nc; i++) h->iperm[h->perm[i]] = i;
@@ -1925,7 +1981,7 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
             for (int e = 0; e < nf; e++) { emx[e] = m->edge_xy[0][eperm[e]]; emy[e] = m->edge_xy[1][eperm[e]]; }
             fvmr::TileInput ti{nc, nf, sx.data(), sy.data(), cnode.data(), eL.data(), eR.data(), et.data(), ep.data(),
                                en0.data(), en1.data(), elen.data(), m->node_xy[0], m->node_xy[1], vol.data(), dtcode.data(),
-                               ccx.data(), ccy.data(), emx.data(), emy.data()};
+                               ccx.data(), ccy.data(), emx.data(), emy.data(), ncases, case_begin};
             auto caps_of = [&](auto cfg) {
                 using Cfg = decltype(cfg);
                 return fvmr::TileCaps{std::min(p->tile_cells ? p->tile_cells : Cfg::kCells, Cfg::kMaxG), Cfg::kMaxG,
@@ -1966,9 +2022,24 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
             for (int i = 0; i < nc; i++)
                 dtr[i] = DtRec{dtcode[i], tm.dtface[i], h->mixed ? (double)(float)vol[i] : vol[i]};
             if ((rc = upload(&h->t_dtrec, dtr, h->stream))) return bail(rc);
+            if ((rc = upload(&h->tile_case_begin, tm.case_tile_begin, h->stream))) return bail(rc);
             // the host vectors of these uploads end with this block
             if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(set_error(FVMR_ERR_HIP, "tile upload failed"));
         }
+    }
+    if (ncases > 1 && !h->tiles)
+        return bail(set_error(FVMR_ERR_UNSUPPORTED, "a batch needs the tile kernels (standard geometry, kernels != CELL)"));
+    {
+        std::vector<CaseBC> cb(ncases);
+        for (int cs = 0; cs < ncases; cs++) {
+            const fvmr_params *q = case_params ? &case_params[cs] : p;
+            CaseBC &b = cb[cs];
+            for (int k = 0; k < 4; k++) { b.inf[k] = q->inf_ruvp[k]; b.inlet[k] = q->inlet_ruvp[k]; b.outlet[k] = q->outlet_ruvp[k]; }
+            b.CFL = q->CFL;
+            b.pad[0] = b.pad[1] = b.pad[2] = 0.0;
+        }
+        if ((rc = upload(&h->cbc, cb, h->stream))) return bail(rc);
+        if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(set_error(FVMR_ERR_HIP, "case upload failed"));
     }
     // device memory
     const size_t gsz = h->mixed ? sizeof(Grad<float>) : sizeof(Grad<double>);
@@ -1981,19 +2052,107 @@ int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out)
         hipMalloc((void **)&h->bad_list, (size_t)nc * sizeof(int)) != hipSuccess ||
         hipMalloc((void **)&h->partial, (size_t)nparts * sizeof(double)) != hipSuccess ||
         hipMalloc((void **)&h->norm_out, 4 * sizeof(double)) != hipSuccess ||
-        hipMalloc((void **)&h->sc, sizeof(Scalars)) != hipSuccess)
+        hipMalloc((void **)&h->sc, ncases * sizeof(Scalars)) != hipSuccess)
         return bail(set_error(FVMR_ERR_HIP, "hipMalloc failed"));
     if ((rc = upload(&h->cslot, cslot, h->stream)) || (rc = upload(&h->cnode, cnode, h->stream)) ||
         (rc = upload(&h->dtcode, dtcode, h->stream)) || (rc = upload(&h->eint, eint, h->stream)))
         return bail(rc);
     rc = h->mixed ? build_geometry<float>(h, m, eperm) : build_geometry<double>(h, m, eperm);
     if (rc) return bail(rc);
-    Scalars s0{};
+    std::vector<Scalars> s0(ncases);
     if (hipMemsetAsync(h->Ubuf[0], 0, (size_t)nc * sizeof(S4), h->stream) != hipSuccess ||
-        hipMemcpyAsync(h->sc, &s0, sizeof(Scalars), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(h->sc, s0.data(), ncases * sizeof(Scalars), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
         return bail(set_error(FVMR_ERR_HIP, "initial copy failed"));
     *out = h;
+    return FVMR_OK;
+}
+
+extern "C" {
+
+int fvmr_create(const fvmr_mesh_soa *m, const fvmr_params *p, fvmr_handle **out) {
+    if (!m) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
+    const int cb[2] = {0, m->num_element};
+    return create_impl(m, p, 1, cb, nullptr, out);
+}
+
+// Settings every case of a batch shares (all of fvmr_params but the boundary states and CFL).
+static bool same_shared_params(const fvmr_params &a, const fvmr_params &b) {
+    return a.gamma == b.gamma && a.R == b.R && a.Pr == b.Pr && a.Pr_viscous == b.Pr_viscous && a.T0 == b.T0 &&
+           a.mu0 == b.mu0 && a.equation == b.equation && a.flux_scheme == b.flux_scheme &&
+           a.reconstruct == b.reconstruct && a.gradient == b.gradient && a.limiter == b.limiter &&
+           a.time_advance == b.time_advance && a.viscosity_model == b.viscosity_model && a.precision == b.precision &&
+           a.renumber == b.renumber && a.compat_inv2x2 == b.compat_inv2x2 && a.kernels == b.kernels &&
+           a.tile_cells == b.tile_cells && a.wall_temperature == b.wall_temperature && a.venkat_K == b.venkat_K &&
+           a.dsr_shock_x == b.dsr_shock_x && a.dsr_shock_y == b.dsr_shock_y && a.dsr_shock_speed == b.dsr_shock_speed;
+}
+
+int fvmr_create_batch(int ncases, const fvmr_mesh_soa *const *meshes, const fvmr_params *params, fvmr_handle **out) {
+    if (ncases < 1 || !meshes || !params || !out) return set_error(FVMR_ERR_INVALID_ARG, "invalid batch arguments");
+    *out = nullptr;
+    for (int c = 0; c < ncases; c++) {
+        if (!meshes[c]) return set_error(FVMR_ERR_INVALID_ARG, "null mesh in batch");
+        if (!same_shared_params(params[0], params[c]))
+            return set_error(FVMR_ERR_INVALID_ARG, "batch cases differ in a shared setting (only the boundary states and CFL may differ)");
+        int rc = validate_mesh(meshes[c]);
+        if (rc) return rc;
+    }
+    // the cases' SoAs one after the other, indices offset
+    int64_t tn = 0, tc = 0, tf = 0, tb = 0;
+    for (int c = 0; c < ncases; c++) {
+        tn += meshes[c]->num_node; tc += meshes[c]->num_element; tf += meshes[c]->num_edge; tb += meshes[c]->num_boundary_set;
+    }
+    if (tn > INT32_MAX || tc > INT32_MAX || tf > INT32_MAX) return set_error(FVMR_ERR_UNSUPPORTED, "batch too large");
+    std::vector<double> nxy[2], exy[2], evol, elen, gxy[2], gnrm[2];
+    std::vector<int> en[4], ee[4], enb[4], gn[2], gset, gpair, gL, gR, btype;
+    std::vector<int> case_begin{0};
+    int noff = 0, coff = 0, foff = 0, soff = 0;
+    auto add = [](int v, int off) { return v < 0 ? v : v + off; };
+    for (int c = 0; c < ncases; c++) {
+        const fvmr_mesh_soa &q = *meshes[c];
+        for (int k = 0; k < 2; k++) nxy[k].insert(nxy[k].end(), q.node_xy[k], q.node_xy[k] + q.num_node);
+        for (int i = 0; i < q.num_element; i++) {
+            for (int j = 0; j < 4; j++) {
+                en[j].push_back(add(q.element_nodes[j][i], noff));
+                ee[j].push_back(add(q.element_edges[j][i], foff));
+                enb[j].push_back(add(q.element_neighbors[j][i], coff));
+            }
+            for (int k = 0; k < 2; k++) exy[k].push_back(q.element_xy[k][i]);
+            evol.push_back(q.element_volume[i]);
+        }
+        for (int e = 0; e < q.num_edge; e++) {
+            for (int k = 0; k < 2; k++) {
+                gn[k].push_back(q.edge_nodes[k][e] + noff);
+                gxy[k].push_back(q.edge_xy[k][e]);
+                gnrm[k].push_back(q.edge_normal[k][e]);
+            }
+            gset.push_back(add(q.edge_setID[e], soff));
+            gpair.push_back(add(q.edge_periodicPair[e], foff));
+            gL.push_back(q.edge_elementL[e] + coff);
+            gR.push_back(add(q.edge_elementR[e], coff));
+            elen.push_back(q.edge_length[e]);
+        }
+        btype.insert(btype.end(), q.boundary_type, q.boundary_type + q.num_boundary_set);
+        noff += q.num_node; coff += q.num_element; foff += q.num_edge; soff += q.num_boundary_set;
+        case_begin.push_back(coff);
+    }
+    fvmr_mesh_soa m{};
+    m.num_node = noff; m.num_element = coff; m.num_edge = foff; m.num_boundary_set = soff;
+    for (int k = 0; k < 2; k++) {
+        m.node_xy[k] = nxy[k].data(); m.element_xy[k] = exy[k].data(); m.edge_nodes[k] = gn[k].data();
+        m.edge_xy[k] = gxy[k].data(); m.edge_normal[k] = gnrm[k].data();
+    }
+    for (int j = 0; j < 4; j++) { m.element_nodes[j] = en[j].data(); m.element_edges[j] = ee[j].data(); m.element_neighbors[j] = enb[j].data(); }
+    m.element_volume = evol.data(); m.edge_length = elen.data(); m.edge_setID = gset.data();
+    m.edge_periodicPair = gpair.data(); m.edge_elementL = gL.data(); m.edge_elementR = gR.data();
+    m.boundary_type = btype.data();
+    return create_impl(&m, &params[0], ncases, case_begin.data(), params, out);
+}
+
+int fvmr_batch_cases(fvmr_handle *h, int *ncases, int *case_cell_begin) {
+    if (!h || !ncases) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
+    *ncases = h->ncases;
+    if (case_cell_begin) std::copy(h->case_cell_begin.begin(), h->case_cell_begin.end(), case_cell_begin);
     return FVMR_OK;
 }
 
@@ -2071,6 +2230,57 @@ int fvmr_advance(fvmr_handle *h, int nsteps, double *t, double T, fvmr_step_info
 }
 
 int fvmr_step(fvmr_handle *h, double *t, double T, fvmr_step_info *info) { return fvmr_advance(h, 1, t, T, info); }
+
+// The device rows of case c are c's own cells: [case_cell_begin[c], case_cell_begin[c+1]).
+int fvmr_batch_set_state(fvmr_handle *h, int c, const double *const U[4]) {
+    if (!h || !U || c < 0 || c >= h->ncases) return set_error(FVMR_ERR_INVALID_ARG, "invalid case or argument");
+    for (int k = 0; k < 4; k++)
+        if (!U[k]) return set_error(FVMR_ERR_INVALID_ARG, "null state component");
+    const int b = h->case_cell_begin[c], e = h->case_cell_begin[c + 1];
+    std::vector<S4> buf(std::max(1, e - b));
+    for (int i = b; i < e; i++) {
+        const int r = h->perm[i] - b;
+        buf[i - b] = S4{U[0][r], U[1][r], U[2][r], U[3][r]};
+    }
+    if (e > b) HIP_OK(hipMemcpyAsync(h->Ubuf[h->cur] + b, buf.data(), (size_t)(e - b) * sizeof(S4), hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    h->rhs_valid = false;
+    return FVMR_OK;
+}
+
+int fvmr_batch_get_state(fvmr_handle *h, int c, double *const U[4]) {
+    if (!h || !U || c < 0 || c >= h->ncases) return set_error(FVMR_ERR_INVALID_ARG, "invalid case or argument");
+    for (int k = 0; k < 4; k++)
+        if (!U[k]) return set_error(FVMR_ERR_INVALID_ARG, "null output component");
+    const int b = h->case_cell_begin[c], e = h->case_cell_begin[c + 1];
+    std::vector<S4> buf(std::max(1, e - b));
+    if (e > b) HIP_OK(hipMemcpyAsync(buf.data(), h->Ubuf[h->cur] + b, (size_t)(e - b) * sizeof(S4), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    for (int i = b; i < e; i++) {
+        const int r = h->perm[i] - b;
+        U[0][r] = buf[i - b].x; U[1][r] = buf[i - b].y; U[2][r] = buf[i - b].z; U[3][r] = buf[i - b].w;
+    }
+    return FVMR_OK;
+}
+
+int fvmr_batch_advance(fvmr_handle *h, int nsteps, double *t, const double *T) {
+    if (!h || !t || !T || nsteps < 0) return set_error(FVMR_ERR_INVALID_ARG, "invalid argument");
+    if (!h->tiles) return set_error(FVMR_ERR_UNSUPPORTED, "fvmr_batch_advance needs the tile kernels");
+    std::vector<Scalars> s(h->ncases);
+    HIP_OK(hipMemcpyAsync(s.data(), h->sc, s.size() * sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    for (int c = 0; c < h->ncases; c++) { s[c].t = t[c]; s[c].T = T[c]; }
+    s[0].counter = 0; s[0].bad_count = 0; s[0].n_restricted = 0;
+    HIP_OK(hipMemcpyAsync(h->sc, s.data(), s.size() * sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
+    int rc;
+    for (int i = 0; i < nsteps; i++)
+        if ((rc = run_step(h, false))) return rc;
+    HIP_OK(hipMemcpyAsync(s.data(), h->sc, s.size() * sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    if (h->profiling && (rc = prof_collect(h))) return rc;
+    for (int c = 0; c < h->ncases; c++) t[c] = s[c].t;
+    return FVMR_OK;
+}
 
 int fvmr_stage_rhs(fvmr_handle *h, double *const rhs[4]) {
     if (!h || !rhs) return set_error(FVMR_ERR_INVALID_ARG, "null argument");

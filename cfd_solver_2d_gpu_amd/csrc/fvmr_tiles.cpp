@@ -186,8 +186,15 @@ int build_tiles(const TileInput &in, const TileCaps &caps, TileMesh &out) {
         return (int)nodes.size() <= caps.nodes;
     };
 
+    out.case_tile_begin.assign(1, 0);
+    int cs = 0;
     for (int c0 = 0; c0 < nc;) {
-        int s = std::min(caps.cells, nc - c0), nG = 0, nFF = 0;
+        while (cs + 1 < in.ncases && c0 >= in.case_begin[cs + 1]) {
+            cs++;
+            out.case_tile_begin.push_back(out.ntiles);
+        }
+        const int case_end = (cs + 1 < in.ncases) ? in.case_begin[cs + 1] : nc;
+        int s = std::min(caps.cells, case_end - c0), nG = 0, nFF = 0;
         while (!try_tile(c0, s, nG, nFF)) {
             if (s == 1) return set_error(FVMR_ERR_UNSUPPORTED, "one cell and its halo exceed the tile capacity");
             s = std::max(1, s - std::max(1, s / 8));
@@ -195,7 +202,7 @@ int build_tiles(const TileInput &in, const TileCaps &caps, TileMesh &out) {
         const int nL = (int)cells.size(), nF = (int)faces.size(), nN = (int)nodes.size();
         const int info[kTileInfo] = {c0, s, nG, nL, nN, nF, nFF,
                                      (int)out.halo.size(), (int)(out.nxy.size() / 2), (int)(out.face.size() / 4),
-                                     (int)out.lnodes.size(), (int)(out.gcoef.size() / 4)};
+                                     (int)out.lnodes.size(), (int)(out.gcoef.size() / 4), cs};
         out.info.insert(out.info.end(), info, info + kTileInfo);
         for (int i = s; i < nL; i++) out.halo.push_back(cells[i]);
         for (int v : nodes) { out.nxy.push_back(in.node_x[v]); out.nxy.push_back(in.node_y[v]); }
@@ -247,6 +254,7 @@ int build_tiles(const TileInput &in, const TileCaps &caps, TileMesh &out) {
         out.ntiles++;
         c0 += s;
     }
+    while ((int)out.case_tile_begin.size() <= in.ncases) out.case_tile_begin.push_back(out.ntiles);
     return FVMR_OK;
 }
 

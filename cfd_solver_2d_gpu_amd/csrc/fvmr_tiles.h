@@ -16,8 +16,9 @@ constexpr int kLocalNone = kLocalMask;
 // Per-tile header, kTileInfo ints:
 //   c0, nT (tile cells c0 .. c0+nT-1), nG (cells with a gradient: tile + first halo ring),
 //   nL (cells with a state: + second ring), nN (nodes), nF (faces), nFF (faces of tile cells,
-//   listed first), off_halo, off_node, off_face, off_lcell, off_gcell
-enum TileInfoField { kTiC0, kTiNT, kTiNG, kTiNL, kTiNN, kTiNF, kTiNFF, kTiOffH, kTiOffN, kTiOffF, kTiOffL, kTiOffG, kTileInfo };
+//   listed first), off_halo, off_node, off_face, off_lcell, off_gcell, case of a batch
+enum TileInfoField { kTiC0, kTiNT, kTiNG, kTiNL, kTiNN, kTiNF, kTiNFF, kTiOffH, kTiOffN, kTiOffF, kTiOffL, kTiOffG,
+                     kTiCase, kTileInfo };
 
 // Gradient-cell slot word: local face (10 bits) | local cell across (10) | cell is the
 // face's R (bit 20) | periodic face (bit 21).
@@ -49,6 +50,7 @@ struct TileMesh {
     std::vector<double> gcoef;    // per gradient cell: Green-Gauss weight of slots 0-2, volume
     std::vector<int> face;        // 4 ints per face
     std::vector<int> dtface;      // per device cell: its 3 slot faces, local ids in its tile
+    std::vector<int> case_tile_begin;  // tiles of case c: [c] .. [c+1]-1
     int max_nL = 0, max_nG = 0, max_nF = 0, max_nFF = 0, max_nN = 0;
     int64_t sum_nL = 0, sum_nG = 0, sum_nF = 0, sum_nN = 0;
 };
@@ -67,6 +69,8 @@ struct TileInput {
     const int *dtcode;    // per cell
     const double *cx, *cy;  // cell centroids
     const double *mx, *my;  // edge midpoints
+    int ncases;             // cases of a batch: device cells case_begin[c] .. [c+1]-1 (tiles never
+    const int *case_begin;  // cross a case boundary)
 };
 
 // Cells sorted along a Hilbert curve over their centroids (ties by index).

@@ -268,6 +268,24 @@ int fvmr_residual(fvmr_handle *h, int norm, double out[4]);
 int fvmr_permutation(fvmr_handle *h, int *perm);
 
 /* ---------------------------------------------------------------------- */
+/* batches: many independent cases advanced together (no reference         */
+/* counterpart: the reference runs one case per process)                    */
+/* ---------------------------------------------------------------------- */
+/* One handle over ncases meshes, packed one after the other (case c's cells keep their order
+ * within the case; the device renumbers inside each case and never lets a tile cross a case).
+ * The cases must share every setting of fvmr_params except the boundary states
+ * (inf/inlet/outlet_ruvp) and CFL, which are per case; each case has its own dt and time.
+ * Needs the tile kernels.  fvmr_destroy releases it. */
+int fvmr_create_batch(int ncases, const fvmr_mesh_soa *const *meshes, const fvmr_params *params, fvmr_handle **out);
+/* Number of cases and their cell ranges in the packed numbering (ncases + 1 entries). */
+int fvmr_batch_cases(fvmr_handle *h, int *ncases, int *case_cell_begin);
+/* State of case c in its own element order. */
+int fvmr_batch_set_state(fvmr_handle *h, int c, const double *const U[4]);
+int fvmr_batch_get_state(fvmr_handle *h, int c, double *const U[4]);
+/* nsteps steps of every case: t[c] in/out, each case's dt clamped to T[c] - t[c]. */
+int fvmr_batch_advance(fvmr_handle *h, int nsteps, double *t, const double *T);
+
+/* ---------------------------------------------------------------------- */
 /* output quantities (FieldWriter, FVMR/src/output/FieldWriter.cpp)         */
 /* ---------------------------------------------------------------------- */
 /* Node fields of the current state, formed on the device: node ruvp = mean of the primitives

@@ -1,0 +1,91 @@
+"""Batches: independent cases of uneven size on one handle (fvmr_create_batch) advance exactly
+as the same cases on their own handles, with per-case boundary states, CFL, dt and time."""
+import numpy as np
+import pytest
+
+import cfd_solver_2d_gpu_amd as F
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def cases():
+    """Channels and cylinder O-grids of uneven size, Mach 0.3-0.8, AoA -5..5 degrees, plus a
+    periodic vortex: one shared scheme, per-case freestream and CFL."""
+    rng = np.random.default_rng(11)
+    out = []
+    for k in range(6):
+        Ma, aoa = rng.uniform(0.3, 0.8), rng.uniform(-5, 5)
+        inf = F.ruvp_from_mach(Ma, aoa)
+        if k % 2 == 0:
+            ni = int(rng.integers(16, 64))
+            m = F.Mesh.generate(F.GEN_RECT_CHANNEL, ni, ni // 2, a=2.0, b=1.0, jitter=0.1, seed=k)
+        else:
+            m = F.Mesh.generate(F.GEN_CYLINDER_OGRID, int(rng.integers(24, 64)), 16, a=0.5, b=8.0, c=1.1)
+        p = F.default_params(inf_ruvp=inf, inlet_ruvp=inf, outlet_ruvp=inf, CFL=float(rng.uniform(0.3, 0.6)),
+                             time_advance=F.EVO_RK3)
+        out.append((m, p, F.field_init(m, p, F.INIT_UNIFORM)))
+    m = F.Mesh.generate(F.GEN_PERIODIC_SQUARE, 40, 40, a=0.0, b=10.0)
+    p = F.default_params(inf_ruvp=[1, 1, 1, 1], CFL=0.5, time_advance=F.EVO_RK3)
+    out.append((m, p, F.field_init(m, p, F.INIT_ISENTROPIC_VORTEX, (5.0, 5.0, 5.0))))
+    # the uniform freestream of a channel stays uniform: perturb it so that the flow moves
+    for m, p, U in out[:-1]:
+        U *= (1 + 0.02 * np.sin(0.37 * np.arange(U.shape[1])))[None, :]
+    return out
+
+
+@pytest.mark.parametrize("precision", [F.PREC_FP64, F.PREC_MIXED])
+def test_batch_equals_separate_solvers(precision):
+    announce(f"batch of 7 uneven cases vs separate handles, precision={precision}")
+    cs = cases()
+    for _, p, _ in cs:
+        p.precision = precision
+    b = F.BatchSolver([m for m, _, _ in cs], [p for _, p, _ in cs])
+    begin = b.cases()
+    assert begin[-1] == sum(m.num_element for m, _, _ in cs)
+    for c, (_, _, U) in enumerate(cs):
+        b.set_state(c, U)
+    T = np.full(len(cs), 1e9)
+    T[2] = 1e-6                                   # one case reaches its end time and stops
+    t = b.advance(4, np.zeros(len(cs)), T)
+    for c, (m, p, U) in enumerate(cs):
+        g = F.GPUSolver2(m, p)
+        g.set_state(U)
+        tc, _ = g.advance(4, 0.0, T[c])
+        assert tc == t[c], (c, tc, t[c])
+        assert np.array_equal(g.get_state(), b.get_state(c)), c
+        g.close()
+    assert t[2] == 1e-6
+    b.close()
+
+
+def test_batch_case_against_the_oracle():
+    announce("batch case 5 (cylinder O-grid) against the oracle, 3 RK3 steps")
+    cs = cases()
+    b = F.BatchSolver([m for m, _, _ in cs], [p for _, p, _ in cs])
+    for c, (_, _, U) in enumerate(cs):
+        b.set_state(c, U)
+    t = b.advance(3, np.zeros(len(cs)), np.full(len(cs), 1e9))
+    m, p, U = cs[5]
+    o = O.OracleSolver(m, p)
+    o.set_state(U)
+    to = 0.0
+    for _ in range(3):
+        to, _ = o.step(to, 1e9)
+    assert abs(t[5] / to - 1) < 1e-12
+    Uo = o.get_state()
+    assert np.linalg.norm(b.get_state(5) - Uo) / np.linalg.norm(Uo) < 1e-12
+    b.close()
+
+
+def test_batch_refuses_mixed_settings():
+    announce("batch with cases that differ in a shared setting is refused")
+    cs = cases()[:2]
+    cs[1][1].flux_scheme = F.SOL_LLF
+    with pytest.raises(F.FvmrError) as e:
+        F.BatchSolver([m for m, _, _ in cs], [p for _, p, _ in cs])
+    assert "shared setting" in str(e.value)
