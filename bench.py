@@ -209,6 +209,71 @@ def config_c3(steps):
     return res
 
 
+def config_c4(ncases=256, steps=500, seed=0):
+    """Batched sweep: `ncases` independent Euler cases, half jittered channels and half cylinder
+    O-grids, cell counts log-uniform in [2k, 200k], Mach U[0.3, 0.8], AoA U[-5, 5] deg,
+    FP32-mixed, SSP-RK3, on one batch handle; against the same cases on separate handles
+    advanced one after the other (timed on the first 32 cases, 50 steps)."""
+    import cfd_solver_2d_gpu_amd as F
+    rng = np.random.default_rng(seed)
+    meshes, params, states = [], [], []
+    for k in range(ncases):
+        cells = float(np.exp(rng.uniform(np.log(2e3), np.log(2e5))))
+        Ma, aoa = rng.uniform(0.3, 0.8), rng.uniform(-5.0, 5.0)
+        n = max(8, int(round(np.sqrt(cells))))
+        if k % 2 == 0:
+            m = F.Mesh.generate(F.GEN_RECT_CHANNEL, n, max(4, n // 2), a=2.0, b=1.0, jitter=0.1, seed=k)
+        else:
+            m = F.Mesh.generate(F.GEN_CYLINDER_OGRID, n, max(4, n // 2), a=0.5, b=10.0, c=1.05)
+        inf = F.ruvp_from_mach(Ma, aoa)
+        p = F.default_params(inf_ruvp=inf, inlet_ruvp=inf, outlet_ruvp=inf, CFL=0.5, time_advance=F.EVO_RK3,
+                             precision=F.PREC_MIXED)
+        meshes.append(m)
+        params.append(p)
+        states.append(F.field_init(m, p, F.INIT_UNIFORM))
+    total = sum(m.num_element for m in meshes)
+    t0 = time.perf_counter()
+    b = F.BatchSolver(meshes, params)
+    setup = time.perf_counter() - t0
+    for c, U in enumerate(states):
+        b.set_state(c, U)
+    t = b.advance(2, np.zeros(ncases), np.full(ncases, 1e9))
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.advance(steps, t, np.full(ncases, 1e9))
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    b.profile(True)
+    b.profile_reset()
+    b.advance(5, t, np.full(ncases, 1e9))
+    b.synchronize()
+    kern = {name: round(b.profile_query(i)[0], 5) for i, name in ((0, "k_tile"), (1, "k_tile_dt"),
+                                                                    (2, "k_dt_finish_cases"), (3, "k_restrict"))}
+    b.close()
+    sub, sub_steps = min(32, ncases), 50
+    sub_cells = sum(m.num_element for m in meshes[:sub])
+    handles = []
+    for c in range(sub):
+        g = F.GPUSolver2(meshes[c], params[c])
+        g.set_state(states[c])
+        g.advance(2, 0.0, 1e9)
+        handles.append(g)
+    handles[-1].synchronize()
+    t0 = time.perf_counter()
+    for g in handles:
+        g.advance(sub_steps, 0.0, 1e9)
+    for g in handles:
+        g.synchronize()
+    wall_sep = time.perf_counter() - t0
+    for g in handles:
+        g.close()
+    return {"cases": ncases, "cells": total, "steps": steps, "dtype": "f32-mixed",
+            "value": total * 3 * steps / wall, "unit": "cell-RK-stage updates/s", "ms_per_step": wall / steps * 1e3,
+            "setup_s": setup, "kernels_ms": kern,
+            "separate_handles": {"cases": sub, "cells": sub_cells, "steps": sub_steps,
+                                 "value": sub_cells * 3 * sub_steps / wall_sep}}
+
+
 def cpu_baseline(steps):
     """The CPU oracle (single thread, FP64) on the same mesh for a bounded number of steps."""
     import cfd_solver_2d_gpu_amd as F
@@ -294,6 +359,7 @@ def main():
     if not args.no_extra:
         out["c1_vortex"] = config_c1(with_oracle=not args.no_cpu)
         out["c3_ns_cylinder"] = config_c3(2000)
+        out["c4_batch"] = config_c4()
     if not args.no_cpu:
         rate, cw = cpu_baseline(args.cpu_steps)
         out["cpu_baseline"] = {"value": rate, "unit": "cell-RK-stage updates/s", "cores": 1, "kind": "port",
