@@ -16,7 +16,7 @@ from . import _abi
 from ._abi import *  # noqa: F401,F403  (codes)
 from ._abi import FvmrError, GenParams, MeshSoA, Params, StepInfo, check, lib
 
-__all__ = ["Mesh", "GPUSolver2", "BatchSolver", "field_init", "default_params", "ruvp_from_mach", "FvmrError",
+__all__ = ["Mesh", "GPUSolver2", "BatchSolver", "autotune", "field_init", "default_params", "ruvp_from_mach", "FvmrError",
            "Params", "StepInfo", "MeshSoA", "lib"]
 
 
@@ -194,6 +194,14 @@ def _rows(U):
     return U, (C.POINTER(C.c_double) * 4)(*[_dptr(U[k]) for k in range(4)])
 
 
+def autotune(mesh, params, steps=20):
+    """fvmr_autotune: (best tile_cells, ms per step) over candidate tile sizes on this mesh."""
+    best = C.c_int()
+    ms = C.c_double()
+    check(lib().fvmr_autotune(C.byref(mesh.soa), C.byref(params), steps, C.byref(best), C.byref(ms)), "fvmr_autotune")
+    return best.value, ms.value
+
+
 def field_init(mesh, params, init_type=INIT_UNIFORM, args=()):
     """FieldInitializer::setInitialAndBoundaryCondition: U (4 x num_element)."""
     U = np.zeros((4, mesh.num_element))
@@ -282,6 +290,10 @@ class GPUSolver2:
         p = np.zeros(self.n, dtype=np.int32)
         check(lib().fvmr_permutation(self._h, _iptr(p)), "fvmr_permutation")
         return p
+
+    def use_graphs(self, on=True):
+        """Replay runs of three steps from captured HIP graphs (default on)."""
+        check(lib().fvmr_use_graphs(self._h, 1 if on else 0), "fvmr_use_graphs")
 
     def node_field(self):
         """FieldWriter::update_nodeField on the device: (ruvp 4 x num_node, Cp, Ma) of the
@@ -377,6 +389,9 @@ class BatchSolver:
 
     def synchronize(self):
         check(lib().fvmr_synchronize(self._h), "fvmr_synchronize")
+
+    def use_graphs(self, on=True):
+        check(lib().fvmr_use_graphs(self._h, 1 if on else 0), "fvmr_use_graphs")
 
     def profile(self, on=True):
         check(lib().fvmr_profile_enable(self._h, 1 if on else 0), "fvmr_profile_enable")

@@ -106,6 +106,8 @@ SIGNATURES = {
     "fvmr_residual": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "fvmr_permutation": (C.c_int, [C.c_void_p, _ip]),
     "fvmr_node_field": (C.c_int, [C.c_void_p, _dp * 4, _dp, _dp]),
+    "fvmr_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
+    "fvmr_autotune": (C.c_int, [C.POINTER(MeshSoA), C.POINTER(Params), C.c_int, _ip, _dp]),
     "fvmr_create_batch": (C.c_int, [C.c_int, C.POINTER(C.POINTER(MeshSoA)), C.POINTER(Params), C.POINTER(C.c_void_p)]),
     "fvmr_batch_cases": (C.c_int, [C.c_void_p, _ip, _ip]),
     "fvmr_batch_set_state": (C.c_int, [C.c_void_p, C.c_int, _dp * 4]),

@@ -1523,6 +1523,11 @@ struct fvmr_solver {
     double *partial = nullptr, *norm_out = nullptr;
     int *bad_list = nullptr;
     Scalars *sc = nullptr;
+    // steps replayed from captured graphs: three steps bring the rotating state buffers back,
+    // so one graph of three steps per starting buffer (captured on first use)
+    bool use_graphs = false;
+    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
     // cases of a batch (one for fvmr_create): cells case_cell_begin[c] .. [c+1]-1 in the caller's
     // numbering; per case a Scalars entry (sc[c]) and boundary states / CFL (cbc[c])
     int ncases = 1;
@@ -1731,6 +1736,41 @@ int run_step(fvmr_solver *h, bool store_rhs) {
     return device_step<double, double>(h, store_rhs);
 }
 
+// n steps; the last one keeps its stage-1 rhs when store_rhs_last.  Without profiling, runs of
+// three steps replay a captured graph (the same launches with the same arguments).
+int run_steps(fvmr_solver *h, int n, bool store_rhs_last) {
+    int rc, done = 0;
+    if (h->use_graphs && !h->profiling) {
+        const int eager_tail = store_rhs_last ? 1 : 0;
+        while (n - done - eager_tail >= 3) {
+            const int a = h->cur;
+            if (!h->gexec[a]) {
+                HIP_OK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+                rc = FVMR_OK;
+                for (int k = 0; k < 3 && !rc; k++) rc = run_step(h, false);
+                hipGraph_t g = nullptr;
+                const hipError_t e = hipStreamEndCapture(h->stream, &g);
+                h->cur = a;   // capture launches nothing; the buffers rotate when the graph runs
+                if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+                if (e != hipSuccess) return set_error(FVMR_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+                hipGraphExec_t x = nullptr;
+                const hipError_t e2 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+                if (e2 != hipSuccess) {
+                    (void)hipGraphDestroy(g);
+                    return set_error(FVMR_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e2));
+                }
+                h->graph[a] = g;
+                h->gexec[a] = x;
+            }
+            HIP_OK(hipGraphLaunch(h->gexec[a], h->stream));
+            done += 3;
+        }
+    }
+    for (; done < n; done++)
+        if ((rc = run_step(h, store_rhs_last && done == n - 1))) return rc;
+    return FVMR_OK;
+}
+
 template <class G>
 int build_geometry(fvmr_solver *h, const fvmr_mesh_soa *m, const std::vector<int> &eperm) {
     // eperm[new edge] = reference edge
@@ -1755,7 +1795,17 @@ int build_geometry(fvmr_solver *h, const fvmr_mesh_soa *m, const std::vector<int
     return FVMR_OK;
 }
 
+void free_graphs(fvmr_solver *h) {
+    for (int k = 0; k < 3; k++) {
+        if (h->gexec[k]) (void)hipGraphExecDestroy(h->gexec[k]);
+        if (h->graph[k]) (void)hipGraphDestroy(h->graph[k]);
+        h->gexec[k] = nullptr;
+        h->graph[k] = nullptr;
+    }
+}
+
 void free_all(fvmr_solver *h) {
+    free_graphs(h);
     for (auto &b : h->Ubuf) if (b) (void)hipFree(b);
     void *ptrs[] = {h->grad, h->rhs, h->fix, h->cgeo, h->nxy, h->egeo, h->emid, h->cslot, h->cnode, h->dtcode,
                     h->eint, h->partial, h->norm_out, h->bad_list, h->sc, h->t_info, h->t_halo, h->t_lnodes,
@@ -2214,8 +2264,7 @@ int fvmr_advance(fvmr_handle *h, int nsteps, double *t, double T, fvmr_step_info
     s.T = T;
     s.counter = 0; s.bad_count = 0; s.n_restricted = 0;
     HIP_OK(hipMemcpyAsync(h->sc, &s, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
-    for (int i = 0; i < nsteps; i++)
-        if ((rc = run_step(h, i == nsteps - 1))) return rc;
+    if ((rc = run_steps(h, nsteps, true))) return rc;
     if ((rc = read_scalars(h, s))) return rc;
     if (info) {
         info->t = s.t;
@@ -2273,8 +2322,7 @@ int fvmr_batch_advance(fvmr_handle *h, int nsteps, double *t, const double *T) {
     s[0].counter = 0; s[0].bad_count = 0; s[0].n_restricted = 0;
     HIP_OK(hipMemcpyAsync(h->sc, s.data(), s.size() * sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
     int rc;
-    for (int i = 0; i < nsteps; i++)
-        if ((rc = run_step(h, false))) return rc;
+    if ((rc = run_steps(h, nsteps, false))) return rc;
     HIP_OK(hipMemcpyAsync(s.data(), h->sc, s.size() * sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
     if (h->profiling && (rc = prof_collect(h))) return rc;
@@ -2450,6 +2498,59 @@ int fvmr_surface_force(fvmr_handle *h, int nsets, const int *set_offsets, const 
     release();
     if (e != hipSuccess) return set_error(FVMR_ERR_HIP, std::string("surface force: ") + hipGetErrorString(e));
     for (int s = 0; s < nsets; s++) { Cx[s] = out[2 * s]; Cy[s] = out[2 * s + 1]; }
+    return FVMR_OK;
+}
+
+// Times `steps` SSP-RK3 / Euler steps of the mesh's uniform freestream for each candidate tile
+// size (both tile shapes' ranges, clipped to the LDS capacity) and returns the fastest.
+int fvmr_autotune(const fvmr_mesh_soa *m, const fvmr_params *p, int steps, int *best_tile_cells, double *best_ms) {
+    if (!m || !p || !best_tile_cells || steps < 1) return set_error(FVMR_ERR_INVALID_ARG, "invalid argument");
+    const bool mixed = p->precision == FVMR_PREC_MIXED;
+    const int dflt = mixed ? TileCfg<float>::kCells : TileCfg<double>::kCells;
+    const int cap = mixed ? TileCfg<float>::kMaxG : TileCfg<double>::kMaxG;
+    std::vector<int> cand;
+    for (double f : {0.6, 0.75, 0.9, 1.0, 1.15})
+        cand.push_back(std::min(cap, std::max(8, (int)std::lround(f * dflt))));
+    std::vector<double> U0[4];
+    for (auto &u : U0) u.resize(m->num_element);
+    double *Urows[4] = {U0[0].data(), U0[1].data(), U0[2].data(), U0[3].data()};
+    int rc = fvmr_field_init(m, p, FVMR_INIT_UNIFORM, nullptr, Urows);
+    if (rc) return rc;
+    const double *Uc[4] = {Urows[0], Urows[1], Urows[2], Urows[3]};
+    double best = 1e300;
+    int best_c = dflt;
+    for (int c : cand) {
+        fvmr_params q = *p;
+        q.tile_cells = c;
+        q.kernels = FVMR_KERNELS_TILE;
+        fvmr_handle *h = nullptr;
+        if ((rc = fvmr_create(m, &q, &h))) return rc;
+        double t = 0.0;
+        float ms = 0.0f;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        rc = fvmr_set_state(h, Uc);
+        if (!rc) rc = fvmr_advance(h, 2, &t, 1e300, nullptr);
+        if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+            rc = set_error(FVMR_ERR_HIP, "hipEventCreate failed");
+        if (!rc && hipEventRecord(e0, h->stream) != hipSuccess) rc = set_error(FVMR_ERR_HIP, "hipEventRecord failed");
+        if (!rc) rc = fvmr_advance(h, steps, &t, 1e300, nullptr);
+        if (!rc && (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = set_error(FVMR_ERR_HIP, "event timing failed");
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        fvmr_destroy(h);
+        if (rc) return rc;
+        if (ms < best) { best = ms; best_c = c; }
+    }
+    *best_tile_cells = best_c;
+    if (best_ms) *best_ms = best / steps;
+    return FVMR_OK;
+}
+
+int fvmr_use_graphs(fvmr_handle *h, int on) {
+    if (!h) return set_error(FVMR_ERR_INVALID_ARG, "null handle");
+    h->use_graphs = (on != 0);
     return FVMR_OK;
 }
 

@@ -307,10 +307,19 @@ int fvmr_surface_force(fvmr_handle *h, int nsets, const int *set_offsets, const 
 /* ---------------------------------------------------------------------- */
 /* measurement                                                             */
 /* ---------------------------------------------------------------------- */
+/* Replay runs of three steps of fvmr_advance / fvmr_batch_advance from captured HIP graphs
+ * (default off; never while profiling).  No reference counterpart: the reference launches
+ * every kernel from the host each step. */
+int fvmr_use_graphs(fvmr_handle *h, int on);
+/* Tile-size autotuner (no reference counterpart): times `steps` steps of the uniform freestream
+ * of p on mesh m for several tile sizes and returns the fastest (set it as p->tile_cells) and
+ * its ms per step (best_ms may be NULL). */
+int fvmr_autotune(const fvmr_mesh_soa *m, const fvmr_params *p, int steps, int *best_tile_cells, double *best_ms);
 /* 1: record HIP events around every kernel launch on the handle's stream. */
 int fvmr_profile_enable(fvmr_handle *h, int on);
 /* Average ms per launch and launch count of a kernel class since the last reset:
- * kernel: 0 = flux/update (stage), 1 = gradient+limiter, 2 = dt, 3 = restriction, 4 = residual. */
+ * kernel: 0 = stage (k_tile or k_stage), 1 = k_tile_dt or k_gradient (with the dt partial in
+ * stage 1), 2 = dt finish, 3 = restriction, 4 = residual. */
 int fvmr_profile_query(fvmr_handle *h, int kernel, double *avg_ms, int *count);
 int fvmr_profile_reset(fvmr_handle *h);
 /* Wait for all work of the handle. */

@@ -102,3 +102,15 @@ def test_nonstandard_geometry_falls_back_to_cells():
     Ua, _ = run(sm, p, U, 3, F.KERNELS_AUTO)
     Uc, _ = run(sm, p, U, 3, F.KERNELS_CELL)
     assert np.array_equal(Ua, Uc)
+
+
+def test_autotune_returns_a_working_tile_size():
+    announce("autotune: wedge 128x64 Roe, FP64 and mixed")
+    for prec in (F.PREC_FP64, F.PREC_MIXED):
+        m, p, U = wedge_case(128, 64, scheme=F.SOL_ROE)
+        p.precision = prec
+        best, ms = F.autotune(m, p, steps=5)
+        assert 8 <= best <= 400 and ms > 0
+        Ud, _ = run(m, p, U, 3, F.KERNELS_TILE)
+        Ub, _ = run(m, p, U, 3, F.KERNELS_TILE, tile_cells=best)
+        assert rel_l2(Ub, Ud) < (1e-13 if prec == F.PREC_FP64 else 1e-6)
