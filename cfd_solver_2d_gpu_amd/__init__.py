@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _abi
 from ._abi import *  # noqa: F401,F403  (codes)
-from ._abi import FvmrError, GenParams, MeshSoA, Params, StepInfo, check, lib
+from ._abi import AdjointResult, FvmrError, GenParams, MeshSoA, Params, StepInfo, check, lib
 
 __all__ = ["Mesh", "GPUSolver2", "BatchSolver", "autotune", "field_init", "default_params", "ruvp_from_mach", "FvmrError",
            "Params", "StepInfo", "MeshSoA", "lib"]
@@ -292,7 +292,7 @@ class GPUSolver2:
         return p
 
     def use_graphs(self, on=True):
-        """Replay runs of three steps from captured HIP graphs (default on)."""
+        """Replay runs of three steps from captured HIP graphs (default off)."""
         check(lib().fvmr_use_graphs(self._h, 1 if on else 0), "fvmr_use_graphs")
 
     def node_field(self):
@@ -321,6 +321,63 @@ class GPUSolver2:
         check(lib().fvmr_surface_force(self._h, len(sets), _iptr(off), _iptr(edges), float(S_ref), _dptr(cx),
                                        _dptr(cy)), "fvmr_surface_force")
         return cx[:len(sets)], cy[:len(sets)]
+
+    def _sets(self, sets):
+        off = np.zeros(len(sets) + 1, dtype=np.int32)
+        off[1:] = np.cumsum([len(e) for e in sets])
+        edges = np.ascontiguousarray(np.concatenate([np.asarray(e, dtype=np.int32) for e in sets])
+                                     if len(sets) else np.zeros(1, np.int32))
+        return off, edges
+
+    def set_boundary_states(self, inf, inlet=None, outlet=None):
+        """Replace the freestream / inlet / outlet (rho, u, v, p) of the handle."""
+        p = self.params
+        arr = [np.ascontiguousarray(np.asarray(v if v is not None else d, dtype=np.float64))
+               for v, d in ((inf, p.inf_ruvp), (inlet, p.inlet_ruvp), (outlet, p.outlet_ruvp))]
+        check(lib().fvmr_set_boundary_states(self._h, *[_dptr(a) for a in arr]), "fvmr_set_boundary_states")
+        for k in range(4):
+            p.inf_ruvp[k], p.inlet_ruvp[k], p.outlet_ruvp[k] = arr[0][k], arr[1][k], arr[2][k]
+
+    def steady_solve(self, max_iters, tol):
+        """Pseudo-time iteration with per-cell dt to r(U) = 0: (iterations, first, last L2 of
+        the density residual)."""
+        it = C.c_int()
+        res = np.zeros(2)
+        check(lib().fvmr_steady_solve(self._h, int(max_iters), float(tol), C.byref(it), _dptr(res)),
+              "fvmr_steady_solve")
+        return it.value, res[0], res[1]
+
+    def adjoint_vjp(self, psi, alpha=False, coords=False):
+        """(dr/dU)^T psi at the current state, plus psi . dr/dalpha and psi . dr/dX (2 x num_node)
+        when asked."""
+        P, prow = _rows(np.array(psi, dtype=np.float64, copy=True))
+        out, orow = _rows(np.zeros((4, self.n)))
+        da = C.c_double()
+        dxy = np.zeros(2 * self.mesh.soa.num_node)
+        check(lib().fvmr_adjoint_vjp(self._h, prow, orow, C.byref(da) if alpha else None,
+                                     _dptr(dxy) if coords else None), "fvmr_adjoint_vjp")
+        res = [out]
+        if alpha:
+            res.append(da.value)
+        if coords:
+            res.append(dxy.reshape(-1, 2).T.copy())
+        return res[0] if len(res) == 1 else tuple(res)
+
+    def adjoint_solve(self, sets, S_ref=1.0, max_iters=10000, tol=1e-10, coords=False):
+        """Discrete adjoint of J = sum over sets of (-sin a Cx + cos a Cy): returns
+        (AdjointResult, dJ/dX as 2 x num_node or None)."""
+        off, edges = self._sets(sets)
+        out = AdjointResult()
+        dxy = np.zeros(2 * self.mesh.soa.num_node) if coords else None
+        check(lib().fvmr_adjoint_solve(self._h, len(sets), _iptr(off), _iptr(edges), float(S_ref), int(max_iters),
+                                       float(tol), _dptr(dxy) if coords else None, C.byref(out)),
+              "fvmr_adjoint_solve")
+        return out, (dxy.reshape(-1, 2).T.copy() if coords else None)
+
+    def adjoint_state(self):
+        P, rows = _rows(np.zeros((4, self.n)))
+        check(lib().fvmr_adjoint_state(self._h, rows), "fvmr_adjoint_state")
+        return P
 
     def profile(self, on=True):
         check(lib().fvmr_profile_enable(self._h, 1 if on else 0), "fvmr_profile_enable")

@@ -67,6 +67,13 @@ class StepInfo(C.Structure):
     ]
 
 
+class AdjointResult(C.Structure):
+    _fields_ = [
+        ("J", C.c_double), ("dJ_dalpha", C.c_double), ("res0", C.c_double), ("res", C.c_double),
+        ("iters", C.c_int), ("reserved", C.c_int),
+    ]
+
+
 class GenParams(C.Structure):
     _fields_ = [
         ("ni", C.c_int), ("nj", C.c_int),
@@ -108,6 +115,12 @@ SIGNATURES = {
     "fvmr_node_field": (C.c_int, [C.c_void_p, _dp * 4, _dp, _dp]),
     "fvmr_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "fvmr_autotune": (C.c_int, [C.POINTER(MeshSoA), C.POINTER(Params), C.c_int, _ip, _dp]),
+    "fvmr_set_boundary_states": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "fvmr_steady_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _ip, _dp]),
+    "fvmr_adjoint_vjp": (C.c_int, [C.c_void_p, _dp * 4, _dp * 4, _dp, _dp]),
+    "fvmr_adjoint_solve": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, C.c_double, C.c_int, C.c_double, _dp,
+                                     C.POINTER(AdjointResult)]),
+    "fvmr_adjoint_state": (C.c_int, [C.c_void_p, _dp * 4]),
     "fvmr_create_batch": (C.c_int, [C.c_int, C.POINTER(C.POINTER(MeshSoA)), C.POINTER(Params), C.POINTER(C.c_void_p)]),
     "fvmr_batch_cases": (C.c_int, [C.c_void_p, _ip, _ip]),
     "fvmr_batch_set_state": (C.c_int, [C.c_void_p, C.c_int, _dp * 4]),

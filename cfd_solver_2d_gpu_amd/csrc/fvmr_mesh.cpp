@@ -489,7 +489,8 @@ int gen_ogrid(const fvmr_gen_params &g, RawLists &r, const std::vector<double> *
 }
 
 // NACA 4-digit symmetric/cambered surface, closed trailing edge, cosine spacing;
-// ni points ordered counter-clockwise starting at the trailing edge (lower side first).
+// ni points ordered counter-clockwise starting at the trailing edge (upper side first), so
+// that point i lies on the ray of the farfield point at angle 2 pi i / ni.
 void naca_surface(int digits, int ni, std::vector<double> &xs, std::vector<double> &ys) {
     double m = (digits / 1000) / 100.0, pp = ((digits / 100) % 10) / 10.0, t = (digits % 100) / 100.0;
     xs.resize(ni); ys.resize(ni);
@@ -503,7 +504,7 @@ void naca_surface(int digits, int ni, std::vector<double> &xs, std::vector<doubl
             else { yc = m / ((1 - pp) * (1 - pp)) * ((1 - 2 * pp) + 2 * pp * x - x * x); dyc = 2 * m / ((1 - pp) * (1 - pp)) * (pp - x); }
         }
         double thc = std::atan(dyc);
-        double sgn = (i <= ni / 2) ? -1.0 : 1.0;     // lower surface first (counter-clockwise)
+        double sgn = (i <= ni / 2) ? 1.0 : -1.0;     // upper surface first (counter-clockwise)
         xs[i] = x - sgn * yt * std::sin(thc) - 0.5;  // centred on mid-chord
         ys[i] = yc + sgn * yt * std::cos(thc);
     }

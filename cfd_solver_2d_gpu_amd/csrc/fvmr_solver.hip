@@ -263,8 +263,8 @@ __device__ __forceinline__ void ghost_slip(const FaceState<C> &L, FaceState<C> &
 }
 
 // get_UR_farField_device (FluxGPU.cu:87-167): Riemann-invariant ghost in four regimes
-template <class C>
-__device__ __forceinline__ void ghost_farfield(const FaceState<C> &L, const double *ruvp_inf, FaceState<C> &R, C nx,
+template <class C, class P>
+__device__ __forceinline__ void ghost_farfield(const FaceState<C> &L, const P *ruvp_inf, FaceState<C> &R, C nx,
                                                C ny, C gamma) {
     const C ga1 = gamma - 1;
     const C rho_f = (C)ruvp_inf[0], u_f = (C)ruvp_inf[1], v_f = (C)ruvp_inf[2], p_f = (C)ruvp_inf[3];
@@ -442,6 +442,49 @@ __device__ __forceinline__ void flux_viscous(const Cells &cells, const Phys &ph,
     F[3] = (Phix * nx + Phiy * ny) * area;
 }
 
+// The boundary state a farfield-type face uses: inlet and outlet their own, the double Mach
+// reflection boundary inlet or outlet by the shock position at t_rk, the others the
+// freestream (FluxGPU.cu:236-276).
+__device__ __forceinline__ const double *bc_state(const Phys &ph, const CaseBC &bc, int bType, double xm, double ym,
+                                                  double t_rk) {
+    if (bType == FVMR_BC_INLET) return bc.inlet;
+    if (bType == FVMR_BC_OUTLET) return bc.outlet;
+    if (bType == FVMR_BC_DSR) {
+        const double right = ph.dsr_x + (ym + ph.dsr_speed * t_rk) / sqrt(3.0);
+        return (xm < right) ? bc.inlet : bc.outlet;
+    }
+    return bc.inf;
+}
+
+// Ghost state R of a boundary face of type bType from the reconstructed interior state L
+// (FluxGPU.cu:63-167): slip walls and symmetry reflect the normal velocity, the adiabatic
+// wall keeps rho and rhoE with zero momenta, the isothermal wall is new, every other type
+// is a Riemann-invariant farfield against ruvp.
+template <class C, class P>
+__device__ __forceinline__ void ghost_state(const Phys &ph, int bType, const P *ruvp, const FaceState<C> &L,
+                                            FaceState<C> &R, C nx, C ny, C gamma) {
+    if (bType == FVMR_BC_SYMMETRY || bType == FVMR_BC_WALL_NONVISCOUS) {
+        ghost_slip(L, R, nx, ny, gamma);
+    } else if (bType == FVMR_BC_WALL_ADIABAT) {
+        // FluxGPU.cu:79-85: momenta of a zeroed U_R negated
+        R.U[0] = L.U[0]; R.U[1] = -C(0); R.U[2] = -C(0); R.U[3] = L.U[3];
+        set_prim(R, gamma);
+    } else if (bType == FVMR_BC_WALL_ISOTHERMAL) {
+        ghost_isothermal(L, R, (C)ph.wall_T, (C)ph.R, gamma);
+    } else {
+        ghost_farfield(L, ruvp, R, nx, ny, gamma);
+    }
+}
+
+// get_flux_RiemannSolve_device (FluxGPU.cu:190-205): LLF or Roe on the two face states.
+template <class C>
+__device__ __forceinline__ void riemann_flux(const Phys &ph, const FaceState<C> &L, const FaceState<C> &R, C nx, C ny,
+                                             C len, C F[4]) {
+    const C gamma = (C)ph.gamma, g_ga1 = (C)ph.g_ga1;
+    if (ph.scheme == FVMR_SOL_LLF) flux_llf(L, R, nx, ny, len, F, gamma, g_ga1);
+    else flux_roe(L, R, nx, ny, len, F, gamma, g_ga1, (C)ph.inv_g);
+}
+
 // edge_convection_flux_device_kernel body (FluxGPU.cu:207-283) (+ viscous part for NS):
 // the canonical flux of a face, oriented L -> R.  bType: BC code of the face's set (-1
 // interior); (pmx, pmy): midpoint of the periodic pair face, where R is reconstructed.
@@ -468,27 +511,10 @@ __device__ __forceinline__ void face_flux(const Cells &cells, const Phys &ph, co
             for (int i = 0; i < 4; i++) R.U[i] = C(0);
             set_prim(R, gamma);
         }
-    } else if (bType == FVMR_BC_SYMMETRY || bType == FVMR_BC_WALL_NONVISCOUS) {
-        ghost_slip(L, R, nx, ny, gamma);
-    } else if (bType == FVMR_BC_WALL_ADIABAT) {
-        // FluxGPU.cu:79-85: momenta of a zeroed U_R negated
-        R.U[0] = L.U[0]; R.U[1] = -C(0); R.U[2] = -C(0); R.U[3] = L.U[3];
-        set_prim(R, gamma);
-    } else if (bType == FVMR_BC_WALL_ISOTHERMAL) {
-        ghost_isothermal(L, R, (C)ph.wall_T, (C)ph.R, gamma);
     } else {
-        const double *ruvp = bc.inf;
-        if (bType == FVMR_BC_INLET) ruvp = bc.inlet;
-        else if (bType == FVMR_BC_OUTLET) ruvp = bc.outlet;
-        else if (bType == FVMR_BC_DSR) {
-            const double right = ph.dsr_x + ((double)em.y + ph.dsr_speed * t_rk) / sqrt(3.0);
-            ruvp = ((double)em.x < right) ? bc.inlet : bc.outlet;
-        }
-        ghost_farfield(L, ruvp, R, nx, ny, gamma);
+        ghost_state(ph, bType, bc_state(ph, bc, bType, (double)em.x, (double)em.y, t_rk), L, R, nx, ny, gamma);
     }
-    const C g_ga1 = (C)ph.g_ga1;
-    if (ph.scheme == FVMR_SOL_LLF) flux_llf(L, R, nx, ny, len, F, gamma, g_ga1);
-    else flux_roe(L, R, nx, ny, len, F, gamma, g_ga1, (C)ph.inv_g);
+    riemann_flux(ph, L, R, nx, ny, len, F);
     if (NS) {
         C Fv[4];
         flux_viscous<C, G>(cells, ph, Lc, Rc, nx, ny, len, Fv);
@@ -1548,6 +1574,9 @@ struct fvmr_solver {
     std::vector<int> bedge_ref;               // caller ids of boundary-set edges, ascending
     std::vector<int> bedge_nodes;             // 2 per boundary edge
     std::vector<double> bedge_geo;            // nx, ny, L per boundary edge
+    std::vector<int> enodes;                  // 2 per device edge: its nodes in the caller's order
+    int bc_flags = 0;                         // boundary kinds present (periodic, DSR, isothermal)
+    void *adj = nullptr;                      // steady-solve / adjoint workspace
     bool rhs_valid = false;
     bool profiling = false;
     Prof prof[kNumProf];
@@ -1804,8 +1833,11 @@ void free_graphs(fvmr_solver *h) {
     }
 }
 
+void adj_free(fvmr_solver *h);
+
 void free_all(fvmr_solver *h) {
     free_graphs(h);
+    adj_free(h);
     for (auto &b : h->Ubuf) if (b) (void)hipFree(b);
     void *ptrs[] = {h->grad, h->rhs, h->fix, h->cgeo, h->nxy, h->egeo, h->emid, h->cslot, h->cnode, h->dtcode,
                     h->eint, h->partial, h->norm_out, h->bad_list, h->sc, h->t_info, h->t_halo, h->t_lnodes,
@@ -1939,6 +1971,17 @@ static int create_impl(const fvmr_mesh_soa *m, const fvmr_params *p, int ncases,
             if (enew[e] < 0) { enew[e] = (int)eperm.size(); eperm.push_back(e); }
         }
     for (int e = 0; e < nf; e++) if (enew[e] < 0) { enew[e] = (int)eperm.size(); eperm.push_back(e); }
+    h->enodes.resize(2 * (size_t)nf);
+    for (int e = 0; e < nf; e++) {
+        const int r = eperm[e];
+        h->enodes[2 * (size_t)e] = m->edge_nodes[0][r];
+        h->enodes[2 * (size_t)e + 1] = m->edge_nodes[1][r];
+        const int s = m->edge_setID[r];
+        const int t = (s == -1) ? -1 : m->boundary_type[s - 1];
+        if (m->edge_periodicPair[r] >= 0 || (t >= FVMR_BC_PERIODIC_0 && t <= FVMR_BC_PERIODIC_0 + 9)) h->bc_flags |= 1;
+        if (t == FVMR_BC_DSR) h->bc_flags |= 2;
+        if (t == FVMR_BC_WALL_ISOTHERMAL) h->bc_flags |= 4;
+    }
     // connectivity in the new numbering
     if (nf > kSlotEdgeMask) return bail(set_error(FVMR_ERR_UNSUPPORTED, "too many edges for the face-slot encoding"));
     std::vector<int2> cslot(3 * (size_t)nc);
@@ -2584,3 +2627,5 @@ int fvmr_synchronize(fvmr_handle *h) {
 }
 
 }  // extern "C"
+
+#include "fvmr_adjoint.inl"

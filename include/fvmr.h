@@ -305,6 +305,42 @@ int fvmr_surface_force(fvmr_handle *h, int nsets, const int *set_offsets, const 
                        double *Cx, double *Cy);
 
 /* ---------------------------------------------------------------------- */
+/* steady states and the discrete adjoint (no reference counterpart)       */
+/* ---------------------------------------------------------------------- */
+typedef struct fvmr_adjoint_result {
+    double J;            /* lift functional of the state (below) */
+    double dJ_dalpha;    /* total derivative, per radian of the freestream angle */
+    double res0, res;    /* L2 of (dr/dU)^T psi + dJ/dU at the first check and at the end */
+    int    iters;        /* pseudo-time iterations done */
+} fvmr_adjoint_result;
+
+/* Replace the boundary states of a one-case handle (the reference reads them once from
+ * input.toml; here a freestream angle can be varied without a new handle). */
+int fvmr_set_boundary_states(fvmr_handle *h, const double inf[4], const double inlet[4], const double outlet[4]);
+/* Drive the current state towards r(U) = 0 by pseudo-time iteration with per-cell time steps
+ * dt_c = CFL V_c / sum over the cell's faces of (|u.n| + a) L (face-mean primitives; NS adds a
+ * viscous term), the handle's time scheme and no restriction.  Stops after max_iters or once the L2 norm of the
+ * density residual, checked every 50 iterations, is at most tol times its first value;
+ * res[0] = first norm, res[1] = last (res may be NULL).  FP64 one-case handles. */
+int fvmr_steady_solve(fvmr_handle *h, int max_iters, double tol, int *iters, double res[2]);
+/* Products with the transposed residual Jacobian at the current state U (caller's cell and node
+ * order): out = (dr/dU)^T psi; *dalpha = psi . dr/dalpha, alpha the freestream angle
+ * (d(u_inf, v_inf)/dalpha = (-v_inf, u_inf)); dxy[2v], dxy[2v+1] = psi . dr/d(x_v, y_v).
+ * dalpha and dxy may be NULL.  FP64 one-case Euler handles without periodic, double Mach
+ * reflection or isothermal boundaries. */
+int fvmr_adjoint_vjp(fvmr_handle *h, const double *const psi[4], double *const out[4], double *dalpha, double *dxy);
+/* Discrete adjoint of the lift functional J = sum over the listed sets of
+ * (-sin a Cx + cos a Cy), a = atan2(v_inf, u_inf), Cx, Cy as fvmr_surface_force, at the current
+ * (steady) state: psi from the pseudo-time iteration of (dr/dU)^T psi = -dJ/dU (per-cell dt of
+ * the state, the handle's time scheme, from psi = 0; stops as fvmr_steady_solve does, on the L2
+ * norm of all four components), then dJ/dalpha and, when dJ_dxy is not NULL, dJ/d(x_v, y_v)
+ * (2 nv, interleaved).  Same restrictions as fvmr_adjoint_vjp. */
+int fvmr_adjoint_solve(fvmr_handle *h, int nsets, const int *set_offsets, const int *set_edges, double S_ref,
+                       int max_iters, double tol, double *dJ_dxy, fvmr_adjoint_result *out);
+/* The adjoint field psi of the last fvmr_adjoint_solve (caller's cell order). */
+int fvmr_adjoint_state(fvmr_handle *h, double *const psi[4]);
+
+/* ---------------------------------------------------------------------- */
 /* measurement                                                             */
 /* ---------------------------------------------------------------------- */
 /* Replay runs of three steps of fvmr_advance / fvmr_batch_advance from captured HIP graphs

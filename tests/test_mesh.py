@@ -89,6 +89,12 @@ def check_invariants(m):
     nn = np.hypot(*a["edge_normal"])
     assert np.max(np.abs(nn - 1)) < 1e-7
     assert np.all(a["element_volume"] > 0)
+    # no overlapping cells: the areas sum to the domain's area, the boundary integral of x n_x
+    # (boundary edges have the domain outside their R side)
+    b = R < 0
+    b |= per
+    area = np.sum(a["edge_xy"][0][b] * a["edge_normal"][0][b] * a["edge_length"][b])
+    assert abs(a["element_volume"].sum() / area - 1) < 1e-6
     # periodic pairs are symmetric translates and R is the pair's L
     pe = a["edge_periodicPair"][per]
     assert np.array_equal(a["edge_periodicPair"][pe], np.nonzero(per)[0])
@@ -106,6 +112,8 @@ def test_invariants_on_the_1M_wedge():
 def test_invariants_on_periodic_and_ogrid():
     assert check_invariants(F.Mesh.generate(F.GEN_PERIODIC_SQUARE, 64, 64, b=10.0)) == 1
     assert check_invariants(F.Mesh.generate(F.GEN_CYLINDER_OGRID, 64, 32, a=0.5, b=30.0, c=1.02)) == 0
+    assert check_invariants(F.Mesh.generate(F.GEN_NACA_OGRID, 64, 24, b=20.0, c=1.1, naca_digits=12)) == 0
+    assert check_invariants(F.Mesh.generate(F.GEN_NACA_OGRID, 64, 24, b=20.0, c=1.1, naca_digits=2412)) == 0
 
 
 def test_builder_errors(tmp_path):
