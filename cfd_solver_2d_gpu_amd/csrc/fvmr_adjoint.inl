@@ -36,12 +36,18 @@ using fvmr::Dual;
 constexpr int kAdjBlock = 128;
 constexpr int kAdjCheck = 50;       // iterations between residual checks
 constexpr int kXRec = 18;           // node entries per cell of the coordinate product
+#ifndef FVMR_ADJ_NT
+#define FVMR_ADJ_NT 4               // derivative directions per pass of the product kernel
+#endif
+#ifndef FVMR_ADJ_WAVES
+#define FVMR_ADJ_WAVES 1            // minimum waves per SIMD the product kernel is compiled for
+#endif
 
 constexpr int kBcFlagPeriodic = 1, kBcFlagDsr = 2, kBcFlagIsothermal = 4;
 
 // Device workspace of the steady solve and the adjoint (allocated on first use).
 struct AdjWork {
-    S4 *psi[3] = {nullptr, nullptr, nullptr};
+    S4 *psi[4] = {nullptr, nullptr, nullptr, nullptr};   // 0, 1 iterates; 2, 3 stages
     S4 *base = nullptr, *wedge = nullptr, *g = nullptr, *res = nullptr;
     double *dtl = nullptr, *wnode = nullptr, *partial = nullptr;
     int2 *enode = nullptr;
@@ -55,7 +61,7 @@ struct AdjWork {
 void adj_free(fvmr_solver *h) {
     AdjWork *w = (AdjWork *)h->adj;
     if (!w) return;
-    void *ptrs[] = {w->psi[0], w->psi[1], w->psi[2], w->base, w->wedge, w->g, w->res, w->dtl, w->wnode,
+    void *ptrs[] = {w->psi[0], w->psi[1], w->psi[2], w->psi[3], w->base, w->wedge, w->g, w->res, w->dtl, w->wnode,
                     w->partial, w->enode, w->gbar, w->xval, w->xnode};
     for (void *q : ptrs) if (q) (void)hipFree(q);
     delete w;
@@ -198,26 +204,39 @@ __global__ __launch_bounds__(kBlock) void k_steady_stage(Mesh<double> m, Phys ph
     st(Uout, c, Unew);
 }
 
-// Gradient slots of grad_limit with state derivatives: tangent 0 = the cell's own state,
-// tangent 1 + i = the state across slot i (every variable at once: variables do not mix).
+// Gradient slots of grad_limit with state derivatives in NT of the four directions
+// (0 = the cell's own state, 1 + i = the state across slot i; every variable at once, as the
+// variables do not mix).
+template <int NT>
 struct SlotsU {
     double a[3][3];
-    Dual<4> uo[3][4];
+    Dual<NT> uo[3][4];
     double n[3][2];
-    __device__ __forceinline__ Dual<4> A(int i, int q) const { return Dual<4>(a[i][q]); }
-    __device__ __forceinline__ Dual<4> Uo(int i, int k) const { return uo[i][k]; }
-    __device__ __forceinline__ Dual<4> N(int i, int q) const { return Dual<4>(n[i][q]); }
+    __device__ __forceinline__ Dual<NT> A(int i, int q) const { return Dual<NT>(a[i][q]); }
+    __device__ __forceinline__ Dual<NT> Uo(int i, int k) const { return uo[i][k]; }
+    __device__ __forceinline__ Dual<NT> N(int i, int q) const { return Dual<NT>(n[i][q]); }
 };
+
+// x with derivative 1 in direction dir if dir is one of p0 .. p0+NT-1
+template <int NT>
+__device__ __forceinline__ Dual<NT> seed_in(double x, int dir, int p0) {
+    Dual<NT> r(x);
+#pragma unroll
+    for (int t = 0; t < NT; t++) r.d[t] = (dir == p0 + t) ? 1.0 : 0.0;
+    return r;
+}
 
 // The product with (dr/dU)^T, part 1 (one thread per cell c): base[c] = the terms that land on
 // U_c, wedge[2 e + side] = the terms c sends to the cell across its edge e (side 1 when c is
-// the edge's R).  With gbar, also the adjoint of c's limited gradient.
-template <bool GBAR>
-__global__ __launch_bounds__(kAdjBlock) void k_adj_vjp(Mesh<double> m, Phys ph, const S4 *__restrict__ U,
+// the edge's R).  With gbar, also the adjoint of c's limited gradient.  The four derivative
+// directions of each local Jacobian are taken NT at a time (Dual<NT>, the values recomputed
+// per pass), which bounds the registers of the flux and limiter templates.
+template <bool GBAR, int NT>
+__global__ __launch_bounds__(kAdjBlock, FVMR_ADJ_WAVES) void k_adj_vjp(Mesh<double> m, Phys ph, const S4 *__restrict__ U,
                                                         const Grad<double> *__restrict__ grad,
                                                         const S4 *__restrict__ psi, S4 *__restrict__ base,
                                                         S4 *__restrict__ wedge, Grad<double> *__restrict__ gbar) {
-    using D = Dual<4>;
+    using D = Dual<NT>;
     const int c = blockIdx.x * kAdjBlock + threadIdx.x;
     const int nc = m.nc;
     if (c >= nc) return;
@@ -232,6 +251,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_vjp(Mesh<double> m, Phys ph, 
     int outs[3], edges[3];
     bool isR[3];
     const D gam(ph.gamma);
+#pragma unroll 1
     for (int j = 0; j < 3; j++) {
         const int2 sl = m.cslot[j * nc + c];
         const int e = sl.x & kSlotEdgeMask, out = sl.y;
@@ -256,49 +276,57 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_vjp(Mesh<double> m, Phys ph, 
 #pragma unroll
             for (int k = 0; k < 4; k++) mu[k] = lc[k];
         }
-        double Uf[4];
+        double Uf[4], Ufo[4] = {0, 0, 0, 0};
         const bool lin = recon_state(ph, Uc, gc, cg.x, cg.y, em.x, em.y, Uf);
-        FaceState<D> So, Sx;
-#pragma unroll
-        for (int i = 0; i < 4; i++) So.U[i] = D::seed(Uf[i], i);
-        set_prim(So, gam);
         if (bType == -1) {
-            double Uo[4], Ufo[4];
+            double Uo[4];
             ld(U, out, Uo);
             const V4<double> og = m.cgeo[out];
             recon_state(ph, Uo, grad[out], og.x, og.y, em.x, em.y, Ufo);
-#pragma unroll
-            for (int i = 0; i < 4; i++) Sx.U[i] = D(Ufo[i]);
-            set_prim(Sx, gam);
-        } else {
-            ghost_state(ph, bType, bc_state(ph, ph.bc, bType, em.x, em.y, 0.0), So, Sx, D(eg.x), D(eg.y), gam);
         }
-        D F[4];
-        if (cR) riemann_flux(ph, Sx, So, D(eg.x), D(eg.y), D(eg.z), F);
-        else riemann_flux(ph, So, Sx, D(eg.x), D(eg.y), D(eg.z), F);
+        const double *ruvp = bc_state(ph, ph.bc, bType, em.x, em.y, 0.0);
+        double u4[4];
+#pragma unroll 1
+        for (int p0 = 0; p0 < 4; p0 += NT) {
+            FaceState<D> So, Sx;
+#pragma unroll
+            for (int i = 0; i < 4; i++) So.U[i] = seed_in<NT>(Uf[i], i, p0);
+            set_prim(So, gam);
+            if (bType == -1) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) Sx.U[i] = D(Ufo[i]);
+                set_prim(Sx, gam);
+            } else {
+                ghost_state(ph, bType, ruvp, So, Sx, D(eg.x), D(eg.y), gam);
+            }
+            D F[4];
+            if (cR) riemann_flux(ph, Sx, So, D(eg.x), D(eg.y), D(eg.z), F);
+            else riemann_flux(ph, So, Sx, D(eg.x), D(eg.y), D(eg.z), F);
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+                u4[p0 + t] = mu[0] * F[0].d[t] + mu[1] * F[1].d[t] + mu[2] * F[2].d[t] + mu[3] * F[3].d[t];
+        }
         const double dx = em.x - cg.x, dy = em.y - cg.y;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const double u = mu[0] * F[0].d[i] + mu[1] * F[1].d[i] + mu[2] * F[2].d[i] + mu[3] * F[3].d[i];
-            ub[i] += u;
+            ub[i] += u4[i];
             if (lin) {
-                gbx[i] += u * dx;
-                gby[i] += u * dy;
+                gbx[i] += u4[i] * dx;
+                gby[i] += u4[i] * dy;
             }
         }
     }
     double self[4] = {0, 0, 0, 0}, w[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
     if (ph.reconstruct == FVMR_REC_LINEAR) {
-        SlotsU S;
+        double Uo[3][4];
+        SlotsU<NT> S;
         int fl[3];
+#pragma unroll
         for (int j = 0; j < 3; j++) {
             const int out = outs[j];
             const V4<double> eg = m.egeo[edges[j]];
             const V2<double> em = m.emid[edges[j]];
-            double Uo[4];
-            ld(U, out >= 0 ? out : c, Uo);
-#pragma unroll
-            for (int k = 0; k < 4; k++) S.uo[j][k] = D::seed(Uo[k], 1 + j);
+            ld(U, out >= 0 ? out : c, Uo[j]);
             const V2<double> nd = m.nxy[m.cnode[j * nc + c]];
             S.n[j][0] = nd.x - cg.x;
             S.n[j][1] = nd.y - cg.y;
@@ -312,15 +340,28 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_vjp(Mesh<double> m, Phys ph, 
             slot_coef<double>(ph, out >= 0, cg.x, cg.y, em.x, em.y, Fx, Fy, eg.x * eg.z * sgn, eg.y * eg.z * sgn, S.a[j]);
             fl[j] = (out >= 0) ? 3 : 0;
         }
-        D Ud[4], gx[4], gy[4];
+#pragma unroll 1
+        for (int p0 = 0; p0 < 4; p0 += NT) {
+            D Ud[4], gx[4], gy[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) Ud[k] = D::seed(Uc[k], 0);
-        grad_limit<D>(ph, Ud, S, fl, D(cg.z), gx, gy);
+            for (int k = 0; k < 4; k++) {
+                Ud[k] = seed_in<NT>(Uc[k], 0, p0);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            self[k] = gbx[k] * gx[k].d[0] + gby[k] * gy[k].d[0];
+                for (int j = 0; j < 3; j++) S.uo[j][k] = seed_in<NT>(Uo[j][k], 1 + j, p0);
+            }
+            grad_limit<D>(ph, Ud, S, fl, D(cg.z), gx, gy);
 #pragma unroll
-            for (int j = 0; j < 3; j++) w[j][k] = gbx[k] * gx[k].d[1 + j] + gby[k] * gy[k].d[1 + j];
+            for (int t = 0; t < NT; t++) {
+                const int dir = p0 + t;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const double v = gbx[k] * gx[k].d[t] + gby[k] * gy[k].d[t];
+                    if (dir == 0) self[k] = v;
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+                        if (dir == 1 + j) w[j][k] = v;
+                }
+            }
         }
     }
     base[c] = S4{ub[0] + self[0], ub[1] + self[1], ub[2] + self[2], ub[3] + self[3]};
@@ -762,10 +803,10 @@ int adj_product(fvmr_solver *h, const S4 *U, const S4 *psi, bool with_gbar) {
     if (with_gbar) {
         if (!w->gbar && hipMalloc((void **)&w->gbar, (size_t)h->nc * sizeof(Grad<double>)) != hipSuccess)
             return set_error(FVMR_ERR_HIP, "hipMalloc adjoint gradient");
-        k_adj_vjp<true><<<adj_grid(h->nc), kAdjBlock, 0, h->stream>>>(m, h->ph, U, (const Grad<double> *)h->grad, psi,
+        k_adj_vjp<true, FVMR_ADJ_NT><<<adj_grid(h->nc), kAdjBlock, 0, h->stream>>>(m, h->ph, U, (const Grad<double> *)h->grad, psi,
                                                                        w->base, w->wedge, w->gbar);
     } else {
-        k_adj_vjp<false><<<adj_grid(h->nc), kAdjBlock, 0, h->stream>>>(m, h->ph, U, (const Grad<double> *)h->grad, psi,
+        k_adj_vjp<false, FVMR_ADJ_NT><<<adj_grid(h->nc), kAdjBlock, 0, h->stream>>>(m, h->ph, U, (const Grad<double> *)h->grad, psi,
                                                                         w->base, w->wedge, nullptr);
     }
     HIP_OK(hipGetLastError());
@@ -860,6 +901,24 @@ int j_sets(const fvmr_solver *h, int nsets, const int *off, const int *edges, JS
         js.en.push_back(h->bedge_nodes[2 * b + 1]);
         for (int q = 0; q < 3; q++) js.eg.push_back(h->bedge_geo[3 * b + q]);
     }
+    return FVMR_OK;
+}
+
+// One pseudo-time iteration of the adjoint: out = Phi(x) with source g (NULL: the
+// homogeneous map T x).  Stage buffers psi[2], psi[3]; res (optional) = (dr/dU)^T x + g.
+int adj_iteration(fvmr_solver *h, const S4 *U, const S4 *g, const S4 *x, S4 *out, S4 *res) {
+    AdjWork *w = (AdjWork *)h->adj;
+    S4 *Pb = w->psi[2], *Pc = w->psi[3];
+    int rc;
+    if (h->p.time_advance == FVMR_EVO_RK3) {
+        if ((rc = adj_product(h, U, x, false)) || (rc = adj_update(h, g, x, x, Pb, true, 0.0, 1.0, 1, res)) ||
+            (rc = adj_product(h, U, Pb, false)) || (rc = adj_update(h, g, Pb, x, Pc, true, 0.75, 0.25, 0, nullptr)) ||
+            (rc = adj_product(h, U, Pc, false)) ||
+            (rc = adj_update(h, g, Pc, x, out, true, 1.0 / 3.0, 2.0 / 3.0, 0, nullptr)))
+            return rc;
+        return FVMR_OK;
+    }
+    if ((rc = adj_product(h, U, x, false)) || (rc = adj_update(h, g, x, x, out, true, 0.0, 1.0, 1, res))) return rc;
     return FVMR_OK;
 }
 
@@ -971,23 +1030,13 @@ int fvmr_adjoint_solve(fvmr_handle *h, int nsets, const int *set_offsets, const 
     // pseudo-time iteration from psi = 0 with the primal's local dt at U
     k_local_dt<<<grid_of(h->nc), kBlock, 0, h->stream>>>(mesh_args<double>(h), h->ph, U, w->dtl);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipMemsetAsync(w->psi[0], 0, (size_t)h->nc * sizeof(S4), h->stream));
     double nrm[4], r0 = -1.0, r = -1.0;
     int it = 0, a = 0;
+    HIP_OK(hipMemsetAsync(w->psi[0], 0, (size_t)h->nc * sizeof(S4), h->stream));
     while (it < max_iters) {
-        const int b = (a + 1) % 3, c = (a + 2) % 3;
-        S4 *Pa = w->psi[a], *Pb = w->psi[b], *Pc = w->psi[c];
         const bool check = (it % kAdjCheck == 0);
-        if ((rc = adj_product(h, U, Pa, false)) ||
-            (rc = adj_update(h, w->g, Pa, Pa, Pb, true, 0.0, 1.0, 1, check ? w->res : nullptr)))
-            return rc;
-        if (h->p.time_advance == FVMR_EVO_RK3) {
-            if ((rc = adj_product(h, U, Pb, false)) || (rc = adj_update(h, w->g, Pb, Pa, Pc, true, 0.75, 0.25, 0, nullptr)) ||
-                (rc = adj_product(h, U, Pc, false)) ||
-                (rc = adj_update(h, w->g, Pc, Pa, Pb, true, 1.0 / 3.0, 2.0 / 3.0, 0, nullptr)))
-                return rc;
-        }
-        a = b;
+        if ((rc = adj_iteration(h, U, w->g, w->psi[a], w->psi[1 - a], check ? w->res : nullptr))) return rc;
+        a = 1 - a;
         it++;
         if (check) {
             if ((rc = field_norm2(h, w->res, nrm))) return rc;
@@ -997,6 +1046,14 @@ int fvmr_adjoint_solve(fvmr_handle *h, int nsets, const int *set_offsets, const 
             if (r <= tol * r0) break;
         }
     }
+    // the residual of the adjoint equation at the start (psi = 0: dJ/dU) and at the end
+    if ((rc = field_norm2(h, w->g, nrm))) return rc;
+    r0 = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2] + nrm[3] * nrm[3]);
+    if ((rc = adj_product(h, U, w->psi[a], false)) ||
+        (rc = adj_update(h, w->g, nullptr, nullptr, w->res, false, 0, 0, 1, nullptr)) ||
+        (rc = field_norm2(h, w->res, nrm)))
+        return rc;
+    r = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2] + nrm[3] * nrm[3]);
     w->cur = a;
     w->psi_valid = true;
     double pa = 0.0;

@@ -310,7 +310,7 @@ int fvmr_surface_force(fvmr_handle *h, int nsets, const int *set_offsets, const 
 typedef struct fvmr_adjoint_result {
     double J;            /* lift functional of the state (below) */
     double dJ_dalpha;    /* total derivative, per radian of the freestream angle */
-    double res0, res;    /* L2 of (dr/dU)^T psi + dJ/dU at the first check and at the end */
+    double res0, res;    /* L2 of (dr/dU)^T psi + dJ/dU at psi = 0 and at the end */
     int    iters;        /* pseudo-time iterations done */
 } fvmr_adjoint_result;
 
@@ -331,10 +331,12 @@ int fvmr_steady_solve(fvmr_handle *h, int max_iters, double tol, int *iters, dou
 int fvmr_adjoint_vjp(fvmr_handle *h, const double *const psi[4], double *const out[4], double *dalpha, double *dxy);
 /* Discrete adjoint of the lift functional J = sum over the listed sets of
  * (-sin a Cx + cos a Cy), a = atan2(v_inf, u_inf), Cx, Cy as fvmr_surface_force, at the current
- * (steady) state: psi from the pseudo-time iteration of (dr/dU)^T psi = -dJ/dU (per-cell dt of
- * the state, the handle's time scheme, from psi = 0; stops as fvmr_steady_solve does, on the L2
- * norm of all four components), then dJ/dalpha and, when dJ_dxy is not NULL, dJ/d(x_v, y_v)
- * (2 nv, interleaved).  Same restrictions as fvmr_adjoint_vjp. */
+ * (steady) state: psi solves (dr/dU)^T psi = -dJ/dU through the pseudo-time iteration
+ * psi <- psi + D ((dr/dU)^T psi + dJ/dU) (per-cell dt of the state, the handle's time scheme, from
+ * psi = 0), stopped after max_iters iterations or once the L2 norm of all four components of
+ * (dr/dU)^T psi + dJ/dU, checked every 50 iterations, is at most tol times its value at psi = 0.
+ * Then dJ/dalpha and, when dJ_dxy is not NULL, dJ/d(x_v, y_v) (2 nv, interleaved).  Same
+ * restrictions as fvmr_adjoint_vjp. */
 int fvmr_adjoint_solve(fvmr_handle *h, int nsets, const int *set_offsets, const int *set_edges, double S_ref,
                        int max_iters, double tol, double *dJ_dxy, fvmr_adjoint_result *out);
 /* The adjoint field psi of the last fvmr_adjoint_solve (caller's cell order). */

@@ -273,3 +273,38 @@ def test_dJ_dalpha_matches_converged_differences(limiter):
     print(f"FD dJ/da = {fd:.10f}, rel diff {rel(fd, res.dJ_dalpha):.2e}", flush=True)
     g.close()
     assert rel(fd, res.dJ_dalpha) < 1e-5
+
+
+def test_dJ_dX_matches_converged_differences():
+    announce("dJ/dX vs central differences of converged J under a smooth airfoil deformation (NACA 48 x 16)")
+    m = naca_mesh(48, 16, 1.2)
+    p = naca_params(limiter=F.LIM_NONE)
+    a = F.soa_arrays(m.soa)
+    xy = a["node_xy"].copy()
+    sid = [i + 1 for i, (name, t, e) in enumerate(m.boundary_sets()) if name == "airfoil"][0]
+    sets = [np.nonzero(a["edge_setID"] == sid)[0].astype(np.int32)]
+    # a bump lifting the middle of the airfoil, fading out by a chord
+    x, y = xy
+    d = np.stack([np.zeros_like(x), 0.02 * np.exp(-(x * x + y * y) / 0.3 ** 2)])
+
+    def J_of(xy_, U0, iters):
+        g = F.GPUSolver2(SoaMesh(standard_geometry_arrays(a, xy_)), p)
+        g.set_state(U0)
+        g.steady_solve(iters, 1e-14)
+        cx, cy = g.surface_force(1.0, sets)
+        al = np.arctan2(p.inf_ruvp[2], p.inf_ruvp[1])
+        out = (-np.sin(al) * cx.sum() + np.cos(al) * cy.sum(), g.get_state(), g)
+        return out
+
+    J0, Ustar, g = J_of(xy, F.field_init(m, p, F.INIT_UNIFORM), 60000)
+    res, dJdX = g.adjoint_solve(sets, 1.0, 60000, 1e-13, coords=True)
+    g.close()
+    assert rel(res.J, J0) < 1e-12
+    ad = np.sum(dJdX * d)
+    h = 1e-4
+    Jp = J_of(xy + h * d, Ustar, 60000)
+    Jm = J_of(xy - h * d, Ustar, 60000)
+    fd = (Jp[0] - Jm[0]) / (2 * h)
+    Jp[2].close(); Jm[2].close()
+    print(f"dJ/dX . d: adjoint {ad:.10f}, central differences {fd:.10f}, rel {rel(ad, fd):.2e}", flush=True)
+    assert rel(ad, fd) < 1e-5
