@@ -274,6 +274,69 @@ def config_c4(ncases=256, steps=500, seed=0):
                                  "value": sub_cells * 3 * sub_steps / wall_sep}}
 
 
+def config_c5(primal_iters=260000, adjoint_iters=150000, fd_iters=140000, h=1e-6):
+    """Discrete adjoint on the NACA 0012 O-grid 512 x 384 x 2 = 393,216 triangles, farfield at 20
+    chords (radial growth 1.01), M 0.5, AoA 2 deg, Euler, Roe, Green-Gauss, linear reconstruction
+    without limiter (smooth subsonic flow), SSP-RK3 pseudo-time with per-cell dt at CFL 2, FP64.
+    J = lift coefficient of the airfoil (pressure force normal to the freestream, chord 1).  The
+    primal is converged from the freestream, the adjoint iterated from psi = 0; dJ/dalpha is
+    checked against central differences (h rad) of J over two more steady solves warm-started from
+    the converged state."""
+    import cfd_solver_2d_gpu_amd as F
+    m = F.Mesh.generate(F.GEN_NACA_OGRID, 512, 384, b=20.0, c=1.01, naca_digits=12)
+    inf = list(F.ruvp_from_mach(0.5, 2.0))
+    p = F.default_params(inf_ruvp=inf, inlet_ruvp=inf, outlet_ruvp=inf, CFL=2.0, flux_scheme=F.SOL_ROE,
+                         limiter=F.LIM_NONE, time_advance=F.EVO_RK3)
+    sets = [np.asarray(e, dtype=np.int32) for (name, t, e) in m.boundary_sets() if name == "airfoil"]
+    g = F.GPUSolver2(m, p)
+    g.set_state(F.field_init(m, p, F.INIT_UNIFORM))
+    alpha = float(np.arctan2(inf[2], inf[1]))
+    speed = float(np.hypot(inf[1], inf[2]))
+
+    def J(a):
+        cx, cy = g.surface_force(1.0, sets)
+        return -np.sin(a) * cx.sum() + np.cos(a) * cy.sum()
+
+    g.synchronize()
+    t0 = time.perf_counter()
+    it_p, r0, r1 = g.steady_solve(primal_iters, 1e-12)
+    g.synchronize()
+    t_primal = time.perf_counter() - t0
+    print(f"[c5] primal: {it_p} iterations, residual {r0:.3e} -> {r1:.3e}, {t_primal:.1f} s", file=sys.stderr, flush=True)
+    J0 = J(alpha)
+    Ustar = g.get_state()
+    t0 = time.perf_counter()
+    res, _ = g.adjoint_solve(sets, 1.0, adjoint_iters, 1e-12)
+    t_adj = time.perf_counter() - t0
+    print(f"[c5] adjoint: {res.iters} iterations, residual {res.res0:.3e} -> {res.res:.3e}, {t_adj:.1f} s",
+          file=sys.stderr, flush=True)
+    Js = []
+    t0 = time.perf_counter()
+    for s in (1, -1):
+        a = alpha + s * h
+        g.set_boundary_states([inf[0], speed * np.cos(a), speed * np.sin(a), inf[3]])
+        g.set_state(Ustar)
+        g.steady_solve(fd_iters, 0.0)
+        Js.append(J(a))
+    t_fd = time.perf_counter() - t0
+    g.close()
+    fd = (Js[0] - Js[1]) / (2 * h)
+    rel = abs(res.dJ_dalpha - fd) / abs(fd)
+    print(f"[c5] dJ/dalpha adjoint {res.dJ_dalpha:.10f}, central differences {fd:.10f}, rel {rel:.2e}",
+          file=sys.stderr, flush=True)
+    nc = m.num_element
+    return {"workload": "NACA 0012 O-grid 512x384x2 = 393,216 triangles, farfield 20c, M 0.5, AoA 2 deg, Euler, "
+                        "Roe, Green-Gauss, no limiter, SSP-RK3 pseudo-time CFL 2 (per-cell dt), FP64",
+            "cells": nc, "J_lift": J0, "dJ_dalpha_adjoint": res.dJ_dalpha, "dJ_dalpha_central_diff": fd,
+            "fd_h_rad": h, "rel_diff": rel,
+            "primal": {"iterations": it_p, "residual_first": r0, "residual_last": r1, "seconds": t_primal,
+                       "ms_per_iteration": t_primal / it_p * 1e3},
+            "adjoint": {"iterations": res.iters, "residual_first": res.res0, "residual_last": res.res,
+                        "seconds": t_adj, "ms_per_iteration": t_adj / res.iters * 1e3,
+                        "cell_products_per_s": nc * 3 * res.iters / t_adj},
+            "fd_solves": {"iterations_each": fd_iters, "seconds": t_fd}}
+
+
 def cpu_baseline(steps):
     """The CPU oracle (single thread, FP64) on the same mesh for a bounded number of steps."""
     import cfd_solver_2d_gpu_amd as F
@@ -297,6 +360,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="FP64 Roe only")
+    ap.add_argument("--no-c5", action="store_true", help="skip the adjoint leg (about two minutes)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (calibrates rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -360,6 +424,8 @@ def main():
         out["c1_vortex"] = config_c1(with_oracle=not args.no_cpu)
         out["c3_ns_cylinder"] = config_c3(2000)
         out["c4_batch"] = config_c4()
+    if not args.no_extra and not args.no_c5:
+        out["c5_adjoint"] = config_c5()
     if not args.no_cpu:
         rate, cw = cpu_baseline(args.cpu_steps)
         out["cpu_baseline"] = {"value": rate, "unit": "cell-RK-stage updates/s", "cores": 1, "kind": "port",

@@ -769,9 +769,21 @@ int steady_rhs(fvmr_solver *h, const S4 *U, S4 *rhs) {
 }
 
 int steady_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, double a, double b, int first, S4 *rhs) {
+    const AdjWork *w = (const AdjWork *)h->adj;
+    if (h->tiles) {   // the tile kernel with per-cell dt (the same residual as the cell kernels)
+        TileArgs<double> t = tile_args<double>(h);
+        t.dtl = w->dtl;
+        if (h->ph.equation == FVMR_EQ_NS)
+            k_tile<double, double, true, true><<<h->ntiles, TileCfg<double>::kThreads, 0, h->stream>>>(
+                t, h->ph, Uin, Un, Uout, rhs, h->sc, h->bad_list, h->nc, a, b, first);
+        else
+            k_tile<double, double, false, true><<<h->ntiles, TileCfg<double>::kThreads, 0, h->stream>>>(
+                t, h->ph, Uin, Un, Uout, rhs, h->sc, h->bad_list, h->nc, a, b, first);
+        HIP_OK(hipGetLastError());
+        return FVMR_OK;
+    }
     int rc = launch_gradient<double, double>(h, Uin, -1);
     if (rc) return rc;
-    const AdjWork *w = (const AdjWork *)h->adj;
     const Mesh<double> m = mesh_args<double>(h);
     if (h->p.equation == FVMR_EQ_NS)
         k_steady_stage<true><<<grid_of(h->nc), kBlock, 0, h->stream>>>(m, h->ph, Uin, Un, Uout, (const Grad<double> *)h->grad,

@@ -959,6 +959,7 @@ struct TileArgs {
     const int4 *face;         // per face: nodes | BC code, L | R | pair face, float length, edge
     const DtRec *dtrec;       // per device cell
     const CaseBC *cbc;        // per case of the batch (one case: the handle's)
+    const double *dtl;        // per-cell time steps of a steady solve (k_tile<..., true>), else NULL
 };
 
 __device__ __forceinline__ int lfield(int w, int k) { return (w >> (fvmr::kLocalBits * k)) & fvmr::kLocalMask; }
@@ -1017,7 +1018,7 @@ struct TileCells {
 };
 
 // One RK stage over the tiles (the work of k_gradient + k_stage, see there).
-template <class C, class G, bool NS>
+template <class C, class G, bool NS, bool LDT = false>
 __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, Phys ph, const S4 *__restrict__ Uin,
                                                         const S4 *__restrict__ Un, S4 *__restrict__ Uout,
                                                         S4 *__restrict__ rhs_out, Scalars *__restrict__ sc,
@@ -1153,10 +1154,11 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
     __syncthreads();
     if (f < nFF) sFl[f] = V4<C>{F[0], F[1], F[2], F[3]};
     __syncthreads();
-    // 5. cell-owned sum in slot order, /(-V), RK update (as k_stage)
-    const double dt = sc[cs].dt;
+    // 5. cell-owned sum in slot order, /(-V), RK update (as k_stage); LDT: the cell's own dt
+    double dt = sc[cs].dt;
     if (i < nT) {
         const int c = c0 + i;
+        if (LDT) dt = ta.dtl[c];
         const int sw[3] = {gs.x, gs.y, gs.z};
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -1605,6 +1607,7 @@ TileArgs<G> tile_args(const fvmr_solver *h) {
     t.info = h->t_info; t.halo = h->t_halo; t.nxy = h->t_nxy; t.lnodes = h->t_lnodes;
     t.gslot = h->t_gslot; t.gcoef = (const V4<G> *)h->t_gcoef; t.face = h->t_face; t.dtrec = h->t_dtrec;
     t.cbc = h->cbc;
+    t.dtl = nullptr;
     return t;
 }
 
