@@ -361,6 +361,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="FP64 Roe only")
     ap.add_argument("--no-c5", action="store_true", help="skip the adjoint leg (about two minutes)")
+    ap.add_argument("--c5-only", action="store_true", help="run only the adjoint leg and print its JSON")
+    ap.add_argument("--c5-iters", type=int, nargs=3, default=[260000, 150000, 140000],
+                    metavar=("PRIMAL", "ADJOINT", "FD"), help="iteration budgets of the adjoint leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (calibrates rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -370,6 +373,9 @@ def main():
         sys.exit(2)
     import cfd_solver_2d_gpu_amd as F
 
+    if args.c5_only:
+        print(json.dumps({"c5_adjoint": config_c5(*args.c5_iters)}), flush=True)
+        return
     if args.copy_probe_mib:
         import torch
         n = args.copy_probe_mib * (1 << 20) // 8
@@ -425,7 +431,7 @@ def main():
         out["c3_ns_cylinder"] = config_c3(2000)
         out["c4_batch"] = config_c4()
     if not args.no_extra and not args.no_c5:
-        out["c5_adjoint"] = config_c5()
+        out["c5_adjoint"] = config_c5(*args.c5_iters)
     if not args.no_cpu:
         rate, cw = cpu_baseline(args.cpu_steps)
         out["cpu_baseline"] = {"value": rate, "unit": "cell-RK-stage updates/s", "cores": 1, "kind": "port",

@@ -322,6 +322,24 @@ class GPUSolver2:
                                        _dptr(cy)), "fvmr_surface_force")
         return cx[:len(sets)], cy[:len(sets)]
 
+    # CudaGrid::CudaGrid's colour map (FVMR/src/gui/cuda/CudaGrid.cpp:33-39): {r, g, b, position}
+    DEFAULT_COLORMAP = [(1.0, 0.0, 0.0, 0.00), (1.0, 1.0, 0.0, 0.25), (0.0, 1.0, 0.0, 0.50),
+                        (0.0, 0.0, 1.0, 0.75), (1.0, 0.0, 1.0, 1.00)]
+
+    def vertex_colors(self, var=0, lo=0.0, hi=1.0, colormap=None, vertices=None):
+        """GPU::setVerticesColor: r, g, b of every node (num_node x 3, float32) for the primitive
+        var of the current state normalised by [lo, hi]; vertices (num_node x 6 float32, x y z r g b)
+        is updated in place when given."""
+        cmap = np.ascontiguousarray(np.asarray(colormap if colormap is not None else self.DEFAULT_COLORMAP,
+                                               dtype=np.float32).reshape(-1, 4))
+        nv = self.mesh.soa.num_node
+        vert = np.zeros((nv, 6), np.float32) if vertices is None else vertices
+        assert vert.dtype == np.float32 and vert.shape == (nv, 6) and vert.flags.c_contiguous
+        fp = C.POINTER(C.c_float)
+        check(lib().fvmr_vertex_colors(self._h, int(var), float(lo), float(hi), cmap.shape[0], cmap.ctypes.data_as(fp),
+                                       vert.ctypes.data_as(fp), 0), "fvmr_vertex_colors")
+        return vert[:, 3:6].copy()
+
     def _sets(self, sets):
         off = np.zeros(len(sets) + 1, dtype=np.int32)
         off[1:] = np.cumsum([len(e) for e in sets])

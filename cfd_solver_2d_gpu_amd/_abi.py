@@ -116,6 +116,8 @@ SIGNATURES = {
     "fvmr_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "fvmr_autotune": (C.c_int, [C.POINTER(MeshSoA), C.POINTER(Params), C.c_int, _ip, _dp]),
     "fvmr_set_boundary_states": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "fvmr_vertex_colors": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_int, C.POINTER(C.c_float),
+                                     C.POINTER(C.c_float), C.c_int]),
     "fvmr_steady_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _ip, _dp]),
     "fvmr_adjoint_vjp": (C.c_int, [C.c_void_p, _dp * 4, _dp * 4, _dp, _dp]),
     "fvmr_adjoint_solve": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, C.c_double, C.c_int, C.c_double, _dp,

@@ -753,6 +753,24 @@ int adj_alloc(fvmr_solver *h) {
 
 int adj_grid(int n) { return (n + kAdjBlock - 1) / kAdjBlock; }
 
+// Per-cell pseudo-time steps of U into the workspace (the tile kernel's form when the handle
+// has tiles; both give the same values).
+int local_dt(fvmr_solver *h, const S4 *U) {
+    AdjWork *w = (AdjWork *)h->adj;
+    if (h->tiles) {
+        TileArgs<double> t = tile_args<double>(h);
+        t.dtl = w->dtl;
+        if (h->ph.equation == FVMR_EQ_NS)
+            k_tile_dt<double, true, true><<<h->ntiles, TileCfg<double>::kThreads, 0, h->stream>>>(t, h->ph, U, nullptr);
+        else
+            k_tile_dt<double, false, true><<<h->ntiles, TileCfg<double>::kThreads, 0, h->stream>>>(t, h->ph, U, nullptr);
+    } else {
+        k_local_dt<<<grid_of(h->nc), kBlock, 0, h->stream>>>(mesh_args<double>(h), h->ph, U, w->dtl);
+    }
+    HIP_OK(hipGetLastError());
+    return FVMR_OK;
+}
+
 // Residual-only evaluation at U into rhs (gradient first).
 int steady_rhs(fvmr_solver *h, const S4 *U, S4 *rhs) {
     int rc = launch_gradient<double, double>(h, U, -1);
@@ -799,7 +817,7 @@ int steady_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, double a
 int field_norm2(fvmr_solver *h, const S4 *f, double out[4]) {
     k_norm<<<grid_of(h->nc), kBlock, 0, h->stream>>>(h->nc, f, FVMR_NORM_2, h->partial);
     HIP_OK(hipGetLastError());
-    k_norm_finish<<<1, 64, 0, h->stream>>>(grid_of(h->nc), FVMR_NORM_2, h->partial, h->norm_out);
+    k_norm_finish<<<1, kBlock, 0, h->stream>>>(grid_of(h->nc), FVMR_NORM_2, h->partial, h->norm_out);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(out, h->norm_out, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
@@ -964,8 +982,7 @@ int fvmr_steady_solve(fvmr_handle *h, int max_iters, double tol, int *iters, dou
         const int a = h->cur, b = (a + 1) % 3, c = (a + 2) % 3;
         S4 *Ua = h->Ubuf[a], *Ub = h->Ubuf[b], *Uc = h->Ubuf[c];
         const bool check = (it % kAdjCheck == 0);
-        k_local_dt<<<grid_of(h->nc), kBlock, 0, h->stream>>>(m, h->ph, Ua, w->dtl);
-        HIP_OK(hipGetLastError());
+        if ((rc = local_dt(h, Ua))) return rc;
         if ((rc = steady_stage(h, Ua, Ua, Ub, 0.0, 1.0, 1, check ? h->rhs : nullptr))) return rc;
         if (h->p.time_advance == FVMR_EVO_RK3) {
             if ((rc = steady_stage(h, Ub, Ua, Uc, 0.75, 0.25, 0, nullptr)) ||

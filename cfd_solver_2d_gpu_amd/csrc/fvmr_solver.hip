@@ -959,7 +959,7 @@ struct TileArgs {
     const int4 *face;         // per face: nodes | BC code, L | R | pair face, float length, edge
     const DtRec *dtrec;       // per device cell
     const CaseBC *cbc;        // per case of the batch (one case: the handle's)
-    const double *dtl;        // per-cell time steps of a steady solve (k_tile<..., true>), else NULL
+    double *dtl;              // per-cell steps of a steady solve (k_tile_dt / k_tile with LDT), else NULL
 };
 
 __device__ __forceinline__ int lfield(int w, int k) { return (w >> (fvmr::kLocalBits * k)) & fvmr::kLocalMask; }
@@ -1197,7 +1197,10 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
 // Block minima of CFL V / alphaC over the tiles (the dt part of k_gradient<DT>): the
 // spectral radius of every face of the tile once (inviscid part, and the viscous part up
 // to the division by the cell volume), then per cell the sum in the reference edge order.
-template <class G, bool NS>
+// LDT: the per-cell pseudo-time step of a steady solve instead, written to ta.dtl (the
+// consistent spectral radius (|u.n| + a) L of the face-mean primitives summed over the cell's
+// three faces, with k_local_dt's operation order).
+template <class G, bool NS, bool LDT = false>
 __global__ __launch_bounds__(TileCfg<G>::kThreads) void k_tile_dt(TileArgs<G> ta, Phys ph, const S4 *__restrict__ U,
                                                            double *__restrict__ partial) {
     if (!NS) ph.equation = FVMR_EQ_EULER;   // lets the compiler drop the viscous terms
@@ -1235,11 +1238,38 @@ __global__ __launch_bounds__(TileCfg<G>::kThreads) void k_tile_dt(TileArgs<G> ta
         ld(sU, has_R ? R : L, uR);
         prim(uR, rR, ph.gamma);
         double av, vv;
-        edge_alpha(ph, rL, rR, has_R, (double)eg.x, (double)eg.y, (double)eg.z, 1.0, &av, &vv);
+        if (LDT) {
+            double r[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) r[k] = has_R ? 0.5 * (rL[k] + rR[k]) : rL[k];
+            const double Lf = (double)eg.z;
+            av = (fabs(r[1] * (double)eg.x + r[2] * (double)eg.y) + sqrt(ph.gamma * r[3] / r[0])) * Lf;
+            vv = 0.0;
+            if (ph.equation == FVMR_EQ_NS) {
+                const double T = r[3] / ph.R / r[0];
+                vv = fmax(4.0 / 3.0, ph.gamma / ph.Pr) * mu_of_T(ph, T) / r[0] * Lf * Lf;
+            }
+        } else {
+            edge_alpha(ph, rL, rR, has_R, (double)eg.x, (double)eg.y, (double)eg.z, 1.0, &av, &vv);
+        }
         sAi[f] = av;
         sAv[f] = vv;
     }
     __syncthreads();
+    if (LDT) {
+        if (tid < nT) {
+            const double vol_c = dr.vol;
+            double alpha_c = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int f = lfield(dr.faces, j);
+                alpha_c += sAi[f];
+                if (ph.equation == FVMR_EQ_NS) alpha_c += sAv[f] / vol_c;
+            }
+            ta.dtl[c0 + tid] = ta.cbc[ti[fvmr::kTiCase]].CFL * vol_c / alpha_c;
+        }
+        return;
+    }
     double dtc = 1e10;
     if (tid < nT) {
         const int i = tid;
@@ -1315,6 +1345,73 @@ __global__ __launch_bounds__(kBlock) void k_node_field(int nv, const int *__rest
     const double V2 = u * u + w * w;
     const double a2 = gamma * p / rho;
     out[(size_t)5 * nv + v] = sqrt(V2 / a2);
+}
+
+// Colour-map control points of the GUI kernels: {r, g, b, position}, positions increasing.
+constexpr int kMaxColorPoints = 64;
+struct ColorMapArg {
+    int n;
+    float4 p[kMaxColorPoints];
+};
+
+// getColor_device (CudaGrid_kernels.cu:22-65): clamp to [0, 1], the interval of the control
+// points that holds the input, the cubic blend of its end colours with their mean as both
+// inner control points; purple (252, 3, 253) / 255 when no interval holds it.
+__device__ __forceinline__ float3 colormap_color(const ColorMapArg &cm, float x) {
+#pragma clang fp contract(off)
+    if (x < 0.0f) x = 0.0f;
+    if (x > 1.0f) x = 1.0f;
+    for (int i = 0; i + 1 < cm.n; ++i) {
+        const float4 a = cm.p[i], b = cm.p[i + 1];
+        if (x >= a.w && x <= b.w) {
+            const float mx = 0.5f * (a.x + b.x), my = 0.5f * (a.y + b.y), mz = 0.5f * (a.z + b.z);
+            const float n = (x - a.w) / (b.w - a.w);
+            const float omn = 1 - n, omn2 = omn * omn, omn3 = omn2 * omn, n2 = n * n, n3 = n2 * n;
+            const float w1 = 3 * omn2 * n, w2 = 3 * omn * n2;
+            return float3{omn3 * a.x + w1 * mx + w2 * mx + n3 * b.x, omn3 * a.y + w1 * my + w2 * my + n3 * b.y,
+                          omn3 * a.z + w1 * mz + w2 * mz + n3 * b.z};
+        }
+    }
+    return float3{252.0f / 255.0f, 3.0f / 255.0f, 253.0f / 255.0f};
+}
+
+// GPU::setVerticesColor (CudaGrid_kernels.cu:185-232): per node the largest of q_c * n over its
+// n cells, from 0 (accumulateElementDataToNodeData_k, :77-95, without its lost updates), times
+// (float)(1.0 / n) (divideNodeData_k, :97-104), normalised by [lo, hi] and coloured
+// (nodeDataToVertexColor_k, :106-123) into vertices[6 v + 3 .. 5].  q = primitive var of U as
+// float (the reference renders its float ruvp).
+__global__ __launch_bounds__(kBlock) void k_vertex_colors(int nv, const int *__restrict__ off,
+                                                           const int *__restrict__ cells, const S4 *__restrict__ U,
+                                                           double gamma, int var, float lo, float hi, ColorMapArg cm,
+                                                           float *__restrict__ vertices) {
+#pragma clang fp contract(off)
+    const int v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= nv) return;
+    const int k0 = off[v], k1 = off[v + 1], cnt = k1 - k0;
+    float node = 0.0f;
+    for (int k = k0; k < k1; k++) {
+        const S4 u = U[cells[k]];
+        double q;
+        if (var == 0) q = u.x;
+        else if (var == 1) q = u.y / u.x;
+        else if (var == 2) q = u.z / u.x;
+        else {
+            const double uu = u.y / u.x, vv = u.z / u.x;
+            q = u.x * (gamma - 1) * (u.w / u.x - (uu * uu + vv * vv) * 0.5);
+        }
+        float in = (float)q;
+        in *= (float)cnt;
+        node = (node > in) ? node : in;
+    }
+    float div = (float)(1.0 / cnt);
+    if (div > 1e5f) div = 1.0f;
+    node *= div;
+    const float len = fabsf(hi - lo);
+    const float x = (len == 0.0f) ? 0.0f : (node - lo) / len;
+    const float3 c = colormap_color(cm, x);
+    vertices[(size_t)6 * v + 3] = c.x;
+    vertices[(size_t)6 * v + 4] = c.y;
+    vertices[(size_t)6 * v + 5] = c.z;
 }
 
 // ReferenceData::calculate_edgeSet_force (FieldWriter.cpp:174-205): one thread per set, its
@@ -1447,16 +1544,26 @@ __global__ __launch_bounds__(kBlock) void k_norm(int nc, const S4 *__restrict__ 
         partial[4 * blk + k] = b;
     }
 }
-__global__ __launch_bounds__(64) void k_norm_finish(int nparts, int norm, const double *__restrict__ partial,
-                                                    double *__restrict__ out) {
-    if (threadIdx.x < 4) {
-        const int k = threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_norm_finish(int nparts, int norm, const double *__restrict__ partial,
+                                                        double *__restrict__ out) {
+    // per variable: thread t combines partials t, t + kBlock, ... in order, then a fixed-shape
+    // tree over the threads (the same order on every run)
+    __shared__ double s[kBlock];
+    for (int k = 0; k < 4; k++) {
         double b = 0.0;
-        for (int i = 0; i < nparts; i++) {
+        for (int i = threadIdx.x; i < nparts; i += kBlock) {
             const double v = partial[4 * i + k];
             b = (norm == FVMR_NORM_INF) ? fmax(b, v) : b + v;
         }
-        out[k] = (norm == FVMR_NORM_2) ? sqrt(b) : b;
+        s[threadIdx.x] = b;
+        __syncthreads();
+        for (int w = kBlock / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) s[threadIdx.x] = (norm == FVMR_NORM_INF) ? fmax(s[threadIdx.x], s[threadIdx.x + w])
+                                                                          : s[threadIdx.x] + s[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[k] = (norm == FVMR_NORM_2) ? sqrt(s[0]) : s[0];
+        __syncthreads();
     }
 }
 
@@ -2456,7 +2563,7 @@ int fvmr_residual(fvmr_handle *h, int norm, double out[4]) {
     if (rc) return rc;
     k_norm<<<grid_of(h->nc), kBlock, 0, h->stream>>>(h->nc, h->rhs, norm, h->partial);
     HIP_OK(hipGetLastError());
-    k_norm_finish<<<1, 64, 0, h->stream>>>(grid_of(h->nc), norm, h->partial, h->norm_out);
+    k_norm_finish<<<1, kBlock, 0, h->stream>>>(grid_of(h->nc), norm, h->partial, h->norm_out);
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 4, e0))) return rc;
     HIP_OK(hipMemcpyAsync(out, h->norm_out, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -2496,6 +2603,33 @@ int fvmr_node_field(fvmr_handle *h, double *const ruvp[4], double *Cp, double *M
             if (ruvp[k]) std::copy(buf.begin() + k * nv, buf.begin() + (k + 1) * nv, ruvp[k]);
     if (Cp) std::copy(buf.begin() + 4 * nv, buf.begin() + 5 * nv, Cp);
     if (Ma) std::copy(buf.begin() + 5 * nv, buf.begin() + 6 * nv, Ma);
+    return FVMR_OK;
+}
+
+int fvmr_vertex_colors(fvmr_handle *h, int var, float lo, float hi, int npts, const float *colormap, float *vertices,
+                       int on_device) {
+    if (!h || !colormap || !vertices || var < 0 || var > 3 || npts < 1 || npts > kMaxColorPoints)
+        return set_error(FVMR_ERR_INVALID_ARG, "invalid argument");
+    ColorMapArg cm{};
+    cm.n = npts;
+    for (int i = 0; i < npts; i++)
+        cm.p[i] = make_float4(colormap[4 * i], colormap[4 * i + 1], colormap[4 * i + 2], colormap[4 * i + 3]);
+    const size_t nv = h->nv;
+    float *dv = vertices;
+    if (!on_device) {
+        if (hipMalloc((void **)&dv, 6 * nv * sizeof(float)) != hipSuccess) return set_error(FVMR_ERR_HIP, "hipMalloc vertices");
+        if (hipMemcpyAsync(dv, vertices, 6 * nv * sizeof(float), hipMemcpyHostToDevice, h->stream) != hipSuccess) {
+            (void)hipFree(dv);
+            return set_error(FVMR_ERR_HIP, "vertex upload");
+        }
+    }
+    k_vertex_colors<<<grid_of((int)nv), kBlock, 0, h->stream>>>((int)nv, h->nc_off, h->nc_cells, h->Ubuf[h->cur],
+                                                               h->p.gamma, var, lo, hi, cm, dv);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && !on_device) e = hipMemcpyAsync(vertices, dv, 6 * nv * sizeof(float), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (!on_device) (void)hipFree(dv);
+    if (e != hipSuccess) return set_error(FVMR_ERR_HIP, std::string("fvmr_vertex_colors: ") + hipGetErrorString(e));
     return FVMR_OK;
 }
 

@@ -304,6 +304,18 @@ int fvmr_node_field(fvmr_handle *h, double *const ruvp[4], double *Cp, double *M
 int fvmr_surface_force(fvmr_handle *h, int nsets, const int *set_offsets, const int *set_edges, double S_ref,
                        double *Cx, double *Cy);
 
+/* GUI vertex colours of a cell quantity (GPU::setVerticesColor, CudaGrid_kernels.cu:185-232, with
+ * getColor_device :22-65 and the kernels :77-123): per node the largest of q_c * n over its n
+ * cells (from 0) times (float)(1.0 / n), q = primitive var of the current state as float
+ * (0 rho, 1 u, 2 v, 3 p); normalised by [lo, hi] (the reference's render range_1, range_2) and
+ * coloured from npts control points colormap[4 i .. 4 i + 3] = {r, g, b, position} (positions
+ * increasing; at most 64) by the reference's cubic blend; written as r, g, b to
+ * vertices[6 v + 3 .. 6 v + 5] (the x, y, z, r, g, b vertex buffer of the renderer; x, y, z are
+ * left as they are).  on_device = 1: vertices is device memory (a mapped GL buffer), 0: host
+ * memory. */
+int fvmr_vertex_colors(fvmr_handle *h, int var, float lo, float hi, int npts, const float *colormap, float *vertices,
+                       int on_device);
+
 /* ---------------------------------------------------------------------- */
 /* steady states and the discrete adjoint (no reference counterpart)       */
 /* ---------------------------------------------------------------------- */

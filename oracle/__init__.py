@@ -50,6 +50,8 @@ _SIG = {
     "orc_inv2x2": (None, [_dp, C.c_int]),
     "orc_node_field": (None, [C.POINTER(MeshSoA), _dp * 4, C.c_double, _dp, _dp * 4, _dp, _dp]),
     "orc_surface_force": (None, [C.POINTER(MeshSoA), _dp, C.c_int, _ip, _ip, C.c_double, _dp, _dp]),
+    "orc_vertex_colors": (None, [C.POINTER(MeshSoA), _dp * 4, C.c_double, C.c_int, C.c_float, C.c_float, C.c_int,
+                                 C.POINTER(C.c_float), C.POINTER(C.c_float)]),
 }
 _lib = None
 
@@ -301,3 +303,14 @@ def surface_force(soa, Cp, sets, S_ref):
     lib().orc_surface_force(C.byref(soa), Cp.ctypes.data_as(_dp), len(sets), off.ctypes.data_as(_ip),
                             edges.ctypes.data_as(_ip), S_ref, cd.ctypes.data_as(_dp), cl.ctypes.data_as(_dp))
     return cd, cl
+
+
+def vertex_colors(soa, U, gamma, var, lo, hi, colormap):
+    """r, g, b per node (num_node x 3, float32) as GPU::setVerticesColor forms them."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    cmap = np.ascontiguousarray(np.asarray(colormap, dtype=np.float32).reshape(-1, 4))
+    vert = np.zeros((soa.num_node, 6), np.float32)
+    fp = C.POINTER(C.c_float)
+    lib().orc_vertex_colors(C.byref(soa), (_dp * 4)(*[U[k].ctypes.data_as(_dp) for k in range(4)]), gamma, int(var),
+                            float(lo), float(hi), cmap.shape[0], cmap.ctypes.data_as(fp), vert.ctypes.data_as(fp))
+    return vert[:, 3:6].copy()

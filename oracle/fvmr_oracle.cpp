@@ -1237,6 +1237,64 @@ void orc_node_field(const fvmr_mesh_soa *m, const double *const *U, double gamma
     }
 }
 
+/* getColor_device (FVMR/src/gui/cuda/CudaGrid_kernels.cu:22-65): input clamped to [0, 1]; the
+ * first control interval [w_i, w_i+1] holding it; with n its position in the interval and
+ * m = 0.5 (c_i + c_i+1), colour = (1-n)^3 c_i + 3 (1-n)^2 n m + 3 (1-n) n^2 m + n^3 c_i+1 in that
+ * order; purple (252, 3, 253) / 255 when no interval holds it.  cmap: {r, g, b, position}. */
+static void color_of(int npts, const float *cmap, float x, float out[3]) {
+    if (x < 0.0f) x = 0.0f;
+    if (x > 1.0f) x = 1.0f;
+    for (int i = 0; i + 1 < npts; ++i) {
+        const float *a = cmap + 4 * i, *b = cmap + 4 * (i + 1);
+        if (x >= a[3] && x <= b[3]) {
+            const float n = (x - a[3]) / (b[3] - a[3]);
+            const float omn = 1 - n, omn2 = omn * omn, omn3 = omn2 * omn, n2 = n * n, n3 = n2 * n;
+            for (int k = 0; k < 3; k++) {
+                const float mean = 0.5f * (a[k] + b[k]);
+                out[k] = omn3 * a[k] + 3 * omn2 * n * mean + 3 * omn * n2 * mean + n3 * b[k];
+            }
+            return;
+        }
+    }
+    out[0] = 252.0f / 255.0f;
+    out[1] = 3.0f / 255.0f;
+    out[2] = 253.0f / 255.0f;
+}
+
+/* GPU::setVerticesColor (FVMR/src/gui/cuda/CudaGrid_kernels.cu:185-232): node data zeroed; per
+ * element and vertex, node = max(node, q_e * n_v) with q_e the float primitive `var` and n_v the
+ * node's element count (accumulateElementDataToNodeData_k :77-95, GPUSolver2.cpp:185-196); node *=
+ * (float)(1.0 / n_v), 1 when that exceeds 1e5 (divideNodeData_k :97-104); x = (node - lo) /
+ * |hi - lo| (0 for an empty range) coloured into vertices[6 v + 3 .. 5] (nodeDataToVertexColor_k
+ * :106-123). */
+void orc_vertex_colors(const fvmr_mesh_soa *m, const double *const *U, double gamma, int var, float lo, float hi,
+                       int npts, const float *cmap, float *vertices) {
+    const int nv = m->num_node, ne = m->num_element;
+    std::vector<int> cnt(nv, 0);
+    for (int ie = 0; ie < ne; ie++)
+        for (int j = 0; j < 3; j++) cnt[m->element_nodes[j][ie]]++;
+    std::vector<float> node(nv, 0.0f);
+    for (int ie = 0; ie < ne; ie++) {
+        double Ue[4] = {U[0][ie], U[1][ie], U[2][ie], U[3][ie]}, r[4];
+        U2ruvp(Ue, r, gamma);
+        for (int j = 0; j < 3; j++) {
+            const int v = m->element_nodes[j][ie];
+            float in = (float)r[var];
+            in *= cnt[v];
+            const float value = node[v];
+            node[v] = (value > in) ? value : in;
+        }
+    }
+    for (int v = 0; v < nv; v++) {
+        float div = 1.0 / cnt[v];
+        if (div > 1e5) div = 1.0f;
+        node[v] *= div;
+        const float len = std::fabs(hi - lo);
+        const float x = (len == 0) ? 0.0f : (node[v] - lo) / len;
+        color_of(npts, cmap, x, vertices + (size_t)6 * v + 3);
+    }
+}
+
 /* FieldWriter::ReferenceData::calculate_edgeSet_force (FVMR/src/output/FieldWriter.cpp:174-205):
  * per boundary set, over its edges in set order, Cp_e = 0.5 (Cp_0 + Cp_1),
  * dC = Cp_e (1/S_ref) L; CD += dC nx; CL += dC ny.  Set s lists edges off[s] .. off[s+1]-1. */

@@ -91,3 +91,24 @@ def test_checkpoint_and_restart_through_a_continue_file(golden_dir, tmp_path):
     assert t3 == tg
     assert np.array_equal(g2.get_state(), straight)
     g2.close()
+
+
+def test_vertex_colors_match_oracle(golden_dir):
+    announce("GUI vertex colours of rho, u, v, p vs the oracle, cylinder continue-file state")
+    m, p, U = cylinder_case(golden_dir)
+    g = F.GPUSolver2(m, p)
+    g.set_state(U)
+    ranges = [(0.0, 2.0), (-0.2, 0.3), (-0.1, 0.1), (0.5, 1.0)]
+    cmap = F.GPUSolver2.DEFAULT_COLORMAP
+    for var, (lo, hi) in enumerate(ranges):
+        # primitives are formed as the reference's U2ruvp; scale the range to the state's units
+        R, _, _ = O.node_field(m.soa, U, p.gamma, p.inf_ruvp[:])
+        span = float(np.max(np.abs(R[var]))) or 1.0
+        lo_, hi_ = lo * span, hi * span
+        vert = np.zeros((m.soa.num_node, 6), np.float32)
+        vert[:, :3] = 7.0
+        dev = g.vertex_colors(var, lo_, hi_, cmap, vertices=vert)
+        ref = O.vertex_colors(m.soa, U, p.gamma, var, lo_, hi_, cmap)
+        assert np.array_equal(dev, ref), var
+        assert np.all(vert[:, :3] == 7.0)                  # x, y, z untouched
+    g.close()

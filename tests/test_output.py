@@ -59,3 +59,38 @@ def test_surface_force_closed_forms():
     # uniform Cp on a closed boundary: no force
     cd, cl = O.surface_force(m.soa, np.full(len(x), 0.7), sets, 1.0)
     assert np.all(np.abs(cd) < 1e-13) and np.all(np.abs(cl) < 1e-13)
+
+
+CMAP = F.GPUSolver2.DEFAULT_COLORMAP      # CudaGrid.cpp:33-39: red, yellow, green, blue, magenta
+
+
+def test_vertex_colors_closed_forms():
+    """The oracle's GPU::setVerticesColor: control points give their colours, midpoints of an
+    interval the mean of its ends, out-of-range values clamp to the end colours, inputs outside
+    the control points purple, and a node takes the largest value of its cells."""
+    m = F.Mesh.generate(F.GEN_CYLINDER_OGRID, 24, 8, a=0.5, b=4.0, c=1.1)
+    nv, ne = m.soa.num_node, m.num_element
+    a = m.arrays()
+    cm = np.array(CMAP, dtype=np.float32)
+
+    def colors(rho, lo=0.0, hi=1.0, cmap=cm):
+        U = np.stack([np.full(ne, rho) if np.isscalar(rho) else rho, np.zeros(ne), np.zeros(ne), np.full(ne, 2.5)])
+        return O.vertex_colors(m.soa, U, 1.4, 0, lo, hi, cmap)
+
+    for k, (r, g, b, w) in enumerate(CMAP):       # each control point, up to the rounding of rho n (1/n)
+        assert np.allclose(colors(w if w > 0 else 0.0), [r, g, b], atol=2e-6), k
+    mid = colors(0.375)                              # halfway from yellow to green
+    assert np.allclose(mid, [0.5, 1.0, 0.0], atol=1e-6)
+    assert np.array_equal(colors(7.0), np.tile(cm[-1, :3], (nv, 1)))     # clamped high: magenta
+    assert np.array_equal(colors(0.5, lo=2.0, hi=3.0), np.tile(cm[0, :3], (nv, 1)))   # clamped low: red
+    assert np.array_equal(colors(0.0, lo=0.0, hi=0.0), np.tile(cm[0, :3], (nv, 1)))   # empty range -> 0
+    gap = np.array([[1, 0, 0, 0.2], [0, 0, 1, 0.8]], dtype=np.float32)
+    purple = np.array([252, 3, 253], dtype=np.float32) / np.float32(255)
+    assert np.array_equal(colors(0.1, cmap=gap), np.tile(purple, (nv, 1)))
+    # one cell at 1 over 0.5 elsewhere: its three nodes magenta, the rest green
+    rho = np.full(ne, 0.5)
+    rho[5] = 1.0
+    c = colors(rho)
+    hot = set(a["element_nodes"][:3, 5].tolist())
+    for v in range(nv):
+        assert np.allclose(c[v], [1, 0, 1] if v in hot else [0, 1, 0], atol=2e-6), v
