@@ -188,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void k_steady_stage(Mesh<double> m, Phys ph
         acc += f[0][k];
         acc += f[1][k];
         acc += f[2][k];
-        r[k] = acc / w;
+        r[k] = fdiv(acc, w);
     }
     if (rhs_out) st(rhs_out, c, r);
     if (!Uout) return;

