@@ -1520,18 +1520,19 @@ __global__ __launch_bounds__(1024) void k_dt_finish(int nparts, const double *__
 
 // k_dt_finish per case of a batch: block c takes the minimum over the partials of its tiles
 // tile_begin[c] .. tile_begin[c+1]-1 and clamps / advances the times of sc[c].
-__global__ __launch_bounds__(256) void k_dt_finish_cases(const int *__restrict__ tile_begin,
-                                                         const double *__restrict__ partial,
-                                                         Scalars *__restrict__ sc, int advance) {
-    __shared__ double s[4];
+__global__ __launch_bounds__(1024) void k_dt_finish_cases(const int *__restrict__ tile_begin,
+                                                          const double *__restrict__ partial,
+                                                          Scalars *__restrict__ sc, int advance) {
+    __shared__ double s[16];
     const int c = blockIdx.x;
     double b = 1e10;
-    for (int i = tile_begin[c] + (int)threadIdx.x; i < tile_begin[c + 1]; i += 256) b = fmin(b, partial[i]);
+    for (int i = tile_begin[c] + (int)threadIdx.x; i < tile_begin[c + 1]; i += 1024) b = fmin(b, partial[i]);
     b = wave_min(b);
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = b;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double d = fmin(fmin(s[0], s[1]), fmin(s[2], s[3]));
+        double d = s[0];
+        for (int i = 1; i < 16; i++) d = fmin(d, s[i]);
         if (advance) {
             const double t = sc[c].t;
             if (t + d > sc[c].T) d = sc[c].T - t;
@@ -1828,7 +1829,7 @@ int launch_tile_dt(fvmr_solver *h, const S4 *U, int dt_mode) {
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 1, e0))) return rc;
     if ((rc = prof_begin(h, 2, &e0))) return rc;
-    k_dt_finish_cases<<<h->ncases, 256, 0, h->stream>>>(h->tile_case_begin, h->partial, h->sc, dt_mode);
+    k_dt_finish_cases<<<h->ncases, 1024, 0, h->stream>>>(h->tile_case_begin, h->partial, h->sc, dt_mode);
     HIP_OK(hipGetLastError());
     return prof_end(h, 2, e0);
 }
