@@ -36,19 +36,13 @@ using fvmr::Dual;
 constexpr int kAdjBlock = 128;
 constexpr int kAdjCheck = 50;       // iterations between residual checks
 constexpr int kXRec = 18;           // node entries per cell of the coordinate product
-#ifndef FVMR_ADJ_NT
-#define FVMR_ADJ_NT 4               // derivative directions per pass of the product kernel
-#endif
-#ifndef FVMR_ADJ_WAVES
-#define FVMR_ADJ_WAVES 1            // minimum waves per SIMD the product kernel is compiled for
-#endif
 
 constexpr int kBcFlagPeriodic = 1, kBcFlagDsr = 2, kBcFlagIsothermal = 4;
 
 // Device workspace of the steady solve and the adjoint (allocated on first use).
 struct AdjWork {
     S4 *psi[4] = {nullptr, nullptr, nullptr, nullptr};   // 0, 1 iterates; 2, 3 stages
-    S4 *base = nullptr, *wedge = nullptr, *g = nullptr, *res = nullptr;
+    S4 *base = nullptr, *wedge = nullptr, *fadj = nullptr, *g = nullptr, *res = nullptr;
     double *dtl = nullptr, *wnode = nullptr, *partial = nullptr;
     int2 *enode = nullptr;
     Grad<double> *gbar = nullptr;
@@ -61,8 +55,8 @@ struct AdjWork {
 void adj_free(fvmr_solver *h) {
     AdjWork *w = (AdjWork *)h->adj;
     if (!w) return;
-    void *ptrs[] = {w->psi[0], w->psi[1], w->psi[2], w->psi[3], w->base, w->wedge, w->g, w->res, w->dtl, w->wnode,
-                    w->partial, w->enode, w->gbar, w->xval, w->xnode};
+    void *ptrs[] = {w->psi[0], w->psi[1], w->psi[2], w->psi[3], w->base, w->wedge, w->fadj, w->g, w->res, w->dtl,
+                    w->wnode, w->partial, w->enode, w->gbar, w->xval, w->xnode};
     for (void *q : ptrs) if (q) (void)hipFree(q);
     delete w;
     h->adj = nullptr;
@@ -204,108 +198,383 @@ __global__ __launch_bounds__(kBlock) void k_steady_stage(Mesh<double> m, Phys ph
     st(Uout, c, Unew);
 }
 
-// Gradient slots of grad_limit with state derivatives in NT of the four directions
-// (0 = the cell's own state, 1 + i = the state across slot i; every variable at once, as the
-// variables do not mix).
-template <int NT>
-struct SlotsU {
-    double a[3][3];
-    Dual<NT> uo[3][4];
-    double n[3][2];
-    __device__ __forceinline__ Dual<NT> A(int i, int q) const { return Dual<NT>(a[i][q]); }
-    __device__ __forceinline__ Dual<NT> Uo(int i, int k) const { return uo[i][k]; }
-    __device__ __forceinline__ Dual<NT> N(int i, int q) const { return Dual<NT>(n[i][q]); }
+// ---------------------------------------------------------------------------
+// Reverse mode of the face physics (double values).  Each function recomputes the forward
+// values of its template counterpart and accumulates the adjoints of its inputs from the
+// adjoints of its outputs, statement by statement in reverse order; |x| takes the sign the
+// forward-mode fabs uses (+1 at 0), branches follow the values.
+// ---------------------------------------------------------------------------
+// Adjoints of a FaceState's primitives (set_prim's outputs).
+struct PrimBar {
+    double r, u, v, p, ir;
 };
 
-// x with derivative 1 in direction dir if dir is one of p0 .. p0+NT-1
-template <int NT>
-__device__ __forceinline__ Dual<NT> seed_in(double x, int dir, int p0) {
-    Dual<NT> r(x);
-#pragma unroll
-    for (int t = 0; t < NT; t++) r.d[t] = (dir == p0 + t) ? 1.0 : 0.0;
-    return r;
+__device__ __forceinline__ double sgn1(double x) { return x < 0.0 ? -1.0 : 1.0; }
+
+// set_prim: ir = 1/U0, r = U0, u = U1 ir, v = U2 ir, p = r (gamma-1) (U3 ir - (u u + v v) / 2)
+__device__ __forceinline__ void set_prim_adj(const FaceState<double> &s, double gamma, PrimBar b, double Ub[4]) {
+    const double ga1 = gamma - 1.0;
+    const double q = s.U[3] * s.ir - (s.u * s.u + s.v * s.v) * 0.5;
+    b.r += b.p * ga1 * q;
+    const double qb = b.p * s.r * ga1;
+    Ub[3] += qb * s.ir;
+    b.ir += qb * s.U[3];
+    b.u -= qb * s.u;
+    b.v -= qb * s.v;
+    Ub[1] += b.u * s.ir;
+    b.ir += b.u * s.U[1];
+    Ub[2] += b.v * s.ir;
+    b.ir += b.v * s.U[2];
+    Ub[0] += b.r - b.ir * s.ir * s.ir;
 }
 
-// The product with (dr/dU)^T, part 1 (one thread per cell c): base[c] = the terms that land on
-// U_c, wedge[2 e + side] = the terms c sends to the cell across its edge e (side 1 when c is
-// the edge's R).  With gbar, also the adjoint of c's limited gradient.  The four derivative
-// directions of each local Jacobian are taken NT at a time (Dual<NT>, the values recomputed
-// per pass), which bounds the registers of the flux and limiter templates.
-template <bool GBAR, int NT>
-__global__ __launch_bounds__(kAdjBlock, FVMR_ADJ_WAVES) void k_adj_vjp(Mesh<double> m, Phys ph, const S4 *__restrict__ U,
+// flux_roe, reverse: mu = adjoint of the four flux components
+__device__ void roe_adj(const FaceState<double> &L, const FaceState<double> &R, double nx, double ny, double area,
+                        double gamma, double g_ga1, double inv_g, const double mu[4], PrimBar &Lb, PrimBar &Rb) {
+    const double rL = L.r, uL = L.u, vL = L.v, pL = L.p, rR = R.r, uR = R.u, vR = R.v, pR = R.p;
+    const double ul = nx * uL + ny * vL, ur = nx * uR + ny * vR;
+    const double rul = rL * ul, rur = rR * ur;
+    const double V2L = uL * uL + vL * vL, V2R = uR * uR + vR * vR;
+    const double ga1 = gamma - 1.0;
+    const double HL = pL * g_ga1 * L.ir + 0.5 * V2L, HR = pR * g_ga1 * R.ir + 0.5 * V2R;
+    const double rrorl = sqrt(rR * L.ir);
+    const double inv_p1 = 1.0 / (rrorl + 1.0);
+    const double rm = rL * rrorl;
+    const double um = (uL + uR * rrorl) * inv_p1, vm = (vL + vR * rrorl) * inv_p1, Hm = (HL + HR * rrorl) * inv_p1;
+    const double Vm2 = um * um + vm * vm;
+    const double am = sqrt(ga1 * (Hm - 0.5 * Vm2));
+    const double inv_am = 1.0 / am;
+    const double unm = um * nx + vm * ny;
+    const double eig1 = fabs(unm), eig2 = fabs(unm + am), eig3 = fabs(unm - am);
+    const double ER = HR - pR * R.ir, EL = HL - pL * L.ir;
+    const double Etm = Hm * inv_g + ga1 * inv_g * 0.5 * Vm2;
+    const double dW0 = rR - rL;
+    const double dW1 = rm * (uR - uL) + dW0 * um, dW2 = rm * (vR - vL) + dW0 * vm, dW3 = rm * (ER - EL) + dW0 * Etm;
+    const double astar = (eig2 + eig3) * 0.5;
+    const double Mstar = (eig2 - eig3) * 0.5 * inv_am;
+    const double dun = nx * (uR - uL) + ny * (vR - vL);
+    const double dp = pR - pL;
+    const double B = (astar - eig1) * dp / rm, Cq = inv_am * inv_am;
+    const double dUroe = Mstar * dun + B * Cq;
+    const double dProe = Mstar * dp + (astar - eig1) * rm * dun;
+    // central part 0.5 (F_L + F_R) area
+    const double h = 0.5 * area;
+    const double rulb = h * (mu[0] + mu[1] * uL + mu[2] * vL + mu[3] * HL);
+    const double rurb = h * (mu[0] + mu[1] * uR + mu[2] * vR + mu[3] * HR);
+    double uLb = h * mu[1] * rul, vLb = h * mu[2] * rul, HLb = h * mu[3] * rul;
+    double uRb = h * mu[1] * rur, vRb = h * mu[2] * rur, HRb = h * mu[3] * rur;
+    const double psb = h * (mu[1] * nx + mu[2] * ny);
+    double pLb = psb, pRb = psb, rLb = 0.0, rRb = 0.0, irLb = 0.0, irRb = 0.0;
+    // dissipation -0.5 (...) area
+    const double m0 = -h * mu[0], m1 = -h * mu[1], m2 = -h * mu[2], m3 = -h * mu[3];
+    double eig1b = m0 * dW0 + m1 * dW1 + m2 * dW2 + m3 * dW3;
+    double dW0b = m0 * eig1;
+    const double dW1b = m1 * eig1, dW2b = m2 * eig1, dW3b = m3 * eig1;
+    const double dUroeb = (m0 + m1 * um + m2 * vm + m3 * Hm) * rm;
+    double rmb = (m0 + m1 * um + m2 * vm + m3 * Hm) * dUroe;
+    double umb = m1 * dUroe * rm, vmb = m2 * dUroe * rm, Hmb = m3 * dUroe * rm;
+    const double dProeb = m1 * nx + m2 * ny + m3 * unm;
+    double unmb = m3 * dProe;
+    // dProe = Mstar dp + (astar - eig1) rm dun
+    double Mstarb = dProeb * dp, dpb = dProeb * Mstar, astarb = dProeb * rm * dun;
+    eig1b -= dProeb * rm * dun;
+    rmb += dProeb * (astar - eig1) * dun;
+    double dunb = dProeb * (astar - eig1) * rm;
+    // dUroe = Mstar dun + B Cq, B = (astar - eig1) dp / rm, Cq = inv_am^2
+    Mstarb += dUroeb * dun;
+    dunb += dUroeb * Mstar;
+    const double Bb = dUroeb * Cq;
+    double inv_amb = dUroeb * B * 2.0 * inv_am;
+    const double Ab = Bb / rm;
+    rmb -= Bb * B / rm;
+    astarb += Ab * dp;
+    eig1b -= Ab * dp;
+    dpb += Ab * (astar - eig1);
+    pRb += dpb;
+    pLb -= dpb;
+    uRb += dunb * nx; uLb -= dunb * nx; vRb += dunb * ny; vLb -= dunb * ny;
+    // Mstar, astar
+    const double eig2b = 0.5 * inv_am * Mstarb + 0.5 * astarb;
+    const double eig3b = -0.5 * inv_am * Mstarb + 0.5 * astarb;
+    inv_amb += Mstarb * (eig2 - eig3) * 0.5;
+    // dW3, dW2, dW1, dW0
+    rmb += dW3b * (ER - EL) + dW2b * (vR - vL) + dW1b * (uR - uL);
+    const double ERb = dW3b * rm, ELb = -dW3b * rm;
+    dW0b += dW3b * Etm + dW2b * vm + dW1b * um;
+    const double Etmb = dW3b * dW0;
+    vRb += dW2b * rm; vLb -= dW2b * rm; vmb += dW2b * dW0;
+    uRb += dW1b * rm; uLb -= dW1b * rm; umb += dW1b * dW0;
+    rRb += dW0b;
+    rLb -= dW0b;
+    // Etm, ER, EL
+    Hmb += Etmb * inv_g;
+    double Vm2b = Etmb * ga1 * inv_g * 0.5;
+    HRb += ERb; pRb -= ERb * R.ir; irRb -= ERb * pR;
+    HLb += ELb; pLb -= ELb * L.ir; irLb -= ELb * pL;
+    // eigenvalues, unm
+    unmb += eig1b * sgn1(unm) + eig2b * sgn1(unm + am) + eig3b * sgn1(unm - am);
+    double amb = eig2b * sgn1(unm + am) - eig3b * sgn1(unm - am);
+    umb += unmb * nx;
+    vmb += unmb * ny;
+    // inv_am, am = sqrt(ga1 (Hm - Vm2 / 2))
+    amb -= inv_amb * inv_am * inv_am;
+    const double am2b = amb * 0.5 / am;
+    Hmb += am2b * ga1;
+    Vm2b -= am2b * ga1 * 0.5;
+    umb += Vm2b * 2.0 * um;
+    vmb += Vm2b * 2.0 * vm;
+    // Roe averages
+    double rrorlb = 0.0, inv_p1b = 0.0;
+    uLb += umb * inv_p1; uRb += umb * rrorl * inv_p1; rrorlb += umb * uR * inv_p1; inv_p1b += umb * (uL + uR * rrorl);
+    vLb += vmb * inv_p1; vRb += vmb * rrorl * inv_p1; rrorlb += vmb * vR * inv_p1; inv_p1b += vmb * (vL + vR * rrorl);
+    HLb += Hmb * inv_p1; HRb += Hmb * rrorl * inv_p1; rrorlb += Hmb * HR * inv_p1; inv_p1b += Hmb * (HL + HR * rrorl);
+    rLb += rmb * rrorl;
+    rrorlb += rmb * rL;
+    rrorlb -= inv_p1b * inv_p1 * inv_p1;
+    const double sb = rrorlb * 0.5 / rrorl;
+    rRb += sb * L.ir;
+    irLb += sb * rR;
+    // total enthalpies
+    pLb += HLb * g_ga1 * L.ir; irLb += HLb * pL * g_ga1;
+    pRb += HRb * g_ga1 * R.ir; irRb += HRb * pR * g_ga1;
+    uLb += HLb * uL; vLb += HLb * vL;
+    uRb += HRb * uR; vRb += HRb * vR;
+    // mass fluxes
+    rLb += rulb * ul;
+    rRb += rurb * ur;
+    const double ulb = rulb * rL, urb = rurb * rR;
+    uLb += ulb * nx; vLb += ulb * ny;
+    uRb += urb * nx; vRb += urb * ny;
+    Lb.r += rLb; Lb.u += uLb; Lb.v += vLb; Lb.p += pLb; Lb.ir += irLb;
+    Rb.r += rRb; Rb.u += uRb; Rb.v += vRb; Rb.p += pRb; Rb.ir += irRb;
+}
+
+// flux_llf, reverse; the dissipation's conserved jump goes to ULc / URc directly
+__device__ void llf_adj(const FaceState<double> &L, const FaceState<double> &R, double nx, double ny, double len,
+                        double gamma, double g_ga1, const double mu[4], PrimBar &Lb, PrimBar &Rb, double ULc[4],
+                        double URc[4]) {
+    const double unL = nx * L.u + ny * L.v, unR = nx * R.u + ny * R.v;
+    const double cL = sqrt(gamma * L.p * L.ir), cR = sqrt(gamma * R.p * R.ir);
+    const double aL = fabs(unL) + cL, aR = fabs(unR) + cR;
+    const double lam = (aL > aR) ? aL : aR;
+    const double h = 0.5 * len;
+    double lamb = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        lamb -= h * mu[k] * (R.U[k] - L.U[k]);
+        URc[k] -= h * mu[k] * lam;
+        ULc[k] += h * mu[k] * lam;
+    }
+    const double aLb = (aL > aR) ? lamb : 0.0, aRb = (aL > aR) ? 0.0 : lamb;
+    auto side = [&](const FaceState<double> &S, double un, double c, double ab, PrimBar &Sb) {
+        const double hs = S.p * g_ga1 * S.ir + 0.5 * (S.u * S.u + S.v * S.v);
+        const double t = S.r * un;
+        const double tb = h * (mu[0] + mu[1] * S.u + mu[2] * S.v + mu[3] * hs);
+        const double hsb = h * mu[3] * t;
+        double unb = tb * S.r + ab * sgn1(un);
+        Sb.r += tb * un;
+        Sb.u += h * mu[1] * t + hsb * S.u;
+        Sb.v += h * mu[2] * t + hsb * S.v;
+        Sb.p += h * (mu[1] * nx + mu[2] * ny) + hsb * g_ga1 * S.ir;
+        Sb.ir += hsb * S.p * g_ga1;
+        const double wb = ab * 0.5 / c;
+        Sb.p += wb * gamma * S.ir;
+        Sb.ir += wb * gamma * S.p;
+        Sb.u += unb * nx;
+        Sb.v += unb * ny;
+    };
+    side(L, unL, cL, aLb, Lb);
+    side(R, unR, cR, aRb, Rb);
+}
+
+// ghost_farfield, reverse: Rc = adjoint of the ghost's conserved state; into L's primitives
+__device__ void farfield_adj(const FaceState<double> &L, const double *ruvp_inf, double nx, double ny, double gamma,
+                             const double Rc[4], PrimBar &Lb) {
+    const double ga1 = gamma - 1;
+    const double rho_f = ruvp_inf[0], u_f = ruvp_inf[1], v_f = ruvp_inf[2], p_f = ruvp_inf[3];
+    const double rho_e = L.r, u_e = L.u, v_e = L.v, p_e = L.p;
+    const double af = sqrt(gamma * p_f / rho_f);
+    const double ae = sqrt(gamma * p_e / rho_e);
+    const double qnf = u_f * nx + v_f * ny, qne = u_e * nx + v_e * ny;
+    const double rf = qnf - 2.0 * af / ga1, re = qne + 2.0 * ae / ga1;
+    const double qn = 0.5 * (re + rf), as = ga1 * (re - rf) / 4.0;
+    if (qn <= -as) return;   // freestream state: no dependence on L
+    double r0[4];
+    const bool sub = qn < as, in = qn < 0;
+    double s_p = 0, s_r = 0, son = 0, X = 0;
+    if (sub) {
+        s_p = in ? p_f : p_e;
+        s_r = in ? rho_f : rho_e;
+        const double s_u = in ? u_f : u_e, s_v = in ? v_f : v_e, s_qn = in ? qnf : qne;
+        son = s_p / pow(s_r, gamma);
+        X = as * as / son / gamma;
+        r0[0] = pow(X, 1.0 / ga1);
+        r0[1] = s_u - s_qn * nx + qn * nx;
+        r0[2] = s_v - s_qn * ny + qn * ny;
+        r0[3] = as * as * r0[0] / gamma;
+    } else {
+        r0[0] = rho_e; r0[1] = u_e; r0[2] = v_e; r0[3] = p_e;
+    }
+    // ruvp2u, reverse
+    double r0b[4];
+    r0b[0] = Rc[0] + Rc[1] * r0[1] + Rc[2] * r0[2] + Rc[3] * 0.5 * (r0[1] * r0[1] + r0[2] * r0[2]);
+    r0b[1] = Rc[1] * r0[0] + Rc[3] * r0[0] * r0[1];
+    r0b[2] = Rc[2] * r0[0] + Rc[3] * r0[0] * r0[2];
+    r0b[3] = Rc[3] / ga1;
+    if (!sub) {
+        Lb.r += r0b[0]; Lb.u += r0b[1]; Lb.v += r0b[2]; Lb.p += r0b[3];
+        return;
+    }
+    const double r00b = r0b[0] + r0b[3] * as * as / gamma;
+    double asb = r0b[3] * 2.0 * as * r0[0] / gamma;
+    double qnb = r0b[1] * nx + r0b[2] * ny;
+    const double s_ub = r0b[1], s_vb = r0b[2], s_qnb = -r0b[1] * nx - r0b[2] * ny;
+    const double Xb = r00b * (1.0 / ga1) * pow(X, 1.0 / ga1 - 1.0);
+    asb += Xb * 2.0 * as / son / gamma;
+    const double sonb = -Xb * as * as / (son * son) / gamma;
+    double qneb = 0.0;
+    if (!in) {
+        const double srg = pow(s_r, gamma);
+        Lb.p += sonb / srg;
+        Lb.r -= sonb * s_p * gamma * pow(s_r, gamma - 1.0) / (srg * srg);
+        Lb.u += s_ub;
+        Lb.v += s_vb;
+        qneb += s_qnb;
+    }
+    const double reb = 0.5 * qnb + asb * ga1 / 4.0;
+    qneb += reb;
+    const double wb = reb * 2.0 / ga1 * 0.5 / ae;
+    Lb.p += wb * gamma / rho_e;
+    Lb.r -= wb * gamma * p_e / (rho_e * rho_e);
+    Lb.u += qneb * nx;
+    Lb.v += qneb * ny;
+}
+
+// The adjoints of the two reconstructed face states of one face for the flux cotangent mu:
+// ULb = (dF/dU_L^f)^T mu and URb = (dF/dU_R^f)^T mu (a boundary face's ghost folded into ULb).
+__device__ void face_vjp(const Phys &ph, int bType, const double *ruvp, const double ULf[4], const double URf[4],
+                         double nx, double ny, double len, const double mu[4], double ULb[4], double URb[4]) {
+    const double gamma = ph.gamma;
+    FaceState<double> L, R;
+#pragma unroll
+    for (int i = 0; i < 4; i++) L.U[i] = ULf[i];
+    set_prim(L, gamma);
+    if (bType == -1) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) R.U[i] = URf[i];
+        set_prim(R, gamma);
+    } else {
+        ghost_state(ph, bType, ruvp, L, R, nx, ny, gamma);
+    }
+    PrimBar Lb{0, 0, 0, 0, 0}, Rb{0, 0, 0, 0, 0};
+    double ULc[4] = {0, 0, 0, 0}, URc[4] = {0, 0, 0, 0};
+    if (ph.scheme == FVMR_SOL_LLF) llf_adj(L, R, nx, ny, len, gamma, ph.g_ga1, mu, Lb, Rb, ULc, URc);
+    else roe_adj(L, R, nx, ny, len, gamma, ph.g_ga1, ph.inv_g, mu, Lb, Rb);
+    set_prim_adj(R, gamma, Rb, URc);
+    if (bType == -1) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) URb[i] = URc[i];
+    } else if (bType == FVMR_BC_SYMMETRY || bType == FVMR_BC_WALL_NONVISCOUS) {
+        // R.U0 = L.U0, R.U3 = L.U3, R's velocity the reflection of L's
+        const double unL = L.u * nx + L.v * ny, utL = L.u * ny - L.v * nx;
+        const double unR = -unL, utR = utL;
+        const double uxR = unR * nx + utR * ny, uyR = unR * ny - utR * nx;
+        ULc[0] += URc[0] + URc[1] * uxR + URc[2] * uyR;
+        ULc[3] += URc[3];
+        const double uxRb = URc[1] * L.U[0], uyRb = URc[2] * L.U[0];
+        const double unRb = uxRb * nx + uyRb * ny, utRb = uxRb * ny - uyRb * nx;
+        Lb.u += -unRb * nx + utRb * ny;
+        Lb.v += -unRb * ny - utRb * nx;
+    } else if (bType == FVMR_BC_WALL_ADIABAT) {
+        ULc[0] += URc[0];
+        ULc[3] += URc[3];
+    } else {
+        farfield_adj(L, ruvp, nx, ny, gamma, URc, Lb);
+    }
+    set_prim_adj(L, gamma, Lb, ULc);
+#pragma unroll
+    for (int i = 0; i < 4; i++) ULb[i] = ULc[i];
+}
+
+// Part 0 of the product (one thread per face): the adjoints of the face's two reconstructed
+// states, fadj[2 e] for L and fadj[2 e + 1] for R, from mu_e = lambda_L - lambda_R.
+__global__ __launch_bounds__(kBlock) void k_adj_faces(Mesh<double> m, Phys ph, const S4 *__restrict__ U,
+                                                       const Grad<double> *__restrict__ grad,
+                                                       const S4 *__restrict__ psi, S4 *__restrict__ fadj) {
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= m.nf) return;
+    const int4 ei = m.eint[e];
+    const int Lc = ei.x, Rc = ei.y, bType = ei.z;
+    const V4<double> eg = m.egeo[e];
+    const V2<double> em = m.emid[e];
+    const V4<double> gL = m.cgeo[Lc];
+    double UL[4], ULf[4], URf[4] = {0, 0, 0, 0}, mu[4], p[4];
+    ld(U, Lc, UL);
+    recon_state(ph, UL, grad[Lc], gL.x, gL.y, em.x, em.y, ULf);
+    ld(psi, Lc, p);
+#pragma unroll
+    for (int k = 0; k < 4; k++) mu[k] = -p[k] / gL.z;
+    const bool interior = bType == -1;
+    if (interior) {
+        const V4<double> gR = m.cgeo[Rc];
+        double UR[4];
+        ld(U, Rc, UR);
+        recon_state(ph, UR, grad[Rc], gR.x, gR.y, em.x, em.y, URf);
+        ld(psi, Rc, p);
+#pragma unroll
+        for (int k = 0; k < 4; k++) mu[k] += p[k] / gR.z;
+    }
+    double ULb[4], URb[4];
+    face_vjp(ph, bType, bc_state(ph, ph.bc, bType, em.x, em.y, 0.0), ULf, URf, eg.x, eg.y, eg.z, mu, ULb, URb);
+    st(fadj, 2 * e, ULb);
+    if (interior) st(fadj, 2 * e + 1, URb);
+}
+
+// Gradient slots of grad_limit for one conserved variable with its four derivative directions
+// (0 = the cell's own state, 1 + i = the state across slot i).
+struct SlotsU1 {
+    double a[3][3];
+    Dual<4> uo[3];
+    double n[3][2];
+    __device__ __forceinline__ double A(int i, int q) const { return a[i][q]; }
+    __device__ __forceinline__ Dual<4> Uo(int i, int) const { return uo[i]; }
+    __device__ __forceinline__ double N(int i, int q) const { return n[i][q]; }
+};
+
+// The product with (dr/dU)^T, part 1 (one thread per cell c): the adjoints of c's three face
+// states (k_adj_faces) pass through the reconstruction to U_c and to c's limited gradient, whose
+// derivatives in U_c and the three states across (grad_limit on Dual<4>, one conserved variable
+// at a time: the variables do not mix) send them on: base[c] = the terms that land on U_c,
+// wedge[2 e + side] = the terms c sends to the cell across its edge e (side 1 when c is the
+// edge's R).  With gbar, also the adjoint of c's limited gradient.
+template <bool GBAR>
+__global__ __launch_bounds__(kAdjBlock) void k_adj_vjp(Mesh<double> m, Phys ph, const S4 *__restrict__ U,
                                                         const Grad<double> *__restrict__ grad,
-                                                        const S4 *__restrict__ psi, S4 *__restrict__ base,
+                                                        const S4 *__restrict__ fadj, S4 *__restrict__ base,
                                                         S4 *__restrict__ wedge, Grad<double> *__restrict__ gbar) {
-    using D = Dual<NT>;
+    using D = Dual<4>;
     const int c = blockIdx.x * kAdjBlock + threadIdx.x;
     const int nc = m.nc;
     if (c >= nc) return;
     const V4<double> cg = m.cgeo[c];
-    double Uc[4], lc[4];
+    double Uc[4];
     ld(U, c, Uc);
-    ld(psi, c, lc);
-#pragma unroll
-    for (int k = 0; k < 4; k++) lc[k] = -lc[k] / cg.z;
     const Grad<double> gc = grad[c];
     double ub[4] = {0, 0, 0, 0}, gbx[4] = {0, 0, 0, 0}, gby[4] = {0, 0, 0, 0};
     int outs[3], edges[3];
     bool isR[3];
-    const D gam(ph.gamma);
-#pragma unroll 1
+#pragma unroll
     for (int j = 0; j < 3; j++) {
         const int2 sl = m.cslot[j * nc + c];
-        const int e = sl.x & kSlotEdgeMask, out = sl.y;
+        const int e = sl.x & kSlotEdgeMask;
         const bool cR = (sl.x & kSlotCellIsR) != 0;
-        outs[j] = out;
+        outs[j] = sl.y;
         edges[j] = e;
         isR[j] = cR;
-        const int bType = (sl.x & kSlotBoundary) ? m.eint[e].z : -1;
-        const V4<double> eg = m.egeo[e];
         const V2<double> em = m.emid[e];
-        double mu[4];
-        if (out >= 0) {
-            double po[4];
-            ld(psi, out, po);
-            const double vo = m.cgeo[out].z;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const double lo = -po[k] / vo;
-                mu[k] = cR ? lo - lc[k] : lc[k] - lo;
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < 4; k++) mu[k] = lc[k];
-        }
-        double Uf[4], Ufo[4] = {0, 0, 0, 0};
+        double Uf[4], u4[4];
         const bool lin = recon_state(ph, Uc, gc, cg.x, cg.y, em.x, em.y, Uf);
-        if (bType == -1) {
-            double Uo[4];
-            ld(U, out, Uo);
-            const V4<double> og = m.cgeo[out];
-            recon_state(ph, Uo, grad[out], og.x, og.y, em.x, em.y, Ufo);
-        }
-        const double *ruvp = bc_state(ph, ph.bc, bType, em.x, em.y, 0.0);
-        double u4[4];
-#pragma unroll 1
-        for (int p0 = 0; p0 < 4; p0 += NT) {
-            FaceState<D> So, Sx;
-#pragma unroll
-            for (int i = 0; i < 4; i++) So.U[i] = seed_in<NT>(Uf[i], i, p0);
-            set_prim(So, gam);
-            if (bType == -1) {
-#pragma unroll
-                for (int i = 0; i < 4; i++) Sx.U[i] = D(Ufo[i]);
-                set_prim(Sx, gam);
-            } else {
-                ghost_state(ph, bType, ruvp, So, Sx, D(eg.x), D(eg.y), gam);
-            }
-            D F[4];
-            if (cR) riemann_flux(ph, Sx, So, D(eg.x), D(eg.y), D(eg.z), F);
-            else riemann_flux(ph, So, Sx, D(eg.x), D(eg.y), D(eg.z), F);
-#pragma unroll
-            for (int t = 0; t < NT; t++)
-                u4[p0 + t] = mu[0] * F[0].d[t] + mu[1] * F[1].d[t] + mu[2] * F[2].d[t] + mu[3] * F[3].d[t];
-        }
+        ld(fadj, 2 * e + (cR ? 1 : 0), u4);   // the adjoint of this cell's face state on e
         const double dx = em.x - cg.x, dy = em.y - cg.y;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -319,7 +588,7 @@ __global__ __launch_bounds__(kAdjBlock, FVMR_ADJ_WAVES) void k_adj_vjp(Mesh<doub
     double self[4] = {0, 0, 0, 0}, w[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
     if (ph.reconstruct == FVMR_REC_LINEAR) {
         double Uo[3][4];
-        SlotsU<NT> S;
+        SlotsU1 S;
         int fl[3];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
@@ -341,27 +610,15 @@ __global__ __launch_bounds__(kAdjBlock, FVMR_ADJ_WAVES) void k_adj_vjp(Mesh<doub
             fl[j] = (out >= 0) ? 3 : 0;
         }
 #pragma unroll 1
-        for (int p0 = 0; p0 < 4; p0 += NT) {
-            D Ud[4], gx[4], gy[4];
+        for (int k = 0; k < 4; k++) {
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                Ud[k] = seed_in<NT>(Uc[k], 0, p0);
+            for (int j = 0; j < 3; j++) S.uo[j] = D::seed(Uo[j][k], 1 + j);
+            const D Ud[1] = {D::seed(Uc[k], 0)};
+            D gx[1], gy[1];
+            grad_limit<D, SlotsU1, 1, double>(ph, Ud, S, fl, cg.z, gx, gy);
+            self[k] = gbx[k] * gx[0].d[0] + gby[k] * gy[0].d[0];
 #pragma unroll
-                for (int j = 0; j < 3; j++) S.uo[j][k] = seed_in<NT>(Uo[j][k], 1 + j, p0);
-            }
-            grad_limit<D>(ph, Ud, S, fl, D(cg.z), gx, gy);
-#pragma unroll
-            for (int t = 0; t < NT; t++) {
-                const int dir = p0 + t;
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const double v = gbx[k] * gx[k].d[t] + gby[k] * gy[k].d[t];
-                    if (dir == 0) self[k] = v;
-#pragma unroll
-                    for (int j = 0; j < 3; j++)
-                        if (dir == 1 + j) w[j][k] = v;
-                }
-            }
+            for (int j = 0; j < 3; j++) w[j][k] = gbx[k] * gx[0].d[1 + j] + gby[k] * gy[0].d[1 + j];
         }
     }
     base[c] = S4{ub[0] + self[0], ub[1] + self[1], ub[2] + self[2], ub[3] + self[3]};
@@ -734,6 +991,7 @@ int adj_alloc(fvmr_solver *h) {
     for (auto &q : w->psi) ok = ok && hipMalloc((void **)&q, nc * sizeof(S4)) == hipSuccess;
     ok = ok && hipMalloc((void **)&w->base, nc * sizeof(S4)) == hipSuccess &&
          hipMalloc((void **)&w->wedge, 2 * nf * sizeof(S4)) == hipSuccess &&
+         hipMalloc((void **)&w->fadj, 2 * nf * sizeof(S4)) == hipSuccess &&
          hipMalloc((void **)&w->g, nc * sizeof(S4)) == hipSuccess &&
          hipMalloc((void **)&w->res, nc * sizeof(S4)) == hipSuccess &&
          hipMalloc((void **)&w->dtl, nc * sizeof(double)) == hipSuccess &&
@@ -830,14 +1088,16 @@ int adj_product(fvmr_solver *h, const S4 *U, const S4 *psi, bool with_gbar) {
     if (rc) return rc;
     AdjWork *w = (AdjWork *)h->adj;
     const Mesh<double> m = mesh_args<double>(h);
+    k_adj_faces<<<grid_of(h->nf), kBlock, 0, h->stream>>>(m, h->ph, U, (const Grad<double> *)h->grad, psi, w->fadj);
+    HIP_OK(hipGetLastError());
     if (with_gbar) {
         if (!w->gbar && hipMalloc((void **)&w->gbar, (size_t)h->nc * sizeof(Grad<double>)) != hipSuccess)
             return set_error(FVMR_ERR_HIP, "hipMalloc adjoint gradient");
-        k_adj_vjp<true, FVMR_ADJ_NT><<<adj_grid(h->nc), kAdjBlock, 0, h->stream>>>(m, h->ph, U, (const Grad<double> *)h->grad, psi,
-                                                                       w->base, w->wedge, w->gbar);
+        k_adj_vjp<true><<<adj_grid(h->nc), kAdjBlock, 0, h->stream>>>(m, h->ph, U, (const Grad<double> *)h->grad,
+                                                                                   w->fadj, w->base, w->wedge, w->gbar);
     } else {
-        k_adj_vjp<false, FVMR_ADJ_NT><<<adj_grid(h->nc), kAdjBlock, 0, h->stream>>>(m, h->ph, U, (const Grad<double> *)h->grad, psi,
-                                                                        w->base, w->wedge, nullptr);
+        k_adj_vjp<false><<<adj_grid(h->nc), kAdjBlock, 0, h->stream>>>(m, h->ph, U, (const Grad<double> *)h->grad,
+                                                                                    w->fadj, w->base, w->wedge, nullptr);
     }
     HIP_OK(hipGetLastError());
     return FVMR_OK;

@@ -32,6 +32,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fvmr.h"
@@ -613,19 +614,24 @@ __device__ __forceinline__ void edge_alpha(const Phys &ph, const double ra[4], c
 // correct 2x2 inverse unless compat; then LimiterBarth_device_kernel, GradientGPU.cu:453-502,
 // or Venkatakrishnan).  Per slot i: fl bit 0 = a cell across, bit 1 = a non-periodic
 // neighbour; A = (gC, Sx s, Sy s) for Green-Gauss or (dx, dy) for least squares; Uo = the
-// state across (own state when none); N = vertex i relative to the centroid.
-template <class C, class Slots>
-__device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[4], const Slots &S, const int fl[3], C vol,
-                                           C gx[4], C gy[4]) {
+// state across (own state when none); N = vertex i relative to the centroid.  NV: the number of
+// conserved variables formed (they do not mix; the adjoint forms one at a time).
+template <class C, class Slots, int NV = 4, class V = C>
+__device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[NV], const Slots &S, const int fl[3], V vol,
+                                           C gx[NV], C gy[NV]) {
+    // the slot coefficients and vertex offsets keep the accessor's type (the adjoint's are
+    // plain doubles beside dual states)
+    using A_t = typename std::decay<decltype(S.A(0, 0))>::type;
+    using N_t = typename std::decay<decltype(S.N(0, 0))>::type;
     if (ph.gradient == FVMR_GRA_GREENGAUSS) {
-        const C inv_vol = frcp(vol);
+        const V inv_vol = frcp(vol);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < NV; k++) {
             C s0 = C(0), s1 = C(0);
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                const C g = S.A(i, 0);
-                const C phif = (fl[i] & 1) ? g * Uc[k] + (C(1.0) - g) * S.Uo(i, k) : Uc[k];
+                const A_t g = S.A(i, 0);
+                const C phif = (fl[i] & 1) ? g * Uc[k] + (A_t(1.0) - g) * S.Uo(i, k) : Uc[k];
                 s0 += phif * S.A(i, 1);
                 s1 += phif * S.A(i, 2);
             }
@@ -633,31 +639,31 @@ __device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[4], const 
             gy[k] = inv_vol * s1;
         }
     } else {
-        C M[4];   // A^T A
+        A_t M[4];   // A^T A
 #pragma unroll
         for (int x = 0; x < 2; x++)
 #pragma unroll
             for (int y = 0; y < 2; y++) {
-                C acc = C(0);
+                A_t acc = A_t(0);
 #pragma unroll
                 for (int i = 0; i < 3; i++)
                     if (fl[i] & 2) acc += S.A(i, x) * S.A(i, y);
                 M[x * 2 + y] = acc;
             }
-        C det = M[0] * M[3] - M[1] * M[2];
-        const C eps = C(fvmr::kEPS);
+        A_t det = M[0] * M[3] - M[1] * M[2];
+        const A_t eps = A_t(fvmr::kEPS);
         if (ph.compat_inv) {  // MatrixKernel.h:48-49 as written
-            if (det >= C(0) && det < eps) det = eps;
+            if (det >= A_t(0) && det < eps) det = eps;
             else if (det > -eps) det = -eps;
         } else {
-            if (det >= C(0) && det < eps) det = eps;
-            else if (det < C(0) && det > -eps) det = -eps;
+            if (det >= A_t(0) && det < eps) det = eps;
+            else if (det < A_t(0) && det > -eps) det = -eps;
         }
-        const C idet = frcp(det);
-        const C i00 = M[3] * idet, i01 = -M[1] * idet, i10 = -M[2] * idet, i11 = M[0] * idet;
+        const A_t idet = frcp(det);
+        const A_t i00 = M[3] * idet, i01 = -M[1] * idet, i10 = -M[2] * idet, i11 = M[0] * idet;
         const bool any = (fl[0] | fl[1] | fl[2]) & 2;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < NV; k++) {
             C a = C(0), b = C(0);
 #pragma unroll
             for (int i = 0; i < 3; i++)
@@ -672,11 +678,11 @@ __device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[4], const 
     if (ph.limiter != FVMR_LIM_NONE) {
         C eps2 = C(0);
         if (ph.limiter == FVMR_LIM_VENKATAKRISHNAN) {
-            const C Kh = (C)ph.venkat_K * fsqrt(vol);
+            const V Kh = (V)ph.venkat_K * fsqrt(vol);
             eps2 = Kh * Kh * Kh;
         }
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < NV; k++) {
             // bounds over the cell and its non-periodic neighbours (fmin/fmax: a NaN bound
             // already makes the Green-Gauss or least-squares gradient NaN)
             C lo = Uc[k], hi = Uc[k];
@@ -714,7 +720,7 @@ __device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[4], const 
             } else {
 #pragma unroll
                 for (int jn = 0; jn < 3; jn++) {
-                    const C ndx = S.N(jn, 0), ndy = S.N(jn, 1);
+                    const N_t ndx = S.N(jn, 0), ndy = S.N(jn, 1);
                     C pn = C(1.0);
                     const C d2 = gx[k] * ndx + gy[k] * ndy;
                     if (d2 != C(0)) {
