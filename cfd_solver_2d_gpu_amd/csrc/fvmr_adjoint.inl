@@ -13,15 +13,16 @@
 // matrix I + D (dr/dU)^T has the eigenvalues of the primal's linearised I + D dr/dU.
 //
 // Vector-Jacobian product.  psi . r = sum_e mu_e . F_e with mu_e = lambda_L - lambda_R and
-// lambda_c = -psi_c / V_c (0 outside the domain).  Per cell c (k_adj_vjp), for each face:
-// the flux's derivatives in c's reconstructed face state (forward-mode Dual<4> through
-// ghost_state and riemann_flux), contracted with mu_e, give the adjoint of that face state;
-// reconstruction passes it to U_c and to c's limited gradient.  The limited gradient's
-// derivatives in U_c and in the three states across (Dual<4> through grad_limit, the branch
-// taken by the values) then send it to U_c and to the neighbours: the share of neighbour n
-// goes to a per-(edge, side) slot, which n's gather (k_adj_update) adds in slot order, so the
-// product is deterministic.  Gradients, limiters and flux derivatives are recomputed from U*
-// at every product; nothing is stored between products but the state.
+// lambda_c = -psi_c / V_c (0 outside the domain).  Per face (k_adj_faces): the reverse mode of
+// the face physics (primitives, ghost state, Riemann flux, written out by hand) contracted with
+// mu_e gives the adjoints of the face's two reconstructed states.  Per cell (k_adj_vjp):
+// reconstruction passes its three face-state adjoints to U_c and to c's limited gradient; the
+// limited gradient's derivatives in U_c and in the three states across (forward mode: the
+// solver's grad_limit on Dual<4>, one conserved variable at a time, the branch taken by the
+// values) send that on to U_c and to the neighbours: the share of neighbour n goes to a
+// per-(edge, side) slot, which n's gather (k_adj_update) adds in slot order, so the product is
+// deterministic.  Gradients, limiters and flux derivatives are recomputed from U* at every
+// product; nothing is stored between products but the state.
 //
 // Scope: FP64 handles of one case, Euler, no periodic faces and no double Mach reflection
 // boundary (a time-dependent state).  dJ/dX assumes the standard geometry relations (centroid =
