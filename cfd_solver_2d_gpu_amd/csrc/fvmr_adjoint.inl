@@ -21,8 +21,9 @@
 // solver's grad_limit on Dual<4>, one conserved variable at a time, the branch taken by the
 // values) send that on to U_c and to the neighbours: the share of neighbour n goes to a
 // per-(edge, side) slot, which n's gather (k_adj_update) adds in slot order, so the product is
-// deterministic.  Gradients, limiters and flux derivatives are recomputed from U* at every
-// product; nothing is stored between products but the state.
+// deterministic.  The limited gradients of U* (a primal quantity) are formed once per solve;
+// the derivatives of flux, gradient and limiter are recomputed in every product, and no
+// Jacobian is stored.
 //
 // Scope: FP64 handles of one case, Euler, no periodic faces and no double Mach reflection
 // boundary (a time-dependent state).  dJ/dX assumes the standard geometry relations (centroid =
@@ -51,6 +52,7 @@ struct AdjWork {
     int *xnode = nullptr;
     int cur = 0;
     bool psi_valid = false;
+    bool grad_ready = false;   // h->grad holds the limited gradients of the state being differentiated
 };
 
 void adj_free(fvmr_solver *h) {
@@ -1085,9 +1087,12 @@ int field_norm2(fvmr_solver *h, const S4 *f, double out[4]) {
 
 // One product with (dr/dU)^T at U (gradients recomputed first): base/wedge, then the gather.
 int adj_product(fvmr_solver *h, const S4 *U, const S4 *psi, bool with_gbar) {
-    int rc = launch_gradient<double, double>(h, U, -1);
-    if (rc) return rc;
     AdjWork *w = (AdjWork *)h->adj;
+    if (!w->grad_ready) {   // the limited gradients of U (a primal quantity) once per solve
+        const int rc = launch_gradient<double, double>(h, U, -1);
+        if (rc) return rc;
+        w->grad_ready = true;
+    }
     const Mesh<double> m = mesh_args<double>(h);
     k_adj_faces<<<grid_of(h->nf), kBlock, 0, h->stream>>>(m, h->ph, U, (const Grad<double> *)h->grad, psi, w->fadj);
     HIP_OK(hipGetLastError());
@@ -1134,6 +1139,7 @@ int adj_alpha(fvmr_solver *h, const S4 *U, const S4 *psi, double *out) {
 // psi . dr/dX into dxy (2 nv, interleaved x, y; accumulated in cell and entry order).
 int adj_coords(fvmr_solver *h, const S4 *U, const S4 *psi, double *dxy) {
     AdjWork *w = (AdjWork *)h->adj;
+    w->grad_ready = false;   // steady_rhs below recomputes h->grad
     const size_t nc = h->nc;
     if (!w->xnode && (hipMalloc((void **)&w->xnode, nc * kXRec * sizeof(int)) != hipSuccess ||
                       hipMalloc((void **)&w->xval, nc * kXRec * 2 * sizeof(double)) != hipSuccess))
@@ -1273,6 +1279,7 @@ int fvmr_adjoint_vjp(fvmr_handle *h, const double *const psi[4], double *const o
     if (rc || (rc = adj_alloc(h))) return rc;
     AdjWork *w = (AdjWork *)h->adj;
     const S4 *U = h->Ubuf[h->cur];
+    w->grad_ready = false;
     if ((rc = upload_field(h, psi, w->psi[0]))) return rc;
     if ((rc = adj_product(h, U, w->psi[0], false))) return rc;
     if ((rc = adj_update(h, nullptr, nullptr, nullptr, w->psi[1], false, 0, 0, 1, nullptr))) return rc;
@@ -1292,6 +1299,7 @@ int fvmr_adjoint_solve(fvmr_handle *h, int nsets, const int *set_offsets, const 
     if ((rc = j_sets(h, nsets, set_offsets, set_edges, js))) return rc;
     AdjWork *w = (AdjWork *)h->adj;
     const S4 *U = h->Ubuf[h->cur];
+    w->grad_ready = false;
     const int nv = h->nv, ne = (int)js.en.size() / 2;
     // J and its explicit parts from the node Cp of U
     if ((rc = node_field_device(h))) return rc;
