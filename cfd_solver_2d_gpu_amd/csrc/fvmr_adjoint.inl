@@ -550,8 +550,11 @@ struct SlotsU1 {
 // at a time: the variables do not mix) send them on: base[c] = the terms that land on U_c,
 // wedge[2 e + side] = the terms c sends to the cell across its edge e (side 1 when c is the
 // edge's R).  With gbar, also the adjoint of c's limited gradient.
+#ifndef FVMR_ADJ_WAVES
+#define FVMR_ADJ_WAVES 2   // minimum waves per SIMD k_adj_vjp is compiled for (3 spills: slower)
+#endif
 template <bool GBAR>
-__global__ __launch_bounds__(kAdjBlock) void k_adj_vjp(Mesh<double> m, Phys ph, const S4 *__restrict__ U,
+__global__ __launch_bounds__(kAdjBlock, FVMR_ADJ_WAVES) void k_adj_vjp(Mesh<double> m, Phys ph, const S4 *__restrict__ U,
                                                         const Grad<double> *__restrict__ grad,
                                                         const S4 *__restrict__ fadj, S4 *__restrict__ base,
                                                         S4 *__restrict__ wedge, Grad<double> *__restrict__ gbar) {
