@@ -6,7 +6,9 @@ Host-side mirror of the reference solver surface over the C ABI of libfvmr.so:
   field_init    FieldInitializer (FVMR/src/input/FieldInitializer.cpp:22-229)
   GPUSolver2    GPU::GPUSolver2 + CIteration::iteration_device
                 (FVMR/src/solvers/GPUSolver2.h:32-89, FVMR/src/time/CIteration.cpp:89-161)
-  config        input.toml keys of TomlFileManager (FVMR/src/input/TomlFileManager.cpp:110-265)
+  driver        input.toml reader (TomlFileManager, FVMR/src/input/TomlFileManager.cpp:110-265), the
+                CDriver::start loop and the Tecplot/hist/continue writers (FVMR/src/drivers/CDriver.cpp:31-257,
+                FVMR/src/output/FieldWriter.cpp:44-262); `python -m cfd_solver_2d_gpu_amd.driver input.toml`
 """
 import ctypes as C
 

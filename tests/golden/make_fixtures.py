@@ -1,9 +1,10 @@
 """Copies the reference's own data files used as fixtures into tests/golden (gzip).
 
 Run in a checkout where the reference repository is at /root/reference.  The files are
-data: quick_start SU2 meshes and one continue file the reference wrote at step 20000
-(FVMR/src/output/FieldWriter.cpp:264-326), which pins the post-CCW element node lists,
-the reference edge numbering of the boundary edges and a realistic FP32 state.
+data: the quick_start cases' input.toml files (copied as they are), quick_start SU2 meshes and
+one continue file the reference wrote at step 20000 (FVMR/src/output/FieldWriter.cpp:264-326),
+which pins the post-CCW element node lists, the reference edge numbering of the boundary edges
+and a realistic FP32 state.
 """
 import gzip
 import os
@@ -18,7 +19,19 @@ FILES = [
     ("1D_shocktube/shocktube_100.su2", "shocktube_100.su2.gz"),
 ]
 
+# the quick_start cases' input.toml files, copied as they are (config data for the driver tests)
+CONFIGS = [
+    ("1D_shocktube/input.toml", "shocktube.input.toml"),
+    ("2D_Neuvillette_shock/input.toml", "neuvillette.input.toml"),
+    ("2D_cylinder/input.toml", "cylinder.input.toml"),
+    ("2D_double_shock_reflect/input.toml", "dsr.input.toml"),
+    ("2D_isentropic_vortex/input.toml", "ivortex.input.toml"),
+]
+
 if __name__ == "__main__":
+    for src, dst in CONFIGS:
+        shutil.copyfile(os.path.join(REF, src), os.path.join(HERE, dst))
+        print(dst, os.path.getsize(os.path.join(HERE, dst)))
     for src, dst in FILES:
         with open(os.path.join(REF, src), "rb") as f, open(os.path.join(HERE, dst), "wb") as raw:
             with gzip.GzipFile(fileobj=raw, mode="wb", compresslevel=9, mtime=0) as g:

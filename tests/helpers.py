@@ -1,5 +1,8 @@
-"""Shared case builders of the test-suite (meshes, parameters, initial fields)."""
+"""Shared case builders of the test-suite (meshes, parameters, initial fields, input.toml trees)."""
+import os
+
 import numpy as np
+import tomli
 
 import cfd_solver_2d_gpu_amd as F
 
@@ -88,3 +91,48 @@ class SoaMesh:
         s.boundary_type = ip(a["boundary_type"])
         self.soa = s
         self.num_element = s.num_element
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def toml_tree(name):
+    """The parsed tree of tests/golden/<name> (a quick_start input.toml)."""
+    with open(os.path.join(GOLDEN, name), "rb") as f:
+        return tomli.load(f)
+
+
+def tree_set(tree, key, value):
+    """Set (or, with None, remove) a dotted key of a parsed TOML tree."""
+    node = tree
+    parts = key.split(".")
+    for p in parts[:-1]:
+        node = node.setdefault(p, {})
+    if value is None:
+        node.pop(parts[-1], None)
+    else:
+        node[parts[-1]] = value
+
+
+def write_toml(path, tree):
+    """Write a tree of tables, strings, booleans and numbers as TOML."""
+    lines = []
+
+    def walk(prefix, node):
+        scalars = {k: v for k, v in node.items() if not isinstance(v, dict)}
+        if scalars and prefix:
+            lines.append(f"[{'.'.join(prefix)}]")
+            for k, v in scalars.items():
+                if isinstance(v, bool):
+                    lines.append(f"{k} = {'true' if v else 'false'}")
+                elif isinstance(v, str):
+                    lines.append(f'{k} = "{v}"')
+                else:
+                    lines.append(f"{k} = {v!r}")
+        for k, v in node.items():
+            if isinstance(v, dict):
+                walk(prefix + [k], v)
+
+    walk([], tree)
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")

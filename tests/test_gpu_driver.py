@@ -1,0 +1,180 @@
+"""The host driver on the device: input.toml cases of the reference's quick_start folder run
+through Driver, checked against direct runs of the solver on the same inputs (final state bit
+for bit, Tecplot and hist values to their printed precision), a bit-exact restart from the
+driver's own continue file, and a continue from the reference's step-20000 file."""
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+import cfd_solver_2d_gpu_amd as F
+from cfd_solver_2d_gpu_amd import driver as D
+from helpers import toml_tree, tree_set, write_toml
+
+pytestmark = pytest.mark.gpu
+
+
+def make_case(tmp_path, golden_dir, config, mesh, over, name="case"):
+    d = tmp_path / name
+    d.mkdir(exist_ok=True)
+    shutil.copyfile(golden_dir / (mesh + ".su2"), d / (mesh + ".su2"))
+    t = toml_tree(config)
+    for k, v in over.items():
+        tree_set(t, k, v)
+    write_toml(d / "input.toml", t)
+    return d
+
+
+def read_tecplot_nodes(path, nv):
+    with open(path) as f:
+        lines = f.read().splitlines()
+    return np.array([l.split() for l in lines[3:3 + nv]], dtype=float)
+
+
+def read_hist(path):
+    rows = [l.split() for l in open(path).read().splitlines()[3:]]
+    return {int(r[0]): np.array(r[1:], dtype=float) for r in rows}
+
+
+def direct(cfg, mesh_path, steps_until, probe_steps=()):
+    """The solver on the case's inputs, stepped without the driver: state after steps_until steps,
+    and at each probe step s the node field and wall forces before step s and the infinity norm
+    of the rhs of step s."""
+    m = F.Mesh.read_su2(str(mesh_path), cfg.equation, cfg.mesh_scale_factor)
+    p = cfg.params()
+    U = F.field_init(m, p, cfg.initial_type, cfg.init_args())
+    walls = [e for _, typ, e in m.boundary_sets() if typ in D._WALLS]
+    probes = {}
+    with F.GPUSolver2(m, p) as s:
+        s.set_state(U)
+        t, done = 0.0, 0
+        for ps in sorted(probe_steps):
+            if ps - 1 > done:
+                t, _ = s.advance(ps - 1 - done, t, cfg.max_physical_time)
+            ruvp, cp, ma = s.node_field()
+            cx, cy = s.surface_force(cfg.referenceArea, walls) if walls else (np.zeros(1), np.zeros(1))
+            t, _ = s.iteration_device(t, cfg.max_physical_time)
+            probes[ps] = (ruvp, cp, ma, float(np.sum(cy)), float(np.sum(cx)), s.updateResidualVector(F.NORM_INF), t)
+            done = ps
+        if steps_until > done:
+            t, _ = s.advance(steps_until - done, t, cfg.max_physical_time)
+        return m, s.get_state(), t, probes
+
+
+def test_driver_shocktube(tmp_path, golden_dir):
+    over = {"basic.continue": False, "output.maxIteration": 120, "output.step_per_print": 50,
+            "output.step_per_output_field": 100, "output.step_per_output_hist": 50}
+    d = make_case(tmp_path, golden_dir, "shocktube.input.toml", "shocktube_100", over)
+    print("case: shocktube_100 via input.toml, 120 steps", flush=True)
+    drv = D.Driver(str(d / "input.toml"), log=None)
+    s = drv.run()
+    assert s["signal"] == "TimeReached" and s["istep"] == 120
+    m, U, t, probes = direct(drv.cfg, d / "shocktube_100.su2", 120, probe_steps=(1, 50, 100))
+    # final state: the continue file (17 significant digits) holds the direct run's state exactly
+    _, Uc, tc, ic, cflc = F.Mesh.read_continue(str(d / "pause_shocktube_100[0120].dat"))
+    assert ic == 120 and tc == t and cflc == 2.5
+    assert np.array_equal(Uc, U)
+    # Tecplot volume files at steps 1, 100 and the final 120 hold the node field before the step
+    nv = m.soa.num_node
+    for k in (1, 100):
+        got = read_tecplot_nodes(d / f"shocktube_100[{k:04d}].dat", nv)
+        ruvp, cp, ma = probes[k][:3]
+        want = np.column_stack([m.arrays()["node_xy"].T, ruvp.T, ma])
+        assert np.allclose(got, want, rtol=1e-6, atol=1e-12), k
+    assert os.path.exists(d / "shocktube_100[0120].dat") and os.path.exists(d / "shocktube_100_tec_boundary.dat")
+    # hist rows at 1, 50, 100: t, the rhs norm of the step, wall force coefficients before the step
+    h = read_hist(d / "shocktube_100_hist.dat")
+    assert sorted(h) == [1, 50, 100]
+    for k, row in h.items():
+        _, _, _, CL, CD, res, tk = probes[k]
+        assert np.allclose(row[0], tk, rtol=1e-6)
+        assert np.allclose(row[1:5], res, rtol=1e-6, atol=1e-300)
+        # no freestream in a shock tube (inf state zero): Cp and the coefficients are NaN, as in the reference
+        assert np.allclose(row[5:7], [CL, CD], rtol=1e-6, atol=1e-12, equal_nan=True)
+
+
+def test_driver_restart_is_bit_exact(tmp_path, golden_dir):
+    over = {"basic.continue": False, "output.maxIteration": 60, "output.step_per_print": 25,
+            "output.step_per_output_field": 1000, "output.step_per_output_hist": 1000}
+    d = make_case(tmp_path, golden_dir, "shocktube.input.toml", "shocktube_100", over)
+    print("case: shocktube_100, 60 steps then continue to 100", flush=True)
+    s1 = D.Driver(str(d / "input.toml"), log=None).run()
+    assert s1["istep"] == 60 and os.path.exists(d / "pause_shocktube_100[0060].dat")
+    t = toml_tree("shocktube.input.toml")
+    for k, v in over.items():
+        tree_set(t, k, v)
+    tree_set(t, "basic.continue", True)
+    tree_set(t, "output.maxIteration", 100)
+    write_toml(d / "input.toml", t)
+    drv = D.Driver(str(d / "input.toml"), log=None)
+    s2 = drv.run()
+    assert drv.continued and drv.istep0 == 60 and s2["istep"] == 100
+    _, U, tf, _ = direct(drv.cfg, d / "shocktube_100.su2", 100)
+    _, Uc, tc, ic, _ = F.Mesh.read_continue(str(d / "pause_shocktube_100[0100].dat"))
+    assert ic == 100 and tc == tf
+    assert np.array_equal(Uc, U)
+
+
+def test_driver_vortex_fresh_start(tmp_path, golden_dir):
+    over = {"basic.continue": False, "output.maxIteration": 40, "output.step_per_print": 20,
+            "output.step_per_output_field": 1000, "output.step_per_output_hist": 1000}
+    d = make_case(tmp_path, golden_dir, "ivortex.input.toml", "ivortex50", over)
+    print("case: ivortex50 via input.toml, 40 steps", flush=True)
+    drv = D.Driver(str(d / "input.toml"), log=None)
+    s = drv.run()
+    assert s["istep"] == 40
+    _, U, t, _ = direct(drv.cfg, d / "ivortex50.su2", 40)
+    _, Uc, tc, _, _ = F.Mesh.read_continue(str(d / "pause_ivortex50[0040].dat"))
+    assert tc == t and np.array_equal(Uc, U)
+
+
+def test_driver_cylinder_continues_reference_file(tmp_path, golden_dir):
+    """basic.continue = true picks the reference's pause_cylinder_allfar_50[20000].dat: t, istep and
+    CFL from the file, and, as in the reference, the mesh from its node, element and boundary lists
+    (node coordinates with the 6 significant digits the reference wrote)."""
+    over = {"output.maxIteration": 20050, "output.step_per_print": 25, "output.step_per_output_hist": 25,
+            "output.step_per_output_field": 1000}
+    d = make_case(tmp_path, golden_dir, "cylinder.input.toml", "cylinder_allfar_50", over)
+    pause = d / "pause_cylinder_allfar_50[20000].dat"
+    shutil.copyfile(golden_dir / "pause_cylinder_allfar_50_20000.dat", pause)
+    print("case: cylinder_allfar_50 continued from step 20000, 50 steps", flush=True)
+    drv = D.Driver(str(d / "input.toml"), log=None)
+    s = drv.run()
+    assert drv.continued and drv.istep0 == 20000 and s["istep"] == 20050 and s["signal"] == "TimeReached"
+    assert drv.t0 == 0.120747 and drv.params.CFL == 2.2
+    h = read_hist(d / "cylinder_allfar_50_hist.dat")
+    assert sorted(h) == [20025, 20050]
+    # the same 50 steps without the driver, from the continue file's mesh and state
+    m, U0, t0, _, _ = F.Mesh.read_continue(str(pause), F.EQ_NS)
+    walls = [e for _, typ, e in m.boundary_sets() if typ in D._WALLS]
+    assert len(walls) == 1
+    with F.GPUSolver2(m, drv.cfg.params()) as sol:
+        sol.set_state(U0)
+        t, _ = sol.advance(24, t0, drv.cfg.max_physical_time)
+        cx, cy = sol.surface_force(drv.cfg.referenceArea, walls)
+        t, _ = sol.advance(26, t, drv.cfg.max_physical_time)
+        U = sol.get_state()
+    assert np.allclose(h[20025][5:7], [np.sum(cy), np.sum(cx)], rtol=2e-6, atol=1e-12)
+    _, Uc, tc, ic, _ = F.Mesh.read_continue(str(d / "pause_cylinder_allfar_50[20050].dat"), F.EQ_NS)
+    assert ic == 20050 and tc == t and s["t"] == t
+    assert np.array_equal(Uc, U)
+
+
+def test_driver_steady_mode(tmp_path, golden_dir):
+    over = {"basic.continue": False, "time.is_steady": 1, "output.maxIteration": 30, "output.step_per_print": 10,
+            "output.step_per_output_field": 1000, "output.step_per_output_hist": 1000}
+    d = make_case(tmp_path, golden_dir, "ivortex.input.toml", "ivortex50", over)
+    print("case: ivortex50 steady, 30 iterations", flush=True)
+    drv = D.Driver(str(d / "input.toml"), log=None)
+    s = drv.run()
+    assert s["istep"] == 30 and drv.params.CFL == 0.05
+    m = F.Mesh.read_su2(str(d / "ivortex50.su2"), drv.cfg.equation)
+    p = drv.cfg.params()
+    p.CFL = 0.05
+    with F.GPUSolver2(m, p) as sol:
+        sol.set_state(F.field_init(m, p, drv.cfg.initial_type, drv.cfg.init_args()))
+        sol.steady_solve(30, 0.0)
+        U = sol.get_state()
+    _, Uc, _, _, _ = F.Mesh.read_continue(str(d / "pause_ivortex50[0030].dat"))
+    assert np.array_equal(Uc, U)
