@@ -396,7 +396,8 @@ def main():
     traffic = traffic_from_profiles()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_tile<double,double> (one RK stage: gradients, limiter, fluxes, update)",
+                "kernel": "k_tile<double,double,false,false,1> (one RK stage: gradients, limiter, fluxes, update; "
+                          "Green-Gauss + Barth + Roe fixed at compile time)",
                 "bytes_per_launch": int(b_launch), "avg_launch_ms": round(st_ms, 5), "launches": st_n}
     path_b = path_bytes_per_step(nc, nf, nv, 8, 8, 8)
     out = {
