@@ -1096,11 +1096,19 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
     const int4 gs = (i < nG) ? ta.gslot[offG + i] : make_int4(0, 0, 0, 0);
     const V4<G> gco = (i < nG) ? ta.gcoef[offG + i] : V4<G>{G(0.5), G(0.5), G(0.5), G(1)};
     const G gvol = gco.w;
-    // 1. stage nodes, states and face records
-    for (int k = tid; k < nN; k += kTileThreads) sN[k] = ta.nxy[offN + k];
-    if (i < nL) sU[i] = Uin[i < nT ? c0 + i : ta.halo[offH + i - nT]];
+    // 1. stage nodes, states and face records: every global load is issued before the first
+    // LDS store (a store between loads would make each wave wait for the loads before it)
+    static_assert(kTileMaxN <= kTileThreads && kTileMaxF <= 2 * kTileThreads, "one node and two faces per thread");
+    const int src = (i < nL) ? (i < nT ? c0 + i : ta.halo[offH + i - nT]) : -1;
+    const S4 uin = (src >= 0) ? Uin[src] : S4{0.0, 0.0, 0.0, 0.0};
+    const V2<double> nd = (tid < nN) ? ta.nxy[offN + tid] : V2<double>{0.0, 0.0};
+    const int4 fr0 = (tid < nF) ? ta.face[offF + tid] : make_int4(0, 0, 0, 0);
+    const int4 fr1 = (tid + kTileThreads < nF) ? ta.face[offF + tid + kTileThreads] : make_int4(0, 0, 0, 0);
     const S4 un = (!first && i < nT) ? Un[c0 + i] : S4{0.0, 0.0, 0.0, 0.0};
-    for (int k = tid; k < nF; k += kTileThreads) sFace[k] = ta.face[offF + k];
+    if (tid < nN) sN[tid] = nd;
+    if (tid < nF) sFace[tid] = fr0;
+    if (tid + kTileThreads < nF) sFace[tid + kTileThreads] = fr1;
+    if (i < nL) sU[i] = uin;
     __syncthreads();
     // 2. centroids and face normals
     if (i < nL) {
@@ -1263,10 +1271,16 @@ __global__ __launch_bounds__(TileCfg<G>::kThreads) void k_tile_dt(TileArgs<G> ta
     const int c0 = ti[fvmr::kTiC0], nT = ti[fvmr::kTiNT], nG = ti[fvmr::kTiNG];
     const int nN = ti[fvmr::kTiNN], nFF = ti[fvmr::kTiNFF];
     const int offH = ti[fvmr::kTiOffH], offN = ti[fvmr::kTiOffN], offF = ti[fvmr::kTiOffF], offG = ti[fvmr::kTiOffG];
+    // every global load is issued before the first LDS store
+    static_assert(kTileMaxN <= kTileThreads, "one node per thread");
     const DtRec dr = (tid < nT) ? ta.dtrec[c0 + tid] : DtRec{0, 0, 1.0};
-    for (int i = tid; i < nN; i += kTileThreads) sN[i] = ta.nxy[offN + i];
-    if (tid < nG) sU[tid] = U[tid < nT ? c0 + tid : ta.halo[offH + tid - nT]];
-    if (tid < nFF) sFace[tid] = ta.face[offF + tid];
+    const V2<double> nd = (tid < nN) ? ta.nxy[offN + tid] : V2<double>{0.0, 0.0};
+    const int4 fr = (tid < nFF) ? ta.face[offF + tid] : make_int4(0, 0, 0, 0);
+    const int src = (tid < nG) ? (tid < nT ? c0 + tid : ta.halo[offH + tid - nT]) : -1;
+    const S4 u = (src >= 0) ? U[src] : S4{0.0, 0.0, 0.0, 0.0};
+    if (tid < nN) sN[tid] = nd;
+    if (tid < nFF) sFace[tid] = fr;
+    if (tid < nG) sU[tid] = u;
     __syncthreads();
     // faces of the tile cells: L and R (when present) are tile or first-ring cells
     for (int f = tid; f < nFF; f += kTileThreads) {
