@@ -510,22 +510,24 @@ __global__ __launch_bounds__(kBlock) void k_adj_faces(Mesh<double> m, Phys ph, c
     const int Lc = ei.x, Rc = ei.y, bType = ei.z;
     const V4<double> eg = m.egeo[e];
     const V2<double> em = m.emid[e];
-    const V4<double> gL = m.cgeo[Lc];
-    double UL[4], ULf[4], URf[4] = {0, 0, 0, 0}, mu[4], p[4];
-    ld(U, Lc, UL);
-    recon_state(ph, UL, grad[Lc], gL.x, gL.y, em.x, em.y, ULf);
-    ld(psi, Lc, p);
-#pragma unroll
-    for (int k = 0; k < 4; k++) mu[k] = -p[k] / gL.z;
     const bool interior = bType == -1;
-    if (interior) {
-        const V4<double> gR = m.cgeo[Rc];
-        double UR[4];
-        ld(U, Rc, UR);
-        recon_state(ph, UR, grad[Rc], gR.x, gR.y, em.x, em.y, URf);
-        ld(psi, Rc, p);
+    // both sides are loaded before either is used (a boundary face reads its L cell twice), so
+    // the R loads do not wait behind the L arithmetic
+    const int Rs = interior ? Rc : Lc;
+    const V4<double> gL = m.cgeo[Lc], gR = m.cgeo[Rs];
+    const Grad<double> grL = grad[Lc], grR = grad[Rs];
+    double UL[4], UR[4], pL[4], pR[4], ULf[4], URf[4] = {0, 0, 0, 0}, mu[4];
+    ld(U, Lc, UL);
+    ld(U, Rs, UR);
+    ld(psi, Lc, pL);
+    ld(psi, Rs, pR);
+    recon_state(ph, UL, grL, gL.x, gL.y, em.x, em.y, ULf);
 #pragma unroll
-        for (int k = 0; k < 4; k++) mu[k] += p[k] / gR.z;
+    for (int k = 0; k < 4; k++) mu[k] = -pL[k] / gL.z;
+    if (interior) {
+        recon_state(ph, UR, grR, gR.x, gR.y, em.x, em.y, URf);
+#pragma unroll
+        for (int k = 0; k < 4; k++) mu[k] += pR[k] / gR.z;
     }
     double ULb[4], URb[4];
     face_vjp(ph, bType, bc_state(ph, ph.bc, bType, em.x, em.y, 0.0), ULf, URf, eg.x, eg.y, eg.z, mu, ULb, URb);
@@ -653,19 +655,26 @@ __global__ __launch_bounds__(kBlock) void k_adj_update(Mesh<double> m, const S4 
     const int c = blockIdx.x * kBlock + threadIdx.x;
     const int nc = m.nc;
     if (c >= nc) return;
+    // the three shares are loaded before any is used (every slot has an edge, so the index is in
+    // range; a share counts only where a cell lies across): no load waits for another
+    int2 sl[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) sl[j] = m.cslot[j * nc + c];
+    double q[3][4];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int e = sl[j].x & kSlotEdgeMask;
+        const bool cR = (sl[j].x & kSlotCellIsR) != 0;
+        ld(wedge, 2 * e + (cR ? 0 : 1), q[j]);
+    }
     double v[4];
     ld(base, c, v);
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-        const int2 sl = m.cslot[j * nc + c];
-        if (sl.y < 0) continue;
-        const int e = sl.x & kSlotEdgeMask;
-        const bool cR = (sl.x & kSlotCellIsR) != 0;
-        double q[4];
-        ld(wedge, 2 * e + (cR ? 0 : 1), q);
+    for (int j = 0; j < 3; j++)
+        if (sl[j].y >= 0) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) v[k] += q[k];
-    }
+            for (int k = 0; k < 4; k++) v[k] += q[j][k];
+        }
     if (g) {
         double q[4];
         ld(g, c, q);
