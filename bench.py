@@ -99,10 +99,11 @@ def run_gpu(scheme, precision, steps, warmup, profile, kernels=None):
     return m, wall, state, kern
 
 
-def traffic_from_profiles():
-    """HBM bytes per k_tile launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
-    scaled by the copy-kernel calibration; committed in profiles/ (None when absent)."""
-    path = os.path.join(ROOT, "profiles", "k_tile_pmc_wedge_1M_fp64.txt")
+def traffic_from_profiles(kernel="k_tile"):
+    """HBM bytes per launch of the stage kernel (k_tile, or the cell family's k_stage) measured
+    with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), scaled by the copy-kernel calibration;
+    committed in profiles/ (None when absent)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_pmc_wedge_1M_fp64.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -364,6 +365,9 @@ def main():
     ap.add_argument("--c5-only", action="store_true", help="run only the adjoint leg and print its JSON")
     ap.add_argument("--c5-iters", type=int, nargs=3, default=[260000, 150000, 140000],
                     metavar=("PRIMAL", "ADJOINT", "FD"), help="iteration budgets of the adjoint leg")
+    ap.add_argument("--kernels", choices=("tile", "cell"), default="tile",
+                    help="kernel family of the FP64 headline leg (the cell family is the fallback for meshes "
+                         "whose geometry the tile kernels cannot recompute)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (calibrates rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -385,19 +389,22 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    m, wall, U64, kern = run_gpu(F.SOL_ROE, F.PREC_FP64, args.steps, args.warmup, profile=True)
+    cell = args.kernels == "cell"
+    m, wall, U64, kern = run_gpu(F.SOL_ROE, F.PREC_FP64, args.steps, args.warmup, profile=True,
+                                 kernels=F.KERNELS_CELL if cell else None)
     nc, nf, nv = m.num_element, m.num_edge, m.num_node
     value = nc * 3 * args.steps / wall
     ms_step = wall / args.steps * 1e3
 
-    st_ms, st_n = kern["k_tile"]
+    st_ms, st_n = kern["k_stage" if cell else "k_tile"]
     b_launch = stage_bytes(nc, nf, nv, 8, 8, 8)
     achieved = b_launch / (st_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles()
+    traffic = traffic_from_profiles("k_stage" if cell else "k_tile")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_tile<double,double,false,false,1> (one RK stage: gradients, limiter, fluxes, update; "
-                          "Green-Gauss + Barth + Roe fixed at compile time)",
+                "kernel": ("k_stage<double,double> (one RK stage after k_gradient: fluxes, update; cell family)" if cell
+                           else "k_tile<double,double,false,false,1> (one RK stage: gradients, limiter, fluxes, "
+                                "update; Green-Gauss + Barth + Roe fixed at compile time)"),
                 "bytes_per_launch": int(b_launch), "avg_launch_ms": round(st_ms, 5), "launches": st_n}
     path_b = path_bytes_per_step(nc, nf, nv, 8, 8, 8)
     out = {
