@@ -242,6 +242,35 @@ struct GlobalCells {
     __device__ __forceinline__ bool has(int c) const { return !(c < 0 || c >= m.nc); }
 };
 
+// The records of a face's two cells loaded together before the flux uses them (R = L where the
+// face has no cell across), so the R loads do not wait behind the L arithmetic; answers for
+// those two cells only.
+template <class G>
+struct PairCells {
+    int L, nc;
+    double uL[4], uR[4];
+    V2<G> cL, cR;
+    Grad<G> gL, gR;
+    __device__ __forceinline__ PairCells(const Mesh<G> &m, const S4 *__restrict__ U, const Grad<G> *__restrict__ grad,
+                                         int Lc, int Rc) : L(Lc), nc(m.nc) {
+        const int Rs = (Rc < 0 || Rc >= m.nc) ? Lc : Rc;
+        ld(U, Lc, uL);
+        ld(U, Rs, uR);
+        const V4<G> a = m.cgeo[Lc], b = m.cgeo[Rs];
+        gL = grad[Lc];
+        gR = grad[Rs];
+        cL = V2<G>{a.x, a.y};
+        cR = V2<G>{b.x, b.y};
+    }
+    __device__ __forceinline__ void state(int c, double u[4]) const {
+#pragma unroll
+        for (int k = 0; k < 4; k++) u[k] = (c == L) ? uL[k] : uR[k];
+    }
+    __device__ __forceinline__ V2<G> centroid(int c) const { return (c == L) ? cL : cR; }
+    __device__ __forceinline__ Grad<G> gradient(int c) const { return (c == L) ? gL : gR; }
+    __device__ __forceinline__ bool has(int c) const { return !(c < 0 || c >= nc); }
+};
+
 // get_Uvector_reconstruct_2_device (FluxGPU.cu:47-61): linear (3002) or constant
 // reconstruction; back to the cell value when rho or p leave their range.
 template <class C, class G, class Cells>
@@ -547,8 +576,8 @@ __device__ __forceinline__ void face_flux(const Cells &cells, const Phys &ph, co
 }
 
 // The flux of edge e from the global records; the BC type and pair are read only for
-// boundary-set edges (bset).
-template <class C, class G, bool NS>
+// boundary-set edges (bset).  PAIR: the two cells' records prefetched together (PairCells).
+template <class C, class G, bool NS, bool PAIR = false>
 __device__ __forceinline__ void edge_flux(const Mesh<G> &m, const Phys &ph, const S4 *__restrict__ U,
                                           const Grad<G> *__restrict__ grad, double t_rk, int e, int Lc, int Rc,
                                           bool bset, C F[4]) {
@@ -563,8 +592,13 @@ __device__ __forceinline__ void edge_flux(const Mesh<G> &m, const Phys &ph, cons
     const bool has_pair = pe >= 0 && pe < m.nf;
     V2<G> pm = em;
     if (bType != -1 && has_pair) pm = m.emid[pe];
-    const GlobalCells<G> cells{m, U, grad};
-    face_flux<C, G, NS>(cells, ph, ph.bc, t_rk, Lc, Rc, bType, has_pair, eg, em, pm, F);
+    if (PAIR) {
+        const PairCells<G> cells(m, U, grad, Lc, Rc);
+        face_flux<C, G, NS>(cells, ph, ph.bc, t_rk, Lc, Rc, bType, has_pair, eg, em, pm, F);
+    } else {
+        const GlobalCells<G> cells{m, U, grad};
+        face_flux<C, G, NS>(cells, ph, ph.bc, t_rk, Lc, Rc, bType, has_pair, eg, em, pm, F);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -897,8 +931,10 @@ __global__ __launch_bounds__(kGroup3) FVMR_STAGE_ATTR void k_stage(Mesh<G> m, Ph
         const int2 sl = m.cslot[j * nc + c];
         const bool cR = (sl.x & kSlotCellIsR) != 0;
         C F[4];
-        edge_flux<C, G, NS>(m, ph, Uin, grad, t_rk, sl.x & kSlotEdgeMask, cR ? sl.y : c, cR ? c : sl.y,
-                            (sl.x & kSlotBoundary) != 0, F);
+        // FP64 Euler prefetches both cells' records (the FP32 and viscous variants would lose occupancy)
+        edge_flux<C, G, NS, !NS && std::is_same<C, double>::value>(m, ph, Uin, grad, t_rk, sl.x & kSlotEdgeMask,
+                                                                   cR ? sl.y : c, cR ? c : sl.y,
+                                                                   (sl.x & kSlotBoundary) != 0, F);
         const double s = cR ? -1.0 : 1.0;   // +1 when the cell is not the edge's R
 #pragma unroll
         for (int k = 0; k < 4; k++) sF[j][k][lane] = s * (double)F[k];
