@@ -622,24 +622,30 @@ __device__ __forceinline__ double wave_max(double v) {
 
 // Spectral radius of one edge for the cells a and b (has_b false: boundary, one-sided)
 // (update_alphaC_host, CalculateDt.cpp:24-62).  ra/rb: primitives of a and b.  Returns the
-// inviscid part in *inv and the viscous part divided by V_c in *vis (0 for Euler).
-__device__ __forceinline__ void edge_alpha(const Phys &ph, const double ra[4], const double rb[4], bool has_b,
-                                           double nx, double ny, double len, double vol_c, double *inv, double *vis) {
-    const double gamma = ph.gamma;
-    double rho = ra[0], u = ra[1], v = ra[2], p = ra[3];
+// inviscid part in *inv and the viscous part divided by V_c in *vis (0 for Euler).  T: the
+// arithmetic (FP64; FP32 in mixed precision, whose dt minimum stays FP64); the viscous part is
+// formed in FP64.
+template <class T>
+__device__ __forceinline__ void edge_alpha(const Phys &ph, const T ra[4], const T rb[4], bool has_b, T nx, T ny, T len,
+                                           T vol_c, T *inv, T *vis) {
+    const T gamma = (T)ph.gamma;
+    T rho = ra[0], u = ra[1], v = ra[2], p = ra[3];
     if (has_b) {
-        rho = 0.5 * (ra[0] + rb[0]);
-        u = 0.5 * (ra[1] + rb[1]);
-        v = 0.5 * (ra[2] + rb[2]);
-        p = 0.5 * (ra[3] + rb[3]);
+        rho = T(0.5) * (ra[0] + rb[0]);
+        u = T(0.5) * (ra[1] + rb[1]);
+        v = T(0.5) * (ra[2] + rb[2]);
+        p = T(0.5) * (ra[3] + rb[3]);
     }
-    const double dx = nx * len, dy = ny * len;
-    *inv = fabs(u * dx + v * dy) + sqrt(gamma * p / rho) * sqrt(len);
-    *vis = 0.0;
+    const T dx = nx * len, dy = ny * len;
+    if (std::is_same<T, double>::value)
+        *inv = fabs(u * dx + v * dy) + sqrt(gamma * p / rho) * sqrt(len);
+    else
+        *inv = fabs(u * dx + v * dy) + fsqrt(fdiv(gamma * p, rho)) * fsqrt(len);
+    *vis = T(0);
     if (ph.equation == FVMR_EQ_NS) {
-        const double value_1 = fmax(4.0 / 3.0, gamma / ph.Pr);
-        const double T = p / ph.R / rho;
-        *vis = 2.0 * len * value_1 * mu_of_T(ph, T) / rho / vol_c;
+        const double value_1 = fmax(4.0 / 3.0, ph.gamma / ph.Pr);
+        const double Tt = (double)p / ph.R / (double)rho;
+        *vis = (T)(2.0 * (double)len * value_1 * mu_of_T(ph, Tt) / (double)rho / (double)vol_c);
     }
 }
 
@@ -1327,12 +1333,23 @@ __global__ __launch_bounds__(TileCfg<G>::kThreads) void k_tile_dt(TileArgs<G> ta
         tile_face_geom<G>(sN, fr, eg, em);
         double uL[4], rL[4], uR[4], rR[4];
         ld(sU, L, uL);
-        prim(uL, rL, ph.gamma);
         const bool has_R = R != fvmr::kLocalNone;
         ld(sU, has_R ? R : L, uR);
-        prim(uR, rR, ph.gamma);
         double av, vv;
-        if (LDT) {
+        if (!LDT && !std::is_same<G, double>::value) {
+            // mixed precision: the face's primitives and spectral radius in FP32 (the per-cell
+            // sums and the minimum in FP64)
+            G ua[4], ub[4], ra[4], rb[4], a32, v32;
+#pragma unroll
+            for (int k = 0; k < 4; k++) { ua[k] = (G)uL[k]; ub[k] = (G)uR[k]; }
+            prim(ua, ra, (G)ph.gamma);
+            prim(ub, rb, (G)ph.gamma);
+            edge_alpha<G>(ph, ra, rb, has_R, eg.x, eg.y, eg.z, G(1), &a32, &v32);
+            av = (double)a32;
+            vv = (double)v32;
+        } else if (LDT) {
+            prim(uL, rL, ph.gamma);
+            prim(uR, rR, ph.gamma);
             double r[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) r[k] = has_R ? 0.5 * (rL[k] + rR[k]) : rL[k];
@@ -1344,7 +1361,9 @@ __global__ __launch_bounds__(TileCfg<G>::kThreads) void k_tile_dt(TileArgs<G> ta
                 vv = fmax(4.0 / 3.0, ph.gamma / ph.Pr) * mu_of_T(ph, T) / r[0] * Lf * Lf;
             }
         } else {
-            edge_alpha(ph, rL, rR, has_R, (double)eg.x, (double)eg.y, (double)eg.z, 1.0, &av, &vv);
+            prim(uL, rL, ph.gamma);
+            prim(uR, rR, ph.gamma);
+            edge_alpha<double>(ph, rL, rR, has_R, (double)eg.x, (double)eg.y, (double)eg.z, 1.0, &av, &vv);
         }
         sAi[f] = av;
         sAv[f] = vv;
