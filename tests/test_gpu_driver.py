@@ -178,3 +178,25 @@ def test_driver_steady_mode(tmp_path, golden_dir):
         U = sol.get_state()
     _, Uc, _, _, _ = F.Mesh.read_continue(str(d / "pause_ivortex50[0030].dat"))
     assert np.array_equal(Uc, U)
+
+
+def test_driver_device_table(tmp_path, golden_dir):
+    """[device] keys: mixed precision, Hilbert order, tile kernels and a tile size from the autotuner."""
+    over = {"basic.continue": False, "output.maxIteration": 30, "output.step_per_print": 10,
+            "output.step_per_output_field": 1000, "output.step_per_output_hist": 1000,
+            "device": {"precision": "mixed", "renumber": "hilbert", "kernels": "tile", "tile_cells": "auto"}}
+    d = make_case(tmp_path, golden_dir, "shocktube.input.toml", "shocktube_100", over)
+    print("case: shocktube_100 via input.toml, [device] mixed / hilbert / tile / autotuned, 30 steps", flush=True)
+    drv = D.Driver(str(d / "input.toml"), log=None)
+    s = drv.run()
+    assert s["istep"] == 30
+    p = drv.params
+    assert p.precision == F.PREC_MIXED and p.renumber == F.RENUMBER_HILBERT and p.kernels == F.KERNELS_TILE
+    assert 8 <= p.tile_cells <= 400
+    m = F.Mesh.read_su2(str(d / "shocktube_100.su2"), drv.cfg.equation)
+    with F.GPUSolver2(m, p) as sol:
+        sol.set_state(F.field_init(m, p, drv.cfg.initial_type, drv.cfg.init_args()))
+        t, _ = sol.advance(30, 0.0, drv.cfg.max_physical_time)
+        U = sol.get_state()
+    _, Uc, tc, _, _ = F.Mesh.read_continue(str(d / "pause_shocktube_100[0030].dat"))
+    assert tc == t and np.array_equal(Uc, U)
