@@ -780,121 +780,126 @@ __device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[NV], const
 
 // Gradient_2 (GradientGPU.cu:513-551): GG (GradientGreenGaussKernel :223-425) or LSQ
 // (Gradient.cpp:160-246, correct inverse unless compat), then Barth (:453-502) or
-// Venkatakrishnan.  Waves 0-2 gather face j's neighbour, geometry and vertex j; wave 0
-// then forms the gradient and the limiter.  With DT the kernel also forms alphaC of the
-// cell from the same loads, summed in the reference edge order, and writes the block
-// minimum of CFL V / alphaC (get_global_dt_device, CalculateDtGPU.cu:97-103).
+// Venkatakrishnan, one thread per cell: its three slots' neighbour states, face geometry and
+// vertices are gathered into registers (all loads issued in two rounds before any is used).
+// With DT the kernel also forms alphaC of the cell from the same loads, summed in the
+// reference edge order, and writes the block minimum of CFL V / alphaC (get_global_dt_device,
+// CalculateDtGPU.cu:97-103).
 template <class C, class G, bool DT>
-__global__ __launch_bounds__(kGroup3) FVMR_GRAD_ATTR void k_gradient(Mesh<G> m, Phys ph, const S4 *__restrict__ U,
-                                                      Grad<G> *__restrict__ grad, double *__restrict__ partial) {
-    __shared__ C sUo[3][4][kCellsPerGroup];   // out-cell state of slot j (own state when none)
-    __shared__ C sA[3][3][kCellsPerGroup];    // GG: gC, Sx*s, Sy*s;  LSQ: dx, dy
-    __shared__ C sN[3][2][kCellsPerGroup];    // vertex j relative to the centroid
-    __shared__ int sFl[3][kCellsPerGroup];    // bit 0: slot has an out cell; bit 1: non-periodic neighbour
-    __shared__ double sAl[DT ? 3 : 1][4][kCellsPerGroup];  // dt: own inv, own vis, pair inv, pair vis
-    const int lane = threadIdx.x & 63, j = threadIdx.x >> 6;
+__global__ __launch_bounds__(kBlock) FVMR_GRAD_ATTR void k_gradient(Mesh<G> m, Phys ph, const S4 *__restrict__ U,
+                                                     Grad<G> *__restrict__ grad, double *__restrict__ partial) {
+    __shared__ double sMin[kBlock / 64];
     const int blk = xcd_block();
-    const int c = blk * kCellsPerGroup + lane;
+    const int c = blk * kBlock + threadIdx.x;
     const int nc = m.nc;
-    const bool live = c < nc;
-    if (live) {
-        const V4<G> cg = m.cgeo[c];
-        const C xc = (C)cg.x, yc = (C)cg.y;
-        const int2 sl = m.cslot[j * nc + c];
-        const int e = sl.x & kSlotEdgeMask;
-        const int out = sl.y;
-        const bool periodic = (sl.x & kSlotPeriodic) != 0;
-        const C sgn = (sl.x & kSlotCellIsR) ? C(-1) : C(1);
-        const int pe = periodic ? m.eint[e].w : -1;
-        double Uo[4];
-        ld(U, out != -1 ? out : c, Uo);
-#pragma unroll
-        for (int k = 0; k < 4; k++) sUo[j][k][lane] = (C)Uo[k];
-        const V2<G> nd = m.nxy[m.cnode[j * nc + c]];
-        sN[j][0][lane] = (C)nd.x - xc;
-        sN[j][1][lane] = (C)nd.y - yc;
-        const V4<G> eg = m.egeo[e];
-        if (ph.gradient == FVMR_GRA_GREENGAUSS) {
-            const C len = (C)eg.z;
-            C gC = C(0.5);
-            if (out != -1) {
-                const V2<G> em = m.emid[e];
-                const V4<G> og = m.cgeo[out];
-                const C fx = (C)em.x, fy = (C)em.y;
-                C Fx = (C)og.x, Fy = (C)og.y;
-                if (periodic && pe >= 0) {
-                    const V2<G> pm = m.emid[pe];
-                    Fx += fx - (C)pm.x;
-                    Fy += fy - (C)pm.y;
-                }
-                const C ax = fx - Fx, ay = fy - Fy, bx = xc - Fx, by = yc - Fy;
-                const C dFf = fsqrt(ax * ax + ay * ay);
-                C dFC = fsqrt(bx * bx + by * by);
-                if (dFC < C(fvmr::kEPS)) dFC = C(fvmr::kEPS);
-                gC = fdiv(dFf, dFC);
-            }
-            sA[j][0][lane] = gC;
-            sA[j][1][lane] = (C)eg.x * len * sgn;
-            sA[j][2][lane] = (C)eg.y * len * sgn;
-        } else {
-            const int nbr = periodic ? -1 : out;   // element neighbours exclude periodic faces
-            C dxn = C(0), dyn = C(0);
-            if (nbr != -1) {
-                const V4<G> og = m.cgeo[nbr];
-                dxn = (C)og.x - xc;
-                dyn = (C)og.y - yc;
-            }
-            sA[j][0][lane] = dxn;
-            sA[j][1][lane] = dyn;
-        }
-        sFl[j][lane] = ((out != -1) ? 1 : 0) | ((out != -1 && !periodic) ? 2 : 0);
-        if (DT) {
-            double Uc[4], rc[4], ro[4];
-            ld(U, c, Uc);
-            prim(Uc, rc, ph.gamma);
-            prim(Uo, ro, ph.gamma);
-            const double vol_c = (double)cg.z;
-            double a0, v0, a1 = 0.0, v1 = 0.0;
-            edge_alpha(ph, rc, ro, out != -1, (double)eg.x, (double)eg.y, (double)eg.z, vol_c, &a0, &v0);
-            if (pe >= 0) {
-                const V4<G> pg = m.egeo[pe];
-                edge_alpha(ph, rc, ro, true, (double)pg.x, (double)pg.y, (double)pg.z, vol_c, &a1, &v1);
-            }
-            sAl[j][0][lane] = a0;
-            sAl[j][1][lane] = v0;
-            sAl[j][2][lane] = a1;
-            sAl[j][3][lane] = v1;
-        }
-    }
-    __syncthreads();
-    if (j != 0) return;
     double dtc = 1e10;
-    if (live) {
+    if (c < nc) {
+        // the loads in two rounds: the cell's own records, then what they point to
+        const V4<G> cg = m.cgeo[c];
+        int2 sl[3];
+        int cn[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            sl[j] = m.cslot[j * nc + c];
+            cn[j] = m.cnode[j * nc + c];
+        }
         double Ud[4];
         ld(U, c, Ud);
+        double Uo[3][4];
+        V2<G> nd[3], em[3];
+        V4<G> eg[3], og[3];
+        int pe[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int e = sl[j].x & kSlotEdgeMask, out = sl[j].y;
+            const int oc = out != -1 ? out : c;
+            ld(U, oc, Uo[j]);
+            nd[j] = m.nxy[cn[j]];
+            eg[j] = m.egeo[e];
+            em[j] = m.emid[e];
+            og[j] = m.cgeo[oc];
+            pe[j] = (sl[j].x & kSlotPeriodic) ? m.eint[e].w : -1;
+        }
+        const C xc = (C)cg.x, yc = (C)cg.y;
+        struct {
+            C a[3][3], u[3][4], n[3][2];
+            __device__ __forceinline__ C A(int i, int q) const { return a[i][q]; }
+            __device__ __forceinline__ C Uo(int i, int k) const { return u[i][k]; }
+            __device__ __forceinline__ C N(int i, int q) const { return n[i][q]; }
+        } S;
+        int fl[3];
+        double al[DT ? 3 : 1][4];
+        double rc[4];
+        if (DT) prim(Ud, rc, ph.gamma);
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int out = sl[j].y;
+            const bool periodic = (sl[j].x & kSlotPeriodic) != 0;
+            const C sgn = (sl[j].x & kSlotCellIsR) ? C(-1) : C(1);
+#pragma unroll
+            for (int k = 0; k < 4; k++) S.u[j][k] = (C)Uo[j][k];
+            S.n[j][0] = (C)nd[j].x - xc;
+            S.n[j][1] = (C)nd[j].y - yc;
+            if (ph.gradient == FVMR_GRA_GREENGAUSS) {
+                const C len = (C)eg[j].z;
+                C gC = C(0.5);
+                if (out != -1) {
+                    const C fx = (C)em[j].x, fy = (C)em[j].y;
+                    C Fx = (C)og[j].x, Fy = (C)og[j].y;
+                    if (periodic && pe[j] >= 0) {
+                        const V2<G> pm = m.emid[pe[j]];
+                        Fx += fx - (C)pm.x;
+                        Fy += fy - (C)pm.y;
+                    }
+                    const C ax = fx - Fx, ay = fy - Fy, bx = xc - Fx, by = yc - Fy;
+                    const C dFf = fsqrt(ax * ax + ay * ay);
+                    C dFC = fsqrt(bx * bx + by * by);
+                    if (dFC < C(fvmr::kEPS)) dFC = C(fvmr::kEPS);
+                    gC = fdiv(dFf, dFC);
+                }
+                S.a[j][0] = gC;
+                S.a[j][1] = (C)eg[j].x * len * sgn;
+                S.a[j][2] = (C)eg[j].y * len * sgn;
+            } else {
+                const int nbr = periodic ? -1 : out;   // element neighbours exclude periodic faces
+                C dxn = C(0), dyn = C(0);
+                if (nbr != -1) {
+                    dxn = (C)og[j].x - xc;
+                    dyn = (C)og[j].y - yc;
+                }
+                S.a[j][0] = dxn;
+                S.a[j][1] = dyn;
+                S.a[j][2] = C(0);
+            }
+            fl[j] = ((out != -1) ? 1 : 0) | ((out != -1 && !periodic) ? 2 : 0);
+            if (DT) {
+                double ro[4];
+                prim(Uo[j], ro, ph.gamma);
+                const double vol_c = (double)cg.z;
+                double a0, v0, a1 = 0.0, v1 = 0.0;
+                edge_alpha(ph, rc, ro, out != -1, (double)eg[j].x, (double)eg[j].y, (double)eg[j].z, vol_c, &a0, &v0);
+                if (pe[j] >= 0) {
+                    const V4<G> pg = m.egeo[pe[j]];
+                    edge_alpha(ph, rc, ro, true, (double)pg.x, (double)pg.y, (double)pg.z, vol_c, &a1, &v1);
+                }
+                al[DT ? j : 0][0] = a0;
+                al[DT ? j : 0][1] = v0;
+                al[DT ? j : 0][2] = a1;
+                al[DT ? j : 0][3] = v1;
+            }
+        }
         C Uc[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) Uc[k] = (C)Ud[k];
-        const V4<G> cg = m.cgeo[c];
-        int fl[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) fl[i] = sFl[i][lane];
-        struct {
-            const C (*a)[3][kCellsPerGroup], (*u)[4][kCellsPerGroup], (*n)[2][kCellsPerGroup];
-            int lane;
-            __device__ __forceinline__ C A(int i, int q) const { return a[i][q][lane]; }
-            __device__ __forceinline__ C Uo(int i, int k) const { return u[i][k][lane]; }
-            __device__ __forceinline__ C N(int i, int q) const { return n[i][q][lane]; }
-        } slots{sA, sUo, sN, lane};
         C gx[4], gy[4];
-        grad_limit<C>(ph, Uc, slots, fl, (C)cg.z, gx, gy);
-        Grad<G> out;
+        grad_limit<C>(ph, Uc, S, fl, (C)cg.z, gx, gy);
+        Grad<G> o;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            out.x[k] = (G)gx[k];
-            out.y[k] = (G)gy[k];
+            o.x[k] = (G)gx[k];
+            o.y[k] = (G)gy[k];
         }
-        grad[c] = out;
+        grad[c] = o;
         if (DT) {
             // alphaC in the reference edge order: count (3 bits) + 6 x (slot 2 bits, pair 1 bit)
             const int code = m.dtcode[c];
@@ -903,15 +908,28 @@ __global__ __launch_bounds__(kGroup3) FVMR_GRAD_ATTR void k_gradient(Mesh<G> m, 
             for (int q = 0; q < cnt; q++) {
                 const int ent = (code >> (3 + 3 * q)) & 7;
                 const int slot = ent & 3, pr = ent >> 2;
-                alpha_c += sAl[slot][2 * pr][lane];
-                if (ph.equation == FVMR_EQ_NS) alpha_c += sAl[slot][2 * pr + 1][lane];
+                double av = 0.0, vv = 0.0;
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    if (slot == j) {
+                        av = al[DT ? j : 0][2 * pr];
+                        vv = al[DT ? j : 0][2 * pr + 1];
+                    }
+                alpha_c += av;
+                if (ph.equation == FVMR_EQ_NS) alpha_c += vv;
             }
             dtc = ph.bc.CFL * (double)cg.z / alpha_c;
         }
     }
     if (DT) {
-        const double bmin = wave_min(dtc);
-        if (lane == 0) partial[blk] = bmin;
+        const double wm = wave_min(dtc);
+        if ((threadIdx.x & 63) == 0) sMin[threadIdx.x >> 6] = wm;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double b = sMin[0];
+            for (int w = 1; w < kBlock / 64; w++) b = fmin(b, sMin[w]);
+            partial[blk] = b;
+        }
     }
 }
 
@@ -1874,16 +1892,16 @@ int launch_gradient(fvmr_solver *h, const S4 *U, int dt_mode) {
     int rc = prof_begin(h, 1, &e0);
     if (rc) return rc;
     if (dt_mode >= 0)
-        k_gradient<C, G, true><<<grid3_of(h->nc), kGroup3, 0, h->stream>>>(mesh_args<G>(h), h->ph, U,
+        k_gradient<C, G, true><<<grid_of(h->nc), kBlock, 0, h->stream>>>(mesh_args<G>(h), h->ph, U,
                                                                           (Grad<G> *)h->grad, h->partial);
     else
-        k_gradient<C, G, false><<<grid3_of(h->nc), kGroup3, 0, h->stream>>>(mesh_args<G>(h), h->ph, U,
+        k_gradient<C, G, false><<<grid_of(h->nc), kBlock, 0, h->stream>>>(mesh_args<G>(h), h->ph, U,
                                                                            (Grad<G> *)h->grad, h->partial);
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 1, e0))) return rc;
     if (dt_mode < 0) return FVMR_OK;
     if ((rc = prof_begin(h, 2, &e0))) return rc;
-    k_dt_finish<<<1, 1024, 0, h->stream>>>(grid3_of(h->nc), h->partial, h->sc, dt_mode);
+    k_dt_finish<<<1, 1024, 0, h->stream>>>(grid_of(h->nc), h->partial, h->sc, dt_mode);
     HIP_OK(hipGetLastError());
     return prof_end(h, 2, e0);
 }
