@@ -571,18 +571,27 @@ __global__ __launch_bounds__(kAdjBlock, FVMR_ADJ_WAVES) void k_adj_vjp(Mesh<doub
     double ub[4] = {0, 0, 0, 0}, gbx[4] = {0, 0, 0, 0}, gby[4] = {0, 0, 0, 0};
     int outs[3], edges[3];
     bool isR[3];
+    // the three slots' records first, then their midpoints and face adjoints, before any is used
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         const int2 sl = m.cslot[j * nc + c];
-        const int e = sl.x & kSlotEdgeMask;
-        const bool cR = (sl.x & kSlotCellIsR) != 0;
         outs[j] = sl.y;
-        edges[j] = e;
-        isR[j] = cR;
-        const V2<double> em = m.emid[e];
-        double Uf[4], u4[4];
+        edges[j] = sl.x & kSlotEdgeMask;
+        isR[j] = (sl.x & kSlotCellIsR) != 0;
+    }
+    V2<double> ems[3];
+    double u4s[3][4];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        ems[j] = m.emid[edges[j]];
+        ld(fadj, 2 * edges[j] + (isR[j] ? 1 : 0), u4s[j]);   // the adjoint of this cell's face state on e
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const V2<double> em = ems[j];
+        double Uf[4];
+        const double *u4 = u4s[j];
         const bool lin = recon_state(ph, Uc, gc, cg.x, cg.y, em.x, em.y, Uf);
-        ld(fadj, 2 * e + (cR ? 1 : 0), u4);   // the adjoint of this cell's face state on e
         const double dx = em.x - cg.x, dy = em.y - cg.y;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
