@@ -47,23 +47,26 @@ def build_library(verbose=True, defines=(), lib_path=None):
     return lib_path
 
 
-C_CALLER_SRC = os.path.join(os.path.dirname(HERE), "tests", "c_api", "fvmr_c_caller.cpp")
-C_CALLER = os.path.join(HERE, "_build", "fvmr_c_caller")
+C_API_DIR = os.path.join(os.path.dirname(HERE), "tests", "c_api")
+C_CALLER = os.path.join(HERE, "_build", "fvmr_c_caller")      # include/fvmr.h only
+CPP_MIRROR = os.path.join(HERE, "_build", "fvmr_cpp_mirror")  # include/fvmr.hpp
 
 
-def build_c_caller(out=None, extra=()):
-    """Compiles the C++ caller of the C ABI (tests/c_api) against include/fvmr.h and libfvmr.so;
-    it finds the library beside its own directory (rpath $ORIGIN/..)."""
-    out = os.path.abspath(out or C_CALLER)
+def build_c_caller(out=None, name="fvmr_c_caller"):
+    """Compiles a caller of the C ABI from tests/c_api (fvmr_c_caller: include/fvmr.h only;
+    fvmr_cpp_mirror: include/fvmr.hpp) against libfvmr.so; it finds the library from its own
+    directory (rpath $ORIGIN/...)."""
+    out = os.path.abspath(out or os.path.join(HERE, "_build", name))
     os.makedirs(os.path.dirname(out), exist_ok=True)
     rpath = "$ORIGIN/" + os.path.relpath(HERE, os.path.dirname(out))
-    _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", *extra,
-          "-I" + os.path.join(os.path.dirname(HERE), "include"), C_CALLER_SRC, "-L" + HERE, "-lfvmr",
-          "-Wl,-rpath," + rpath, "-o", out])
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror",
+          "-I" + os.path.join(os.path.dirname(HERE), "include"), os.path.join(C_API_DIR, name + ".cpp"),
+          "-L" + HERE, "-lfvmr", "-Wl,-rpath," + rpath, "-o", out])
     return out
 
 
 if __name__ == "__main__":
     build_library()
     build_c_caller()
+    build_c_caller(name="fvmr_cpp_mirror")
     sys.exit(0)

@@ -1,7 +1,9 @@
-"""The C ABI from C++: tests/c_api/fvmr_c_caller.cpp includes only include/fvmr.h and links
-libfvmr.so, in the order the reference's host code drives its solver.  On the CPU it must
-compile cleanly and its host-only part (SU2 mesh, shock-tube field) must agree with the Python
-path bit for bit."""
+"""The C ABI from C++: tests/c_api/fvmr_c_caller.cpp includes only include/fvmr.h and
+tests/c_api/fvmr_cpp_mirror.cpp the C++ layer include/fvmr.hpp (GPU::GPUSolver2,
+CIteration::iteration_device); both link libfvmr.so and drive it in the order the reference's
+host code drives its solver.  On the CPU they must compile cleanly and the host-only part (SU2
+mesh, shock-tube field) must agree with the Python path bit for bit; on the GPU both runs must
+equal the Python path's."""
 import os
 import subprocess
 
@@ -9,7 +11,7 @@ import numpy as np
 import pytest
 
 import cfd_solver_2d_gpu_amd as F
-from cfd_solver_2d_gpu_amd.build import build_c_caller
+from cfd_solver_2d_gpu_amd.build import CPP_MIRROR, C_CALLER, build_c_caller
 
 MASK = (1 << 64) - 1
 
@@ -43,6 +45,12 @@ def test_c_caller_compiles_and_matches_host_path(tmp_path, golden_dir):
     assert int(w[9], 16) == state_fold(U)
 
 
+def test_cpp_mirror_compiles(tmp_path):
+    exe = build_c_caller(out=str(tmp_path / "fvmr_cpp_mirror"), name="fvmr_cpp_mirror")
+    r = subprocess.run([exe, str(tmp_path / "missing.su2"), "1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and r.stderr.startswith("fvmr_mesh_read_su2: ")
+
+
 def test_c_caller_reports_errors(tmp_path):
     exe = build_c_caller(out=str(tmp_path / "fvmr_c_caller"))
     r = subprocess.run([exe, str(tmp_path / "missing.su2"), "0", "--host-only"], capture_output=True, text=True,
@@ -54,14 +62,17 @@ def test_c_caller_reports_errors(tmp_path):
 def test_c_caller_steps_match_python(golden_dir):
     """The prebuilt caller (built with the library) runs 20 steps on the device; the Python path
     on the same inputs gives the same t, residual and state bit for bit."""
-    from cfd_solver_2d_gpu_amd.build import C_CALLER
-    if not os.path.exists(C_CALLER):
-        raise RuntimeError(f"{C_CALLER} is missing: build() compiles it with the library")
+    for exe in (C_CALLER, CPP_MIRROR):
+        if not os.path.exists(exe):
+            raise RuntimeError(f"{exe} is missing: build() compiles it with the library")
     mesh = str(golden_dir / "shocktube_100.su2")
-    print("case: C++ caller, shocktube_100, 20 SSP-RK3 steps", flush=True)
+    print("case: C++ callers, shocktube_100, 20 SSP-RK3 steps", flush=True)
     r = subprocess.run([C_CALLER, mesh, "20"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     w = r.stdout.split()
+    rm = subprocess.run([CPP_MIRROR, mesh, "20"], capture_output=True, text=True, timeout=120)
+    assert rm.returncode == 0, rm.stderr
+    wm = rm.stdout.split()
     m = F.Mesh.read_su2(mesh)
     p = shocktube_params()
     with F.GPUSolver2(m, p) as g:
@@ -71,5 +82,8 @@ def test_c_caller_steps_match_python(golden_dir):
             t, _ = g.iteration_device(t, 1e9)
         res = g.updateResidualVector(F.NORM_INF)
         U = g.get_state()
+        ruvp, _, _ = g.node_field()
     assert float(w[1]) == t and float(w[3]) == res[0]
     assert int(w[5], 16) == state_fold(U)
+    assert wm[:6] == w[:6]
+    assert float(wm[7]) == pytest.approx(float(np.mean(ruvp[0])), rel=1e-13)
