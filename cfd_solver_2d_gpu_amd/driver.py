@@ -10,6 +10,7 @@ Mirrors the reference's input side and main loop:
                   (ContinueFileReader.cpp:19-62) and COutput::updateFileName (COutput.cpp:5-20)
   write_tecplot_* FieldWriter::write_tecplot_volume_file / _boundary_file / _hist_file
                   (FieldWriter.cpp:44-172,207-262,504-594)
+  run_batch       several cases on one batch handle (no reference counterpart)
 
 Keys are read with the reference's rules: a required key that is missing or has the wrong type
 is an error, all errors of a file are collected and raised together (ConfigError); a float key
@@ -29,7 +30,7 @@ import numpy as np
 import tomli
 
 from . import _abi as F
-from . import GPUSolver2, Mesh, autotune, default_params, field_init, ruvp_from_mach
+from . import BatchSolver, GPUSolver2, Mesh, autotune, default_params, field_init, ruvp_from_mach
 
 EPSILON = 1e-10          # U2NITS::Math::EPSILON (FVMR/src/math/CommonValue.h:21): steady-state stop
 SNAPSHOT_SECONDS = 1800.0  # CDriver::m_snapshot_cold_time (FVMR/src/drivers/CDriver.h:34)
@@ -690,13 +691,84 @@ def run_case(config_path, max_steps=None, log=print):
     return Driver(config_path, log=log).run(max_steps=max_steps)
 
 
+# scheme settings every case of a batch shares (fvmr_create_batch: one kernel configuration)
+_BATCH_SHARED = ("gamma", "R", "Pr", "Pr_viscous", "T0", "mu0", "equation", "flux_scheme", "reconstruct", "gradient",
+                 "limiter", "time_advance", "viscosity_model", "precision", "renumber", "compat_inv2x2", "kernels",
+                 "tile_cells", "wall_temperature", "venkat_K", "dsr_shock_x", "dsr_shock_y", "dsr_shock_speed")
+
+
+def run_batch(config_paths, log=print):
+    """Several input.toml cases on one batch handle (fvmr_create_batch): each case keeps its
+    own mesh, field (or newest continue file), boundary states, CFL, end time and maxIteration;
+    the cases must agree on the scheme settings.  All cases step together; each stops at its own
+    maxIteration (or when t passes its max_physical_time), where its continue file
+    pause_<filename>[%04d].dat is written as the single-case driver writes it at that stop.
+    Output steps (Tecplot, hist, print residuals) and the residual stop rule are the
+    single-case driver's (Driver.run).  Returns one summary per case."""
+    log = log if log is not None else (lambda *a, **k: None)
+    drivers = [Driver(p, log=None) for p in config_paths]
+    if not drivers:
+        raise ConfigError("run_batch needs at least one input.toml")
+    for d in drivers:
+        if d.cfg.is_steady:
+            raise ConfigError(f"{d.config_path}: a batch runs unsteady cases only")
+        if d.cfg.tile_cells == -1:
+            raise ConfigError(f"{d.config_path}: give device.tile_cells a number for a batch")
+        d.read_field()
+    p0 = drivers[0].params
+    for d in drivers[1:]:
+        diff = [k for k in _BATCH_SHARED if getattr(d.params, k) != getattr(p0, k)]
+        if diff:
+            raise ConfigError(f"{d.config_path}: a batch needs the same {', '.join(diff)} as {drivers[0].config_path}")
+    n = len(drivers)
+    bs = BatchSolver([d.mesh for d in drivers], [d.params for d in drivers])
+    try:
+        for c, d in enumerate(drivers):
+            bs.set_state(c, d.U)
+        t = np.array([d.t0 for d in drivers], dtype=np.float64)
+        T = np.array([d.cfg.max_physical_time for d in drivers], dtype=np.float64)
+        istep = [d.istep0 for d in drivers]
+        # the reference takes at least one step before it tests istep >= maxIteration
+        left = [max(1, d.cfg.maxIteration - d.istep0) for d in drivers]
+        out = [None] * n
+        clock0 = time.perf_counter()
+        while any(o is None for o in out):
+            k = min(left[c] for c in range(n) if out[c] is None)
+            t_new = bs.advance(k, t, T)
+            for c in range(n):
+                if out[c] is not None:
+                    continue
+                istep[c] += k
+                left[c] -= k
+                t[c] = t_new[c]
+                if left[c] <= 0 or t[c] > T[c]:
+                    d = drivers[c]
+                    path = os.path.join(d.dir, "pause_" + d.cfg.filename + _step_name(istep[c]) + ".dat")
+                    d.mesh.write_continue(path, bs.get_state(c), t[c], istep[c], d.params.CFL)
+                    out[c] = {"signal": "TimeReached", "istep": istep[c], "t": float(t[c]), "files": [path]}
+                    log(f"case {c} ({d.cfg.filename}): stop at step {istep[c]}, t = {t[c]:.6e}")
+            # finished cases ride along with dt = 0 until the last case stops
+            for c in range(n):
+                if out[c] is not None:
+                    T[c] = t[c]
+        secs = time.perf_counter() - clock0
+        for o in out:
+            o["seconds"] = secs
+        return out
+    finally:
+        bs.close()
+
+
 def main(argv=None):
     import argparse
-    ap = argparse.ArgumentParser(description="Run an FVMR input.toml case on the GPU")
-    ap.add_argument("config", help="path of input.toml; the mesh and outputs are in its directory")
-    ap.add_argument("--max-steps", type=int, default=None)
+    ap = argparse.ArgumentParser(description="Run FVMR input.toml cases on the GPU (several: one batch)")
+    ap.add_argument("config", nargs="+", help="input.toml paths; each case's mesh and outputs are in its directory")
+    ap.add_argument("--max-steps", type=int, default=None, help="one case only: cap the steps of this run")
     a = ap.parse_args(argv)
-    s = run_case(a.config, a.max_steps)
+    if len(a.config) > 1:
+        run_batch(a.config)
+        return 0
+    s = run_case(a.config[0], a.max_steps)
     return 0 if s["signal"] in ("TimeReached", "StableReached", "StepsDone") else 1
 
 

@@ -425,3 +425,75 @@ def test_loop_continue_and_max_steps(tmp_path, golden_dir, monkeypatch):
     # the hist file of a continued run gets no second header
     hist = (d / "shocktube_100_hist.dat").read_text().splitlines()
     assert sum(1 for l in hist if l.startswith("title=")) == 1
+
+
+class FakeBatch:
+    """Stand-in for the batch handle: constant dt per case, state counts steps."""
+
+    def __init__(self, meshes, params):
+        self.U = [None] * len(meshes)
+        self.dt = [0.001 * (c + 1) for c in range(len(meshes))]
+        self.calls = []
+
+    def set_state(self, c, U):
+        self.U[c] = np.array(U, dtype=float)
+
+    def get_state(self, c):
+        return self.U[c].copy()
+
+    def advance(self, n, t, T):
+        self.calls.append(n)
+        t = np.array(t, dtype=float)
+        for c in range(len(t)):
+            for _ in range(n):
+                d = self.dt[c] if t[c] + self.dt[c] <= T[c] else T[c] - t[c]
+                t[c] += d
+                self.U[c][0] += 1.0 if d > 0 else 0.0
+        return t
+
+    def close(self):
+        pass
+
+
+def test_run_batch_stops_each_case(tmp_path, golden_dir, monkeypatch):
+    dirs = []
+    for name, mx in (("a", 30), ("b", 75), ("c", 50)):
+        d = tmp_path / name
+        d.mkdir()
+        shutil.copyfile(golden_dir / "shocktube_100.su2", d / "shocktube_100.su2")
+        t = _tree("shocktube.input.toml")
+        _set(t, "basic.continue", False)
+        _set(t, "output.maxIteration", mx)
+        if name == "b":
+            _set(t, "boundaryCondition.2D.inlet.rho", 2.0)
+            _set(t, "time.CFL", 1.5)
+        write_toml(d / "input.toml", t)
+        dirs.append(d)
+    monkeypatch.setattr(D, "BatchSolver", FakeBatch)
+    out = D.run_batch([str(d / "input.toml") for d in dirs], log=None)
+    assert [o["istep"] for o in out] == [30, 75, 50]
+    assert [o["t"] for o in out] == pytest.approx([30 * 0.001, 75 * 0.002, 50 * 0.003])
+    for d, o in zip(dirs, out):
+        assert os.path.basename(o["files"][0]) == f"pause_shocktube_100[{o['istep']:04d}].dat"
+        _, U, t, istep, cfl = F.Mesh.read_continue(o["files"][0])
+        assert istep == o["istep"] and t == pytest.approx(o["t"])
+        assert cfl == (1.5 if d.name == "b" else 2.5)
+        cfg = D.read_config(str(d / "input.toml"))
+        m = F.Mesh.read_su2(str(d / "shocktube_100.su2"))
+        U0 = F.field_init(m, cfg.params(), cfg.initial_type, cfg.init_args())
+        assert np.array_equal(U[0], U0[0] + o["istep"])   # stepped exactly to its stop (density + 1 per step)
+
+
+def test_run_batch_refuses_mixed_schemes(tmp_path, golden_dir):
+    dirs = []
+    for name, lim in (("a", 1), ("b", 0)):
+        d = tmp_path / name
+        d.mkdir()
+        shutil.copyfile(golden_dir / "shocktube_100.su2", d / "shocktube_100.su2")
+        t = _tree("shocktube.input.toml")
+        _set(t, "basic.continue", False)
+        _set(t, "inviscid_flux_method.flux_limiter", lim)
+        write_toml(d / "input.toml", t)
+        dirs.append(d)
+    with pytest.raises(D.ConfigError, match="limiter"):
+        D.run_batch([str(d / "input.toml") for d in dirs], log=None)

@@ -200,3 +200,28 @@ def test_driver_device_table(tmp_path, golden_dir):
         U = sol.get_state()
     _, Uc, tc, _, _ = F.Mesh.read_continue(str(d / "pause_shocktube_100[0030].dat"))
     assert tc == t and np.array_equal(Uc, U)
+
+
+def test_driver_batch_matches_single_cases(tmp_path, golden_dir):
+    """run_batch: two shock-tube cases of different shock position, inlet state, CFL and stop step
+    on one batch handle end in the states the single-case driver reaches, bit for bit."""
+    specs = {"a": {"output.maxIteration": 30},
+             "b": {"output.maxIteration": 45, "time.CFL": 1.5, "boundaryCondition.2D.inlet.rho": 2.0,
+                   "initialCondition.shockTube.shock_x": 0.2}}
+    base = {"basic.continue": False, "output.step_per_print": 1000, "output.step_per_output_field": 1000,
+            "output.step_per_output_hist": 1000}
+    batch_dirs, single_dirs = [], []
+    for name, over in specs.items():
+        batch_dirs.append(make_case(tmp_path, golden_dir, "shocktube.input.toml", "shocktube_100", {**base, **over},
+                                    name="batch_" + name))
+        single_dirs.append(make_case(tmp_path, golden_dir, "shocktube.input.toml", "shocktube_100", {**base, **over},
+                                     name="single_" + name))
+    print("case: run_batch of two shock tubes (30 and 45 steps) vs single-case runs", flush=True)
+    out = D.run_batch([str(d / "input.toml") for d in batch_dirs], log=None)
+    assert [o["istep"] for o in out] == [30, 45]
+    for o, sd in zip(out, single_dirs):
+        s = D.Driver(str(sd / "input.toml"), log=None).run()
+        assert s["istep"] == o["istep"] and s["t"] == o["t"]
+        _, Ub, tb, _, _ = F.Mesh.read_continue(o["files"][0])
+        _, Us, ts, _, _ = F.Mesh.read_continue(str(sd / f"pause_shocktube_100[{o['istep']:04d}].dat"))
+        assert tb == ts and np.array_equal(Ub, Us)
