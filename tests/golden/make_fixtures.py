@@ -28,7 +28,31 @@ CONFIGS = [
     ("2D_isentropic_vortex/input.toml", "ivortex.input.toml"),
 ]
 
+# the reference's log of its own GPU run of the quick_start shock tube (FP32, CFL 2.5, 1000 steps)
+LOG = "/root/reference/FVMR/working_directory/2024-10-17-142244.LOG"
+
+
+def log_residuals():
+    """(step, residual_rho) of the log's progress lines (CDriver.cpp:119-124)."""
+    import re
+    out = []
+    for line in open(LOG, encoding="utf-8", errors="replace"):
+        m = re.search(r"step = (\d+)\tresidual_rho = (\S+)\tCFL = (\S+)", line)
+        if m:
+            out.append((int(m.group(1)), m.group(2), m.group(3)))
+    return out
+
+
 if __name__ == "__main__":
+    import json
+    rows = log_residuals()
+    with open(os.path.join(HERE, "shocktube_100_reference_log.json"), "w") as f:
+        json.dump({"source": "FVMR/working_directory/2024-10-17-142244.LOG (progress lines)",
+                   "mesh": "shocktube_100.su2", "config": "shocktube.input.toml",
+                   "steps": [r[0] for r in rows], "residual_rho": [r[1] for r in rows],
+                   "CFL": [r[2] for r in rows]}, f, indent=1)
+        f.write("\n")
+    print("shocktube_100_reference_log.json", len(rows))
     for src, dst in CONFIGS:
         shutil.copyfile(os.path.join(REF, src), os.path.join(HERE, dst))
         print(dst, os.path.getsize(os.path.join(HERE, dst)))

@@ -225,3 +225,39 @@ def test_driver_batch_matches_single_cases(tmp_path, golden_dir):
         _, Ub, tb, _, _ = F.Mesh.read_continue(o["files"][0])
         _, Us, ts, _, _ = F.Mesh.read_continue(str(sd / f"pause_shocktube_100[{o['istep']:04d}].dat"))
         assert tb == ts and np.array_equal(Ub, Us)
+
+
+def test_driver_shocktube_against_reference_log(tmp_path, golden_dir):
+    """The quick_start shock tube as its input.toml sets it (forward Euler, CFL 2.5, 1000 steps)
+    against the progress lines of the reference's own GPU run of that case (its log, FP32,
+    CDriver.cpp:115-125).  Step 1 is the rhs of the initial state: the density residual agrees to
+    the six printed digits.  Later steps take other dt: the reference's dt kernel adds each face's
+    term into both cells with unsynchronised read-modify-writes (CalculateDtGPU.cu:47-49), which
+    lose terms and give a larger dt, so it reaches T sooner (between steps 500 and 600; this build
+    at 758).  After T every step has dt = 0 and the residual is that of the state at T, which a
+    different dt history moves by a few per cent: 3.0014 against 3.07145."""
+    import json
+    import re
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "shocktube_100_reference_log.json")))
+    ref = dict(zip(ref["steps"], map(float, ref["residual_rho"])))
+    d = make_case(tmp_path, golden_dir, "shocktube.input.toml", "shocktube_100",
+                  {"basic.continue": False, "output.step_per_output_field": 100000,
+                   "output.step_per_output_hist": 100000, "output.step_per_print": 1})
+    print("case: shocktube_100 via the reference's input.toml, 1000 steps", flush=True)
+    lines = []
+    s = D.Driver(str(d / "input.toml"), log=lines.append).run()
+    T = toml_tree("shocktube.input.toml")["time"]["max_physical_time"]
+    res, t_of = {}, {}
+    for l in lines:
+        m = re.match(r"step = (\d+)\tt = (\S+)\tresidual_rho = (\S+)", l)
+        if m:
+            res[int(m.group(1))], t_of[int(m.group(1))] = float(m.group(3)), float(m.group(2))
+    assert s["istep"] == 1000 and abs(s["t"] - T) <= 1e-12 * T
+    assert set(ref) <= set(res)
+    assert abs(res[1] / ref[1] - 1) < 5e-6
+    for k in (100, 200, 300, 400, 500):
+        assert abs(res[k] / ref[k] - 1) < 0.25, (k, res[k], ref[k])
+    step_T = min(k for k, tt in t_of.items() if tt >= T * (1 - 1e-12))
+    assert 600 < step_T < 800
+    for k in (800, 900, 1000):
+        assert abs(res[k] / res[step_T + 1] - 1) < 1e-9 and abs(res[k] / ref[k] - 1) < 0.05, (k, res[k], ref[k])

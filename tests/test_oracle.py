@@ -1,7 +1,10 @@
 """Pins the CPU oracle with closed forms, invariants and textbook special cases, each
 chosen so that a plausible slip (dropped term, wrong sign or index, transposed operand)
-fails.  The reference ships no tests or golden vectors for this path; the reference-held
-data files are used in test_mesh.py and the GPU parity tests."""
+fails.  The reference ships no tests or golden vectors for this path; the one number its own
+files give on its own inputs (a logged step-1 residual of the shock tube) is checked last, and
+the reference-held data files are used in test_mesh.py and the GPU parity tests."""
+import os
+
 import numpy as np
 import pytest
 
@@ -258,3 +261,25 @@ def test_forward_euler_is_first_order_in_time():
     ref = run(0.02)
     r = np.linalg.norm(run(0.4) - ref) / np.linalg.norm(run(0.2) - ref)
     assert 1.6 < r < 2.6, r
+
+
+def test_shocktube_first_residual_matches_reference_log(golden_dir):
+    """The one number the reference's own files give for this path on its own inputs: the density
+    residual (infinity norm of dU/dt) of step 1 of its GPU run of the quick_start shock tube,
+    logged as 20.7896 (FVMR/working_directory/2024-10-17-142244.LOG, CDriver.cpp:115-125).  Step 1
+    is the rhs of the initial state, so it pins the oracle's reconstruction, boundary ghosts and
+    Roe flux on the real mesh and the case's own input.toml, to the six printed digits."""
+    import json
+    from cfd_solver_2d_gpu_amd import driver as D
+    from helpers import toml_tree
+    with open(os.path.join(os.path.dirname(__file__), "golden", "shocktube_100_reference_log.json")) as f:
+        log = json.load(f)
+    assert log["steps"][0] == 1
+    cfg = D.config_from_tree(toml_tree("shocktube.input.toml"))
+    m = O.OracleMesh.read_su2(golden_dir / "shocktube_100.su2", cfg.equation, cfg.mesh_scale_factor)
+    p = cfg.params()
+    s = O.OracleSolver(m, p)
+    s.set_state(O.field_init(m.soa, p, cfg.initial_type, cfg.init_args()))
+    s.step(0.0, cfg.max_physical_time)
+    r = s.residual(F.NORM_INF)[0]
+    assert abs(r / float(log["residual_rho"][0]) - 1) < 5e-6, r
