@@ -17,6 +17,8 @@ FILES = [
     ("2D_cylinder/cylinder_allfar_50.su2", "cylinder_allfar_50.su2.gz"),
     ("2D_cylinder/pause_cylinder_allfar_50[20000].dat", "pause_cylinder_allfar_50_20000.dat.gz"),
     ("1D_shocktube/shocktube_100.su2", "shocktube_100.su2.gz"),
+    ("2D_Neuvillette_shock/neuvillette.su2", "neuvillette.su2.gz"),
+    ("2D_double_shock_reflect/dsr131new_slippery.su2", "dsr131new_slippery.su2.gz"),
 ]
 
 # the quick_start cases' input.toml files, copied as they are (config data for the driver tests)
@@ -33,7 +35,7 @@ LOG = "/root/reference/FVMR/working_directory/2024-10-17-142244.LOG"
 
 
 def log_residuals():
-    """(step, residual_rho) of the log's progress lines (CDriver.cpp:119-124)."""
+    """(step, residual_rho) of the log's progress lines (CDriver.cpp:115-125)."""
     import re
     out = []
     for line in open(LOG, encoding="utf-8", errors="replace"):

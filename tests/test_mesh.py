@@ -19,7 +19,8 @@ def same_arrays(a, b):
 
 
 @pytest.mark.parametrize("name,eq", [("ivortex50.su2", F.EQ_EULER), ("cylinder_allfar_50.su2", F.EQ_NS),
-                                     ("shocktube_100.su2", F.EQ_EULER)])
+                                     ("shocktube_100.su2", F.EQ_EULER), ("neuvillette.su2", F.EQ_NS),
+                                     ("dsr131new_slippery.su2", F.EQ_EULER)])
 def test_builder_bit_exact_on_reference_meshes(golden_dir, name, eq):
     m = F.Mesh.read_su2(golden_dir / name, equation=eq)
     o = O.OracleMesh.read_su2(golden_dir / name, equation=eq)
