@@ -1075,10 +1075,10 @@ int steady_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, double a
         t.dtl = w->dtl;
         if (h->ph.equation == FVMR_EQ_NS)
             k_tile<double, double, true, true><<<h->ntiles, TileCfg<double>::kThreads, 0, h->stream>>>(
-                t, h->ph, Uin, Un, Uout, rhs, h->sc, h->bad_list, h->nc, a, b, first);
+                t, h->ph, Uin, Un, Uout, rhs, h->sc, h->bad_list, h->nc, a, b, first, 0);
         else
             k_tile<double, double, false, true><<<h->ntiles, TileCfg<double>::kThreads, 0, h->stream>>>(
-                t, h->ph, Uin, Un, Uout, rhs, h->sc, h->bad_list, h->nc, a, b, first);
+                t, h->ph, Uin, Un, Uout, rhs, h->sc, h->bad_list, h->nc, a, b, first, 0);
         HIP_OK(hipGetLastError());
         return FVMR_OK;
     }

@@ -938,14 +938,14 @@ __global__ __launch_bounds__(kBlock) FVMR_GRAD_ATTR void k_gradient(Mesh<G> m, P
 //   stage 0:    Uout = Uin + dt rhs                      (vector_weighted_add_kernel)
 //   stage 1, 2: Uout = a Un + b (Uin + dt rhs)           (SSP-RK3)
 // Wave j evaluates the canonical flux of face slot j of 64 cells; wave 0 sums the three
-// signed fluxes in slot order and updates.  Cells whose new state is NaN or out of range
-// are appended to the restriction list.
+// signed fluxes in slot order and updates.  In the step's last stage (last != 0) cells whose
+// new state is NaN or out of range are appended to the restriction list.
 template <class C, class G, bool NS>
 __global__ __launch_bounds__(kGroup3) FVMR_STAGE_ATTR void k_stage(Mesh<G> m, Phys ph, const S4 *__restrict__ Uin,
                                                    const S4 *__restrict__ Un, S4 *__restrict__ Uout,
                                                    const Grad<G> *__restrict__ grad, S4 *__restrict__ rhs_out,
                                                    Scalars *__restrict__ sc, int *__restrict__ bad_list, double a,
-                                                   double b, int first) {
+                                                   double b, int first, int last) {
     __shared__ double sF[3][4][kCellsPerGroup];
     const int lane = threadIdx.x & 63, j = threadIdx.x >> 6;
     const int c = xcd_block() * kCellsPerGroup + lane;
@@ -982,6 +982,7 @@ __global__ __launch_bounds__(kGroup3) FVMR_STAGE_ATTR void k_stage(Mesh<G> m, Ph
     }
     if (rhs_out) st(rhs_out, c, r);
     st(Uout, c, Unew);
+    if (!last) return;
     double rv[4];
     prim(Unew, rv, ph.gamma);
     bool bad = out_of_range(rv[0], rv[3]);
@@ -1121,7 +1122,7 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
                                                         const S4 *__restrict__ Un, S4 *__restrict__ Uout,
                                                         S4 *__restrict__ rhs_out, Scalars *__restrict__ sc,
                                                         int *__restrict__ bad_list, int nc, double a, double b,
-                                                        int first) {
+                                                        int first, int last) {
     if (SCHEME == kTileGgBarthRoe) {
         ph.gradient = FVMR_GRA_GREENGAUSS;
         ph.limiter = FVMR_LIM_BARTH;
@@ -1294,6 +1295,7 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
         }
         if (rhs_out) st(rhs_out, c, r);
         st(Uout, c, Unew);
+        if (!last) return;
         double rv[4];
         prim(Unew, rv, ph.gamma);
         bool bad = out_of_range(rv[0], rv[3]);
@@ -1925,23 +1927,33 @@ int launch_gradient(fvmr_solver *h, const S4 *U, int dt_mode) {
     return prof_end(h, 2, e0);
 }
 
+// The restriction after the step's last stage (CIteration.cpp:79 and :152 restrict once per
+// iteration, after the evolve).
+template <class G>
+int launch_restrict(fvmr_solver *h, S4 *U) {
+    hipEvent_t e0{};
+    int rc = prof_begin(h, 3, &e0);
+    if (rc) return rc;
+    k_restrict<G><<<1, 1024, 0, h->stream>>>(mesh_args<G>(h), U, h->sc, h->bad_list, h->fix);
+    HIP_OK(hipGetLastError());
+    return prof_end(h, 3, e0);
+}
+
 template <class C, class G>
-int launch_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, double a, double b, int first, S4 *rhs_out) {
+int launch_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, double a, double b, int first, int last,
+                 S4 *rhs_out) {
     hipEvent_t e0{};
     int rc = prof_begin(h, 0, &e0);
     if (rc) return rc;
     if (h->ph.equation == FVMR_EQ_NS)
         k_stage<C, G, true><<<grid3_of(h->nc), kGroup3, 0, h->stream>>>(mesh_args<G>(h), h->ph, Uin, Un, Uout,
-            (const Grad<G> *)h->grad, rhs_out, h->sc, h->bad_list, a, b, first);
+            (const Grad<G> *)h->grad, rhs_out, h->sc, h->bad_list, a, b, first, last);
     else
         k_stage<C, G, false><<<grid3_of(h->nc), kGroup3, 0, h->stream>>>(mesh_args<G>(h), h->ph, Uin, Un, Uout,
-            (const Grad<G> *)h->grad, rhs_out, h->sc, h->bad_list, a, b, first);
+            (const Grad<G> *)h->grad, rhs_out, h->sc, h->bad_list, a, b, first, last);
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 0, e0))) return rc;
-    if ((rc = prof_begin(h, 3, &e0))) return rc;
-    k_restrict<G><<<1, 1024, 0, h->stream>>>(mesh_args<G>(h), Uout, h->sc, h->bad_list, h->fix);
-    HIP_OK(hipGetLastError());
-    return prof_end(h, 3, e0);
+    return last ? launch_restrict<G>(h, Uout) : FVMR_OK;
 }
 
 // Global dt over the tiles (dt_mode as launch_gradient).
@@ -1963,7 +1975,7 @@ int launch_tile_dt(fvmr_solver *h, const S4 *U, int dt_mode) {
 }
 
 template <class C, class G>
-int launch_tile_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, double a, double b, int first,
+int launch_tile_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, double a, double b, int first, int last,
                       S4 *rhs_out) {
     hipEvent_t e0{};
     int rc = prof_begin(h, 0, &e0);
@@ -1972,13 +1984,10 @@ int launch_tile_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, dou
     auto *kern = ns ? (fixed ? k_tile<C, G, true, false, kTileGgBarthRoe> : k_tile<C, G, true>)
                     : (fixed ? k_tile<C, G, false, false, kTileGgBarthRoe> : k_tile<C, G, false>);
     kern<<<h->ntiles, TileCfg<C>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, Uin, Un, Uout, rhs_out, h->sc,
-                                                            h->bad_list, h->nc, a, b, first);
+                                                            h->bad_list, h->nc, a, b, first, last);
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 0, e0))) return rc;
-    if ((rc = prof_begin(h, 3, &e0))) return rc;
-    k_restrict<G><<<1, 1024, 0, h->stream>>>(mesh_args<G>(h), Uout, h->sc, h->bad_list, h->fix);
-    HIP_OK(hipGetLastError());
-    return prof_end(h, 3, e0);
+    return last ? launch_restrict<G>(h, Uout) : FVMR_OK;
 }
 
 // One step on the device with the tile kernels.
@@ -1988,10 +1997,11 @@ int device_step_tiles(fvmr_solver *h, bool store_rhs) {
     const int a = h->cur, b = (a + 1) % 3, c = (a + 2) % 3;
     S4 *Ua = h->Ubuf[a], *Ub = h->Ubuf[b], *Uc = h->Ubuf[c];
     if ((rc = launch_tile_dt<C, G>(h, Ua, 1))) return rc;
-    if ((rc = launch_tile_stage<C, G>(h, Ua, Ua, Ub, 0.0, 1.0, 1, store_rhs ? h->rhs : nullptr))) return rc;
-    if (h->p.time_advance == FVMR_EVO_RK3) {
-        if ((rc = launch_tile_stage<C, G>(h, Ub, Ua, Uc, 0.75, 0.25, 0, nullptr))) return rc;
-        if ((rc = launch_tile_stage<C, G>(h, Uc, Ua, Ub, 1.0 / 3.0, 2.0 / 3.0, 0, nullptr))) return rc;
+    const int rk3 = h->p.time_advance == FVMR_EVO_RK3;
+    if ((rc = launch_tile_stage<C, G>(h, Ua, Ua, Ub, 0.0, 1.0, 1, !rk3, store_rhs ? h->rhs : nullptr))) return rc;
+    if (rk3) {
+        if ((rc = launch_tile_stage<C, G>(h, Ub, Ua, Uc, 0.75, 0.25, 0, 0, nullptr))) return rc;
+        if ((rc = launch_tile_stage<C, G>(h, Uc, Ua, Ub, 1.0 / 3.0, 2.0 / 3.0, 0, 1, nullptr))) return rc;
     }
     h->cur = b;
     if (store_rhs) h->rhs_valid = true;
@@ -2005,14 +2015,15 @@ int device_step(fvmr_solver *h, bool store_rhs) {
     const int a = h->cur, b = (a + 1) % 3, c = (a + 2) % 3;
     S4 *Ua = h->Ubuf[a], *Ub = h->Ubuf[b], *Uc = h->Ubuf[c];
     if ((rc = launch_gradient<C, G>(h, Ua, 1))) return rc;
-    if ((rc = launch_stage<C, G>(h, Ua, Ua, Ub, 0.0, 1.0, 1, store_rhs ? h->rhs : nullptr))) return rc;
-    if (h->p.time_advance == FVMR_EVO_RK3) {
+    const int rk3 = h->p.time_advance == FVMR_EVO_RK3;
+    if ((rc = launch_stage<C, G>(h, Ua, Ua, Ub, 0.0, 1.0, 1, !rk3, store_rhs ? h->rhs : nullptr))) return rc;
+    if (rk3) {
         // SSP-RK3 (Shu-Osher): U1 = U + dt R(U); U2 = 3/4 U + 1/4 (U1 + dt R(U1));
-        // U^{n+1} = 1/3 U + 2/3 (U2 + dt R(U2))
+        // U^{n+1} = 1/3 U + 2/3 (U2 + dt R(U2)); the restriction follows the last stage
         if ((rc = launch_gradient<C, G>(h, Ub, -1))) return rc;
-        if ((rc = launch_stage<C, G>(h, Ub, Ua, Uc, 0.75, 0.25, 0, nullptr))) return rc;
+        if ((rc = launch_stage<C, G>(h, Ub, Ua, Uc, 0.75, 0.25, 0, 0, nullptr))) return rc;
         if ((rc = launch_gradient<C, G>(h, Uc, -1))) return rc;
-        if ((rc = launch_stage<C, G>(h, Uc, Ua, Ub, 1.0 / 3.0, 2.0 / 3.0, 0, nullptr))) return rc;
+        if ((rc = launch_stage<C, G>(h, Uc, Ua, Ub, 1.0 / 3.0, 2.0 / 3.0, 0, 1, nullptr))) return rc;
     }
     h->cur = b;
     if (store_rhs) h->rhs_valid = true;
@@ -2645,15 +2656,15 @@ int fvmr_stage_rhs(fvmr_handle *h, double *const rhs[4]) {
     HIP_OK(hipMemcpyAsync(h->sc, &z, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
     const int a = h->cur, b = (a + 1) % 3;
     if (h->tiles) {
-        rc = h->mixed ? launch_tile_stage<float, float>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs)
-                      : launch_tile_stage<double, double>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs);
+        rc = h->mixed ? launch_tile_stage<float, float>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, 0, h->rhs)
+                      : launch_tile_stage<double, double>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, 0, h->rhs);
         if (rc) return rc;
     } else if (h->mixed) {
         if ((rc = launch_gradient<float, float>(h, h->Ubuf[a], -1)) ||
-            (rc = launch_stage<float, float>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs))) return rc;
+            (rc = launch_stage<float, float>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, 0, h->rhs))) return rc;
     } else {
         if ((rc = launch_gradient<double, double>(h, h->Ubuf[a], -1)) ||
-            (rc = launch_stage<double, double>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, h->rhs))) return rc;
+            (rc = launch_stage<double, double>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, 0, h->rhs))) return rc;
     }
     HIP_OK(hipMemcpyAsync(h->sc, &s, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
     h->rhs_valid = true;

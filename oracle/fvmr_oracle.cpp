@@ -1112,7 +1112,8 @@ double orc_global_dt(void *h) {
     return global_dt(*s, Up);
 }
 
-/* CIteration::iteration_host (CIteration.cpp:28-80) with the GPU restriction; SSP-RK3 for 3103 */
+/* CIteration::iteration_host (CIteration.cpp:28-80) with the GPU restriction; SSP-RK3 for 3103,
+ * restricted once after its last stage as the reference restricts once per iteration */
 int orc_step(void *h, double *t, double T, fvmr_step_info *info) {
     OSolver *s = (OSolver *)h;
     const int ne = s->ne;
@@ -1144,8 +1145,9 @@ int orc_step(void *h, double *t, double T, fvmr_step_info *info) {
                     double stage = s->U[k][c] + dt * s->Flux[k][c];
                     s->U[k][c] = (st == 0) ? stage : a[st] * s->Uold[k][c] + b[st] * stage;
                 }
-            nres += restrict_field(*s, Uw);
         }
+        /* once per iteration, after the evolve (CIteration.cpp:79) */
+        nres += restrict_field(*s, Uw);
     } else {
         return fail(FVMR_ERR_UNSUPPORTED, "invalid temporal_choice");
     }

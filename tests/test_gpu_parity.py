@@ -128,9 +128,12 @@ def test_residual_norms():
     g.close()
 
 
-def test_restriction_repairs_bad_cells():
-    announce("restriction of injected bad cells")
-    m, p, U = vortex_case(32, advance=F.EVO_EXPLICIT)
+@pytest.mark.parametrize("advance", [F.EVO_EXPLICIT, F.EVO_RK3])
+def test_restriction_repairs_bad_cells(advance):
+    """Restriction once per step, after the last stage (with RK3 the bad cells of the first
+    stages feed the later ones, as in the reference, which restricts after the evolve)."""
+    announce(f"restriction of injected bad cells, advance={advance}")
+    m, p, U = vortex_case(32, advance=advance)
     U = U.copy()
     U[0, 100] = -1.0       # negative density
     U[3, 200] = np.nan     # NaN energy
