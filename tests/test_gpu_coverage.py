@@ -141,3 +141,29 @@ def test_full_size_order_and_family_invariance():
     assert outs["hilbert_tile"][0] == outs["rcm_tile"][0]
     assert np.array_equal(outs["hilbert_tile"][1], outs["rcm_tile"][1])
     assert rel_l2(outs["hilbert_cell"][1], outs["hilbert_tile"][1]) < 1e-13
+
+
+TINY = {
+    # name: node x, node y, element types (5 triangle, 9 quad), connectivity, markers
+    "one_triangle": ([0, 1, 0], [0, 0, 1], [5], [[0, 1, 2]], [("far", [(0, 1), (1, 2), (2, 0)])]),
+    "two_triangles": ([0, 1, 1, 0], [0, 0, 1, 1], [5, 5], [[0, 1, 2], [0, 2, 3]],
+                      [("inlet", [(3, 0)]), ("outlet", [(1, 2)]), ("wall_slippery", [(0, 1), (2, 3)])]),
+    "one_quad": ([0, 1, 1, 0], [0, 0, 1, 1], [9], [[0, 1, 2, 3]], [("far", [(0, 1), (1, 2), (2, 3), (3, 0)])]),
+}
+
+
+@pytest.mark.parametrize("kernels", [F.KERNELS_CELL, F.KERNELS_TILE])
+@pytest.mark.parametrize("name", list(TINY))
+def test_tiny_meshes(name, kernels):
+    """The smallest meshes: one cell whose faces are all boundaries, two cells sharing one face,
+    a quad split into two; one tile, one work-group, 3 RK3 steps against the oracle."""
+    announce(f"tiny mesh {name}, kernels={kernels}, 3 RK3 steps")
+    x, y, et, conn, mk = TINY[name]
+    m = F.Mesh.from_lists(np.array(x, float), np.array(y, float), list(range(len(x))), et, conn, mk)
+    p = F.default_params(inf_ruvp=[1.0, 0.3, 0.1, 1.0], inlet_ruvp=[1.2, 0.5, 0.0, 1.1],
+                         outlet_ruvp=[1.0, 0.2, 0.0, 0.9], time_advance=F.EVO_RK3, CFL=0.5, kernels=kernels)
+    U = F.field_init(m, p, F.INIT_UNIFORM)
+    U[0] *= 1.1 + 0.1 * np.arange(U.shape[1])
+    tg, to, Ug, Uo, _, _ = steps_both(m, p, U, 3)
+    assert abs(tg / to - 1) < TOL
+    assert rel_l2(Ug, Uo) < TOL
