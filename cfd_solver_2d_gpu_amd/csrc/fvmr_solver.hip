@@ -121,6 +121,7 @@ struct Phys {
     double g_ga1, inv_g;           // gamma/(gamma-1), 1/gamma
     CaseBC bc;                     // the handle's (first case's) boundary states and CFL
     int equation, scheme, reconstruct, gradient, limiter, mu_model, compat_inv;
+    int has_periodic;              // the mesh has periodic faces
 };
 
 // Device scalars of a handle: t, dt, T, t at the start of the step.
@@ -724,11 +725,18 @@ __device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[NV], const
 #pragma unroll
         for (int k = 0; k < NV; k++) {
             // bounds over the cell and its non-periodic neighbours (fmin/fmax: a NaN bound
-            // already makes the Green-Gauss or least-squares gradient NaN)
+            // already makes the Green-Gauss or least-squares gradient NaN).  Without periodic
+            // faces a slot with no neighbour holds the cell's own state, which leaves both
+            // bounds as they are, so the plain types skip the per-slot test
             C lo = Uc[k], hi = Uc[k];
+            if (std::is_floating_point<C>::value && !ph.has_periodic) {
 #pragma unroll
-            for (int i = 0; i < 3; i++)
-                if (fl[i] & 2) { lo = fmin(lo, S.Uo(i, k)); hi = fmax(hi, S.Uo(i, k)); }
+                for (int i = 0; i < 3; i++) { lo = fmin(lo, S.Uo(i, k)); hi = fmax(hi, S.Uo(i, k)); }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+                    if (fl[i] & 2) { lo = fmin(lo, S.Uo(i, k)); hi = fmax(hi, S.Uo(i, k)); }
+            }
             C phi = C(1.0);
             const C hrel = hi - Uc[k], lrel = lo - Uc[k];
             if (ph.limiter == FVMR_LIM_BARTH && !isnan(hrel) && !isnan(lrel)) {
@@ -2253,6 +2261,7 @@ static int create_impl(const fvmr_mesh_soa *m, const fvmr_params *p, int ncases,
         if (t == FVMR_BC_DSR) h->bc_flags |= 2;
         if (t == FVMR_BC_WALL_ISOTHERMAL) h->bc_flags |= 4;
     }
+    ph.has_periodic = (h->bc_flags & 1) ? 1 : 0;
     // connectivity in the new numbering
     if (nf > kSlotEdgeMask) return bail(set_error(FVMR_ERR_UNSUPPORTED, "too many edges for the face-slot encoding"));
     std::vector<int2> cslot(3 * (size_t)nc);
