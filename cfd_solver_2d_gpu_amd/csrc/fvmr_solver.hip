@@ -671,8 +671,11 @@ __device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[NV], const
             C s0 = C(0), s1 = C(0);
 #pragma unroll
             for (int i = 0; i < 3; i++) {
+                // a slot with no cell across has g = 1/2 and the cell's own state, for which the
+                // blend gives U_C exactly, so the plain types blend every slot
                 const A_t g = S.A(i, 0);
-                const C phif = (fl[i] & 1) ? g * Uc[k] + (A_t(1.0) - g) * S.Uo(i, k) : Uc[k];
+                const bool blend = std::is_floating_point<C>::value || (fl[i] & 1);
+                const C phif = blend ? g * Uc[k] + (A_t(1.0) - g) * S.Uo(i, k) : Uc[k];
                 s0 += phif * S.A(i, 1);
                 s1 += phif * S.A(i, 2);
             }
@@ -745,12 +748,27 @@ __device__ __forceinline__ void grad_limit(const Phys &ph, const C Uc[NV], const
                 // minimum is one division by the extreme denominator, needed only when the
                 // quotient can fall below 1
                 C dpos = C(0), dneg = C(0);
+                if (std::is_floating_point<C>::value) {
+                    // rounding is monotonic, so the largest rel + eps is the largest rel plus
+                    // eps: the extremes first, eps once (the same values, no per-vertex selects)
+                    C mp = C(0), mn = C(0);
 #pragma unroll
-                for (int jn = 0; jn < 3; jn++) {
-                    const C Un = Uc[k] + gx[k] * S.N(jn, 0) + gy[k] * S.N(jn, 1);
-                    const C rel = Un - Uc[k];
-                    if (rel > C(0)) dpos = fmax(dpos, rel + C(fvmr::kEPS));
-                    else if (rel < C(0)) dneg = fmin(dneg, rel - C(fvmr::kEPS));
+                    for (int jn = 0; jn < 3; jn++) {
+                        const C Un = Uc[k] + gx[k] * S.N(jn, 0) + gy[k] * S.N(jn, 1);
+                        const C rel = Un - Uc[k];
+                        mp = fmax(mp, rel);
+                        mn = fmin(mn, rel);
+                    }
+                    if (mp > C(0)) dpos = mp + C(fvmr::kEPS);
+                    if (mn < C(0)) dneg = mn - C(fvmr::kEPS);
+                } else {
+#pragma unroll
+                    for (int jn = 0; jn < 3; jn++) {
+                        const C Un = Uc[k] + gx[k] * S.N(jn, 0) + gy[k] * S.N(jn, 1);
+                        const C rel = Un - Uc[k];
+                        if (rel > C(0)) dpos = fmax(dpos, rel + C(fvmr::kEPS));
+                        else if (rel < C(0)) dneg = fmin(dneg, rel - C(fvmr::kEPS));
+                    }
                 }
                 if (dpos > C(0) && hrel < dpos) phi = tmin(phi, fdiv(hrel, dpos));
                 if (dneg < C(0) && lrel > dneg) phi = tmin(phi, fdiv(lrel, dneg));
