@@ -161,3 +161,23 @@ def test_mixed_precision_close_to_fp64():
         outs.append(g.get_state())
         g.close()
     assert rel_l2(outs[1][0], outs[0][0]) < 1e-5
+
+
+def test_c1_vortex_100_steps():
+    """The periodic isentropic vortex workload as specified (64x64x2 = 8,192 cells, Roe, least
+    squares + Barth, SSP-RK3, CFL 0.5, 100 steps): the FP64 field and t against the oracle."""
+    announce("C1 vortex 8192 cells, LSQ + Barth, 100 RK3 steps")
+    m = F.Mesh.generate(F.GEN_PERIODIC_SQUARE, 64, 64, a=0.0, b=10.0)
+    p = F.default_params(inf_ruvp=[1, 1, 1, 1], CFL=0.5, gradient=F.GRA_LEASTSQUARE, limiter=F.LIM_BARTH,
+                         flux_scheme=F.SOL_ROE, time_advance=F.EVO_RK3)
+    U = F.field_init(m, p, F.INIT_ISENTROPIC_VORTEX, (5.0, 5.0, 5.0))
+    g, o = pair(m, p, U)
+    tg, _ = g.advance(100, 0.0, 1e9)
+    to = 0.0
+    for _ in range(100):
+        to, _ = o.step(to, 1e9)
+    assert abs(tg / to - 1) < TOL
+    Ug, Uo = g.get_state(), o.get_state()
+    for k in range(4):
+        assert rel_l2(Ug[k], Uo[k]) < TOL, (k, rel_l2(Ug[k], Uo[k]))
+    g.close()
