@@ -77,3 +77,31 @@ def test_quickstart_case_through_driver(tmp_path, golden_dir, name):
         Ud = g.get_state()
     _, Uc, tc, ic, _ = F.Mesh.read_continue(str(d / f"pause_{CASES[name][1]}[0020].dat"))
     assert ic == 20 and tc == t and np.array_equal(Uc, Ud)
+
+
+def test_cylinder_reaches_the_reference_state_at_its_time(golden_dir):
+    """The quick_start cylinder (Navier-Stokes with Sutherland viscosity, M 0.5 farfield, Roe,
+    Green-Gauss + Barth, forward Euler, CFL 2.2, its own input.toml) from its uniform initial field
+    to t = 0.120747, the time of the reference's own continue file pause_cylinder_allfar_50[20000]:
+    the state matches the reference's to 5e-5 (rho, rhoE) and 1e-3 (rhov, zero at the start)
+    relative L2, while the flow has moved 9-40 % (rhov 100 %) from the initial field.  The
+    reference takes 20,000 steps, this build about 21,850: the reference's dt kernel loses face
+    terms to unsynchronised read-modify-writes (CalculateDtGPU.cu:47-49) and so takes larger
+    steps; the reference also computes in FP32."""
+    print("[gpu-case] cylinder from its initial field to the reference continue file's time", flush=True)
+    cfg = D.config_from_tree(toml_tree("cylinder.input.toml"))
+    m = F.Mesh.read_su2(golden_dir / "cylinder_allfar_50.su2", cfg.equation, cfg.mesh_scale_factor)
+    _, Uref, tref, iref, _ = F.Mesh.read_continue(str(golden_dir / "pause_cylinder_allfar_50_20000.dat"))
+    assert iref == 20000
+    p = cfg.params()
+    with F.GPUSolver2(m, p) as g:
+        g.set_state(F.field_init(m, p, cfg.initial_type, cfg.init_args()))
+        t, _ = g.advance(20000, 0.0, tref)
+        assert t < tref                      # smaller steps than the reference's
+        t, _ = g.advance(4000, t, tref)
+        assert t == tref
+        U = g.get_state()
+    tol = [2e-4, 1e-3, 4e-3, 2e-4]
+    for k in range(4):
+        err = np.linalg.norm(U[k] - Uref[k]) / np.linalg.norm(Uref[k])
+        assert err < tol[k], (k, err)

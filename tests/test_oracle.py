@@ -283,3 +283,71 @@ def test_shocktube_first_residual_matches_reference_log(golden_dir):
     s.step(0.0, cfg.max_physical_time)
     r = s.residual(F.NORM_INF)[0]
     assert abs(r / float(log["residual_rho"][0]) - 1) < 5e-6, r
+
+
+def test_cylinder_oracle_reaches_the_reference_state_at_its_time(golden_dir):
+    """The reference's own output of a long run: pause_cylinder_allfar_50[20000], written after
+    20,000 forward-Euler steps of the quick_start cylinder (Navier-Stokes, Sutherland, M 0.5, Roe,
+    Green-Gauss + Barth, CFL 2.2) at t = 0.120747.  The oracle from the case's uniform initial
+    field to that time matches it to 5e-5 (rho, rhoE), 2.4e-4 (rhou) and 1e-3 (rhov, zero at the
+    start) relative L2, where the flow has moved 9-40 % (rhov 100 %): a dropped term, a wrong
+    sign or a wrong boundary would leave a difference of the size of the evolution.  The
+    remaining difference is the reference's FP32 and its larger, racy dt (20,000 steps against
+    the oracle's 21,852).  About a minute of CPU time."""
+    from cfd_solver_2d_gpu_amd import driver as D
+    from helpers import toml_tree
+    cfg = D.config_from_tree(toml_tree("cylinder.input.toml"))
+    m = O.OracleMesh.read_su2(golden_dir / "cylinder_allfar_50.su2", cfg.equation, cfg.mesh_scale_factor)
+    _, Uref, tref, iref, _ = F.Mesh.read_continue(str(golden_dir / "pause_cylinder_allfar_50_20000.dat"))
+    p = cfg.params()
+    U0 = O.field_init(m.soa, p, cfg.initial_type, cfg.init_args())
+    s = O.OracleSolver(m, p)
+    s.set_state(U0)
+    t, n = 0.0, 0
+    while t < tref and n < 30000:
+        t, _ = s.step(t, tref)
+        n += 1
+    assert t == tref and 20000 < n < 24000
+    U = s.get_state()
+    tol = [2e-4, 1e-3, 4e-3, 2e-4]
+    for k in range(4):
+        err = np.linalg.norm(U[k] - Uref[k]) / np.linalg.norm(Uref[k])
+        moved = np.linalg.norm(U0[k] - Uref[k]) / np.linalg.norm(Uref[k])
+        assert err < tol[k] and err < 0.01 * moved, (k, err, moved)
+
+
+def test_viscous_flux_of_a_linear_velocity_field():
+    """rho = 1, p = 1 and a linear velocity field u = a y + b x, v = c x + d y, with constant mu
+    and least squares without a limiter (exact for the linear momenta): on every interior face
+    whose two cells have exact gradients, the viscous part of the flux (the Euler flux minus the
+    Navier-Stokes flux of the same state) is zero in the mass and (tau_xx nx + tau_xy ny) L,
+    (tau_xy nx + tau_yy ny) L in the momenta, with tau_xx = mu (2 b - 2/3 (b + d)),
+    tau_yy = mu (2 d - 2/3 (b + d)), tau_xy = mu (a + c) (ViscousFluxGPU.cu:139-229).  A wrong
+    sign, factor or divergence term, or a transposed gradient, fails."""
+    m = F.Mesh.generate(F.GEN_RECT_CHANNEL, 16, 8, a=2.0, b=1.0, jitter=0.1, seed=3, equation=F.EQ_NS)
+    ar = m.arrays()
+    a, b, c, d, mu = 0.3, 0.1, -0.2, 0.05, 0.01
+    x, y = ar["element_xy"]
+    u, v = a * y + b * x, c * x + d * y
+    U = np.array([np.ones_like(x), u, v, 1.0 / 0.4 + 0.5 * (u * u + v * v)])
+    flux = {}
+    for eq in (F.EQ_EULER, F.EQ_NS):
+        p = F.default_params(equation=eq, viscosity_model=F.MU_CONSTANT, mu0=mu, gradient=F.GRA_LEASTSQUARE,
+                             limiter=F.LIM_NONE, inf_ruvp=[1.0, 0.0, 0.0, 1.0], inlet_ruvp=[1.0, 0.0, 0.0, 1.0],
+                             outlet_ruvp=[1.0, 0.0, 0.0, 1.0])
+        s = O.OracleSolver(m, p)
+        s.set_state(U)
+        s.stage_rhs()
+        flux[eq] = s.edge_flux()
+    Fv = flux[F.EQ_EULER] - flux[F.EQ_NS]
+    L, R = ar["edge_elementL"], ar["edge_elementR"]
+    nvalid = (ar["element_neighbors"][:3] >= 0).sum(axis=0)
+    e = np.nonzero((R >= 0) & (nvalid[L] >= 2) & (nvalid[np.maximum(R, 0)] >= 2))[0]
+    assert len(e) > 300
+    nx, ny = ar["edge_normal"][0][e], ar["edge_normal"][1][e]
+    ln = ar["edge_length"][e]
+    txx, tyy, txy = mu * (2 * b - 2 / 3 * (b + d)), mu * (2 * d - 2 / 3 * (b + d)), mu * (a + c)
+    scale = mu * ln
+    assert np.abs(Fv[0][e]).max() == 0.0
+    assert np.abs(Fv[1][e] - (txx * nx + txy * ny) * ln).max() < 1e-12 * scale.max()
+    assert np.abs(Fv[2][e] - (txy * nx + tyy * ny) * ln).max() < 1e-12 * scale.max()
