@@ -639,7 +639,7 @@ __device__ __forceinline__ void edge_alpha(const Phys &ph, const T ra[4], const 
     }
     const T dx = nx * len, dy = ny * len;
     if (std::is_same<T, double>::value)
-        *inv = fabs(u * dx + v * dy) + sqrt(gamma * p / rho) * sqrt(len);
+        *inv = fabs(u * dx + v * dy) + sqrt(fdiv(gamma * p, rho)) * sqrt(len);
     else
         *inv = fabs(u * dx + v * dy) + fsqrt(fdiv(gamma * p, rho)) * fsqrt(len);
     *vis = T(0);
@@ -944,7 +944,7 @@ __global__ __launch_bounds__(kBlock) FVMR_GRAD_ATTR void k_gradient(Mesh<G> m, P
                 alpha_c += av;
                 if (ph.equation == FVMR_EQ_NS) alpha_c += vv;
             }
-            dtc = ph.bc.CFL * (double)cg.z / alpha_c;
+            dtc = fdiv(ph.bc.CFL * (double)cg.z, alpha_c);
         }
     }
     if (DT) {
@@ -1462,7 +1462,7 @@ __global__ __launch_bounds__(TileCfg<G>::kThreads) void k_tile_dt(TileArgs<G> ta
             alpha_c += av;
             if (ph.equation == FVMR_EQ_NS) alpha_c += vv;
         }
-        dtc = fmin(dtc, ta.cbc[ti[fvmr::kTiCase]].CFL * vol_c / alpha_c);
+        dtc = fmin(dtc, fdiv(ta.cbc[ti[fvmr::kTiCase]].CFL * vol_c, alpha_c));
     }
     const double wm = wave_min(dtc);
     if ((tid & 63) == 0) sMin[tid >> 6] = wm;
