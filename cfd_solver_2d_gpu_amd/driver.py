@@ -349,7 +349,7 @@ def config_from_tree(tree):
     c.flag_gradient = g("inviscid_flux_method.flag_gradient", int)
     c.flux_conservation_scheme = g("inviscid_flux_method.flux_conservation_scheme", int)
     c.flux_limiter = g("inviscid_flux_method.flux_limiter", int)
-    # [device]: keys of this build
+    # [device]: keys this library adds (the reference has no such table)
     c.precision = _enum(tr, "device.precision", {"fp64": F.PREC_FP64, "mixed": F.PREC_MIXED}, c.precision)
     c.renumber = _enum(tr, "device.renumber", {"none": F.RENUMBER_NONE, "rcm": F.RENUMBER_RCM,
                                                "hilbert": F.RENUMBER_HILBERT}, c.renumber)

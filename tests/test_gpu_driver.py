@@ -233,7 +233,7 @@ def test_driver_shocktube_against_reference_log(tmp_path, golden_dir):
     CDriver.cpp:115-125).  Step 1 is the rhs of the initial state: the density residual agrees to
     the six printed digits.  Later steps take other dt: the reference's dt kernel adds each face's
     term into both cells with unsynchronised read-modify-writes (CalculateDtGPU.cu:47-49), which
-    lose terms and give a larger dt, so it reaches T sooner (between steps 500 and 600; this build
+    lose terms and give a larger dt, so it reaches T sooner (between steps 500 and 600; this library
     at 758).  After T every step has dt = 0 and the residual is that of the state at T, which a
     different dt history moves by a few per cent: 3.0014 against 3.07145."""
     import json

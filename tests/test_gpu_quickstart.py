@@ -85,7 +85,7 @@ def test_cylinder_reaches_the_reference_state_at_its_time(golden_dir):
     to t = 0.120747, the time of the reference's own continue file pause_cylinder_allfar_50[20000]:
     the state matches the reference's to 5e-5 (rho, rhoE) and 1e-3 (rhov, zero at the start)
     relative L2, while the flow has moved 9-40 % (rhov 100 %) from the initial field.  The
-    reference takes 20,000 steps, this build about 21,850: the reference's dt kernel loses face
+    reference takes 20,000 steps, this library about 21,850: the reference's dt kernel loses face
     terms to unsynchronised read-modify-writes (CalculateDtGPU.cu:47-49) and so takes larger
     steps; the reference also computes in FP32."""
     print("[gpu-case] cylinder from its initial field to the reference continue file's time", flush=True)
