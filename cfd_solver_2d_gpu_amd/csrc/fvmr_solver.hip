@@ -1593,6 +1593,26 @@ __global__ void k_surface_force(int nsets, const int *__restrict__ off, const in
     out[2 * s + 1] = cl;
 }
 
+// Device order -> the caller's order (out: 4 x nc, component-major) and back.
+__global__ __launch_bounds__(kBlock) void k_to_caller(int nc, const int *__restrict__ perm, const S4 *__restrict__ U,
+                                                      double *__restrict__ out) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nc) return;
+    const size_t r = (size_t)perm[i], n = (size_t)nc;
+    const S4 u = U[i];
+    out[r] = u.x;
+    out[n + r] = u.y;
+    out[2 * n + r] = u.z;
+    out[3 * n + r] = u.w;
+}
+__global__ __launch_bounds__(kBlock) void k_from_caller(int nc, const int *__restrict__ perm,
+                                                        const double *__restrict__ in, S4 *__restrict__ U) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nc) return;
+    const size_t r = (size_t)perm[i], n = (size_t)nc;
+    U[i] = S4{in[r], in[n + r], in[2 * n + r], in[3 * n + r]};
+}
+
 // modifyElementFieldU2d_device_kernel (RestrictGPU.cu:6-84) with Jacobi reads: one work-group,
 // phase 1 computes all repairs from the unrepaired field, phase 2 writes them.
 template <class G>
@@ -1848,6 +1868,8 @@ struct fvmr_solver {
     std::vector<int> case_cell_begin;
     CaseBC *cbc = nullptr;
     int *tile_case_begin = nullptr;       // tiles of case c: [c] .. [c+1]-1
+    int *perm_dev = nullptr;              // perm on the device (state transfers, first use)
+    double *io_soa = nullptr;             // 4 x nc doubles in the caller's order (state transfers)
     // tile kernels (fvmr_tiles.h)
     bool tiles = false;
     int ntiles = 0;
@@ -2139,7 +2161,7 @@ void free_all(fvmr_solver *h) {
     void *ptrs[] = {h->grad, h->rhs, h->fix, h->cgeo, h->nxy, h->egeo, h->emid, h->cslot, h->cnode, h->dtcode,
                     h->eint, h->partial, h->norm_out, h->bad_list, h->sc, h->t_info, h->t_halo, h->t_lnodes,
                     h->t_nxy, h->t_gslot, h->t_face, h->t_gcoef, h->t_dtrec,
-                    h->nc_off, h->nc_cells, h->node_buf, h->cbc, h->tile_case_begin};
+                    h->nc_off, h->nc_cells, h->node_buf, h->cbc, h->tile_case_begin, h->perm_dev, h->io_soa};
     for (void *q : ptrs) if (q) (void)hipFree(q);
     for (int k = 0; k < kNumProf; k++)
         for (auto e : h->pending[k]) (void)hipEventDestroy(e);
@@ -2555,17 +2577,36 @@ int fvmr_destroy(fvmr_handle *h) {
     return FVMR_OK;
 }
 
+// The state crosses the boundary as four host arrays in the caller's order: one contiguous copy
+// per component into (out of) a device scratch in that order, and the permutation to (from) the
+// device order as a gather (scatter) on the device.  perm is a permutation of 0 .. nc-1 (built
+// by fvmr_create), so every index stays inside the nc-long arrays.
+static int ensure_io(fvmr_handle *h) {
+    if (h->io_soa) return FVMR_OK;
+    const size_t nc = (size_t)h->nc;
+    if (hipMalloc((void **)&h->perm_dev, nc * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&h->io_soa, 4 * nc * sizeof(double)) != hipSuccess) {
+        if (h->perm_dev) (void)hipFree(h->perm_dev);
+        h->perm_dev = nullptr;
+        h->io_soa = nullptr;
+        return set_error(FVMR_ERR_HIP, "hipMalloc state transfer buffers");
+    }
+    HIP_OK(hipMemcpyAsync(h->perm_dev, h->perm.data(), nc * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    return FVMR_OK;
+}
+
 int fvmr_set_state(fvmr_handle *h, const double *const U[4]) {
     if (!h || !U) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
     for (int k = 0; k < 4; k++)
         if (!U[k]) return set_error(FVMR_ERR_INVALID_ARG, "null state component");
+    int rc = ensure_io(h);
+    if (rc) return rc;
     const int nc = h->nc;
-    std::vector<S4> buf(nc);
-    for (int i = 0; i < nc; i++) {
-        const int r = h->perm[i];
-        buf[i] = S4{U[0][r], U[1][r], U[2][r], U[3][r]};
-    }
-    HIP_OK(hipMemcpyAsync(h->Ubuf[h->cur], buf.data(), buf.size() * sizeof(S4), hipMemcpyHostToDevice, h->stream));
+    for (int k = 0; k < 4; k++)
+        HIP_OK(hipMemcpyAsync(h->io_soa + (size_t)k * nc, U[k], (size_t)nc * sizeof(double), hipMemcpyHostToDevice,
+                              h->stream));
+    k_from_caller<<<grid_of(nc), kBlock, 0, h->stream>>>(nc, h->perm_dev, h->io_soa, h->Ubuf[h->cur]);
+    HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(h->stream));
     h->rhs_valid = false;
     return FVMR_OK;
@@ -2574,14 +2615,15 @@ int fvmr_set_state(fvmr_handle *h, const double *const U[4]) {
 static int read_back(fvmr_handle *h, const S4 *dev, double *const out[4]) {
     for (int k = 0; k < 4; k++)
         if (!out[k]) return set_error(FVMR_ERR_INVALID_ARG, "null output component");
+    int rc = ensure_io(h);
+    if (rc) return rc;
     const int nc = h->nc;
-    std::vector<S4> buf(nc);
-    HIP_OK(hipMemcpyAsync(buf.data(), dev, buf.size() * sizeof(S4), hipMemcpyDeviceToHost, h->stream));
+    k_to_caller<<<grid_of(nc), kBlock, 0, h->stream>>>(nc, h->perm_dev, dev, h->io_soa);
+    HIP_OK(hipGetLastError());
+    for (int k = 0; k < 4; k++)
+        HIP_OK(hipMemcpyAsync(out[k], h->io_soa + (size_t)k * nc, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost,
+                              h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
-    for (int i = 0; i < nc; i++) {
-        const int r = h->perm[i];
-        out[0][r] = buf[i].x; out[1][r] = buf[i].y; out[2][r] = buf[i].z; out[3][r] = buf[i].w;
-    }
     return FVMR_OK;
 }
 
