@@ -1593,23 +1593,24 @@ __global__ void k_surface_force(int nsets, const int *__restrict__ off, const in
     out[2 * s + 1] = cl;
 }
 
-// Device order -> the caller's order (out: 4 x nc, component-major) and back.
-__global__ __launch_bounds__(kBlock) void k_to_caller(int nc, const int *__restrict__ perm, const S4 *__restrict__ U,
-                                                      double *__restrict__ out) {
+// Device order -> the caller's order (out: 4 x nc, component-major) and back, for the nc cells
+// of one case whose perm values are base .. base+nc-1 (the whole handle: base 0).
+__global__ __launch_bounds__(kBlock) void k_to_caller(int nc, int base, const int *__restrict__ perm,
+                                                      const S4 *__restrict__ U, double *__restrict__ out) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= nc) return;
-    const size_t r = (size_t)perm[i], n = (size_t)nc;
+    const size_t r = (size_t)(perm[i] - base), n = (size_t)nc;
     const S4 u = U[i];
     out[r] = u.x;
     out[n + r] = u.y;
     out[2 * n + r] = u.z;
     out[3 * n + r] = u.w;
 }
-__global__ __launch_bounds__(kBlock) void k_from_caller(int nc, const int *__restrict__ perm,
+__global__ __launch_bounds__(kBlock) void k_from_caller(int nc, int base, const int *__restrict__ perm,
                                                         const double *__restrict__ in, S4 *__restrict__ U) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= nc) return;
-    const size_t r = (size_t)perm[i], n = (size_t)nc;
+    const size_t r = (size_t)(perm[i] - base), n = (size_t)nc;
     U[i] = S4{in[r], in[n + r], in[2 * n + r], in[3 * n + r]};
 }
 
@@ -2605,7 +2606,7 @@ int fvmr_set_state(fvmr_handle *h, const double *const U[4]) {
     for (int k = 0; k < 4; k++)
         HIP_OK(hipMemcpyAsync(h->io_soa + (size_t)k * nc, U[k], (size_t)nc * sizeof(double), hipMemcpyHostToDevice,
                               h->stream));
-    k_from_caller<<<grid_of(nc), kBlock, 0, h->stream>>>(nc, h->perm_dev, h->io_soa, h->Ubuf[h->cur]);
+    k_from_caller<<<grid_of(nc), kBlock, 0, h->stream>>>(nc, 0, h->perm_dev, h->io_soa, h->Ubuf[h->cur]);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(h->stream));
     h->rhs_valid = false;
@@ -2618,7 +2619,7 @@ static int read_back(fvmr_handle *h, const S4 *dev, double *const out[4]) {
     int rc = ensure_io(h);
     if (rc) return rc;
     const int nc = h->nc;
-    k_to_caller<<<grid_of(nc), kBlock, 0, h->stream>>>(nc, h->perm_dev, dev, h->io_soa);
+    k_to_caller<<<grid_of(nc), kBlock, 0, h->stream>>>(nc, 0, h->perm_dev, dev, h->io_soa);
     HIP_OK(hipGetLastError());
     for (int k = 0; k < 4; k++)
         HIP_OK(hipMemcpyAsync(out[k], h->io_soa + (size_t)k * nc, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost,
@@ -2668,13 +2669,16 @@ int fvmr_batch_set_state(fvmr_handle *h, int c, const double *const U[4]) {
     if (!h || !U || c < 0 || c >= h->ncases) return set_error(FVMR_ERR_INVALID_ARG, "invalid case or argument");
     for (int k = 0; k < 4; k++)
         if (!U[k]) return set_error(FVMR_ERR_INVALID_ARG, "null state component");
-    const int b = h->case_cell_begin[c], e = h->case_cell_begin[c + 1];
-    std::vector<S4> buf(std::max(1, e - b));
-    for (int i = b; i < e; i++) {
-        const int r = h->perm[i] - b;
-        buf[i - b] = S4{U[0][r], U[1][r], U[2][r], U[3][r]};
+    const int b = h->case_cell_begin[c], n = h->case_cell_begin[c + 1] - b;
+    int rc = ensure_io(h);
+    if (rc) return rc;
+    if (n > 0) {   // the case's cells through the transfer scratch (perm of case c maps into b .. b+n-1)
+        for (int k = 0; k < 4; k++)
+            HIP_OK(hipMemcpyAsync(h->io_soa + (size_t)k * n, U[k], (size_t)n * sizeof(double), hipMemcpyHostToDevice,
+                                  h->stream));
+        k_from_caller<<<grid_of(n), kBlock, 0, h->stream>>>(n, b, h->perm_dev + b, h->io_soa, h->Ubuf[h->cur] + b);
+        HIP_OK(hipGetLastError());
     }
-    if (e > b) HIP_OK(hipMemcpyAsync(h->Ubuf[h->cur] + b, buf.data(), (size_t)(e - b) * sizeof(S4), hipMemcpyHostToDevice, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
     h->rhs_valid = false;
     return FVMR_OK;
@@ -2684,14 +2688,17 @@ int fvmr_batch_get_state(fvmr_handle *h, int c, double *const U[4]) {
     if (!h || !U || c < 0 || c >= h->ncases) return set_error(FVMR_ERR_INVALID_ARG, "invalid case or argument");
     for (int k = 0; k < 4; k++)
         if (!U[k]) return set_error(FVMR_ERR_INVALID_ARG, "null output component");
-    const int b = h->case_cell_begin[c], e = h->case_cell_begin[c + 1];
-    std::vector<S4> buf(std::max(1, e - b));
-    if (e > b) HIP_OK(hipMemcpyAsync(buf.data(), h->Ubuf[h->cur] + b, (size_t)(e - b) * sizeof(S4), hipMemcpyDeviceToHost, h->stream));
-    HIP_OK(hipStreamSynchronize(h->stream));
-    for (int i = b; i < e; i++) {
-        const int r = h->perm[i] - b;
-        U[0][r] = buf[i - b].x; U[1][r] = buf[i - b].y; U[2][r] = buf[i - b].z; U[3][r] = buf[i - b].w;
+    const int b = h->case_cell_begin[c], n = h->case_cell_begin[c + 1] - b;
+    int rc = ensure_io(h);
+    if (rc) return rc;
+    if (n > 0) {
+        k_to_caller<<<grid_of(n), kBlock, 0, h->stream>>>(n, b, h->perm_dev + b, h->Ubuf[h->cur] + b, h->io_soa);
+        HIP_OK(hipGetLastError());
+        for (int k = 0; k < 4; k++)
+            HIP_OK(hipMemcpyAsync(U[k], h->io_soa + (size_t)k * n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                                  h->stream));
     }
+    HIP_OK(hipStreamSynchronize(h->stream));
     return FVMR_OK;
 }
 
