@@ -45,6 +45,7 @@ _SIG = {
     "orc_llf": (None, [_dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
     "orc_farfield": (None, [_dp, _dp, C.c_double, C.c_double, C.c_double, _dp]),
     "orc_slip": (None, [_dp, C.c_double, C.c_double, _dp]),
+    "orc_isothermal": (None, [_dp, C.c_double, C.c_double, C.c_double, _dp]),
     "orc_u2ruvp": (None, [_dp, C.c_double, _dp]),
     "orc_ruvp2u": (None, [_dp, C.c_double, _dp]),
     "orc_inv2x2": (None, [_dp, C.c_int]),
@@ -253,6 +254,14 @@ def slip(UL, nx, ny):
     UL, pl = vec(UL)
     UR, pr = vec(np.zeros(4))
     lib().orc_slip(pl, nx, ny, pr)
+    return UR
+
+
+def isothermal(UL, Tw, R=287.06, gamma=1.4):
+    """Ghost state of the no-slip isothermal wall at temperature Tw."""
+    UL, pl = vec(UL)
+    UR, pr = vec(np.zeros(4))
+    lib().orc_isothermal(pl, Tw, R, gamma, pr)
     return UR
 
 

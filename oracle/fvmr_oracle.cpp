@@ -1200,6 +1200,7 @@ void orc_roe(const double *UL, const double *UR, double nx, double ny, double le
 void orc_llf(const double *UL, const double *UR, double nx, double ny, double len, double gamma, double *F) { llf(UL, UR, nx, ny, len, F, gamma); }
 void orc_farfield(const double *UL, const double *ruvp_inf, double nx, double ny, double gamma, double *UR) { ghost_farfield(UL, ruvp_inf, UR, nx, ny, gamma); }
 void orc_slip(const double *UL, double nx, double ny, double *UR) { ghost_slip(UL, UR, nx, ny); }
+void orc_isothermal(const double *UL, double Tw, double R, double gamma, double *UR) { ghost_isothermal(UL, UR, Tw, R, gamma); }
 void orc_u2ruvp(const double *U, double gamma, double *r) { U2ruvp(U, r, gamma); }
 
 /* FieldWriter::update_nodeField_ruvp (FVMR/src/output/FieldWriter.cpp:371-418): node ruvp =

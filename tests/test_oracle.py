@@ -351,3 +351,200 @@ def test_viscous_flux_of_a_linear_velocity_field():
     assert np.abs(Fv[0][e]).max() == 0.0
     assert np.abs(Fv[1][e] - (txx * nx + txy * ny) * ln).max() < 1e-12 * scale.max()
     assert np.abs(Fv[2][e] - (txy * nx + tyy * ny) * ln).max() < 1e-12 * scale.max()
+
+
+def viscous_part(m, U, **kw):
+    """Viscous part of the oracle's edge fluxes (the Euler flux minus the Navier-Stokes flux of
+    the same state; ViscousFluxGPU.cu:204-213 subtracts it from the convective flux) with least
+    squares without a limiter, plus the interior faces whose two cells have exact gradients of
+    linear fields (two or more element neighbours each)."""
+    ar = m.arrays()
+    ruvp = [1.0, 0.0, 0.0, 1.0]
+    flux = {}
+    for eq in (F.EQ_EULER, F.EQ_NS):
+        p = F.default_params(equation=eq, gradient=F.GRA_LEASTSQUARE, limiter=F.LIM_NONE, inf_ruvp=ruvp,
+                             inlet_ruvp=ruvp, outlet_ruvp=ruvp, **kw)
+        s = O.OracleSolver(m, p)
+        s.set_state(U)
+        s.stage_rhs()
+        flux[eq] = s.edge_flux()
+    L, R = ar["edge_elementL"], ar["edge_elementR"]
+    nvalid = (ar["element_neighbors"][:3] >= 0).sum(axis=0)
+    e = np.nonzero((R >= 0) & (nvalid[L] >= 2) & (nvalid[np.maximum(R, 0)] >= 2))[0]
+    assert len(e) > 300
+    return flux[F.EQ_EULER] - flux[F.EQ_NS], e, ar
+
+
+def ns_mesh():
+    return F.Mesh.generate(F.GEN_RECT_CHANNEL, 16, 8, a=2.0, b=1.0, jitter=0.1, seed=3, equation=F.EQ_NS)
+
+
+def test_viscous_heat_flux_of_a_fluid_at_rest():
+    """rho = 1, u = v = 0 and a linear rho E: T = p / (rho R) = (gamma - 1) rho E / R is linear and
+    the energy row of the viscous flux is the Fourier heat flux k grad T . n L with
+    k = mu Cp / Pr, Cp = gamma R / (gamma - 1): gamma (mu / Pr) grad(rho E) . n L exactly; the
+    momentum rows vanish (ViscousFluxGPU.cu:204-213).  A heat-flux coefficient of Cv instead of
+    Cp is off by the factor gamma."""
+    m = ns_mesh()
+    x, y = m.arrays()["element_xy"]
+    Ex, Ey, mu, Pr = 0.3, -0.2, 0.01, 0.72
+    U = np.array([np.ones_like(x), np.zeros_like(x), np.zeros_like(x), 2.5 + Ex * x + Ey * y])
+    Fv, e, ar = viscous_part(m, U, viscosity_model=F.MU_CONSTANT, mu0=mu, Pr_viscous=Pr)
+    nx, ny, ln = ar["edge_normal"][0][e], ar["edge_normal"][1][e], ar["edge_length"][e]
+    expect = 1.4 * mu / Pr * (Ex * nx + Ey * ny) * ln
+    assert np.abs(Fv[1][e]).max() < 1e-15 and np.abs(Fv[2][e]).max() < 1e-15
+    assert np.abs(Fv[3][e] - expect).max() < 1e-12 * np.abs(expect).max()
+
+
+def test_viscous_energy_flux_of_a_shear_flow():
+    """rho = 1, u = a y, v = c x (tau_xx = tau_yy = 0, tau_xy = mu (a + c)) and a linear rho E.
+    The energy row of the viscous flux is (k grad T + tau . u) . n L with the face velocity
+    u_f = (u_L + u_R) / 2 (the face state is the mean of the two cells) and, from
+    T = (rho E / rho - |u|^2 / 2) / Cv, grad T = (grad(rho E) - u grad u - v grad v) / Cv =
+    (Ex - c v_f, Ey - a u_f) / Cv; tau . u = (tau_xy v_f, tau_xy u_f).  Dropping either work
+    term, or a sign in grad T, fails."""
+    m = ns_mesh()
+    x, y = m.arrays()["element_xy"]
+    a, c, Ex, Ey, mu, Pr = 0.4, -0.15, 0.3, 0.25, 0.02, 0.72
+    u, v = a * y, c * x
+    U = np.array([np.ones_like(x), u, v, 3.0 + Ex * x + Ey * y])
+    Fv, e, ar = viscous_part(m, U, viscosity_model=F.MU_CONSTANT, mu0=mu, Pr_viscous=Pr)
+    L, R = ar["edge_elementL"][e], ar["edge_elementR"][e]
+    uf, vf = 0.5 * (u[L] + u[R]), 0.5 * (v[L] + v[R])
+    nx, ny, ln = ar["edge_normal"][0][e], ar["edge_normal"][1][e], ar["edge_length"][e]
+    Rgas, g = 287.06, 1.4
+    Cv = Rgas / (g - 1)
+    k = mu * g * Cv / Pr
+    txy = mu * (a + c)
+    qx = k * (Ex - c * vf) / Cv + txy * vf
+    qy = k * (Ey - a * uf) / Cv + txy * uf
+    expect = (qx * nx + qy * ny) * ln
+    assert np.abs(Fv[1][e] - txy * ny * ln).max() < 1e-12 * abs(txy)
+    assert np.abs(Fv[2][e] - txy * nx * ln).max() < 1e-12 * abs(txy)
+    assert np.abs(Fv[3][e] - expect).max() < 1e-12 * np.abs(expect).max()
+
+
+@pytest.mark.parametrize("T0", [288.16, 250.0])
+def test_sutherland_viscosity(T0):
+    """Sutherland's law mu(T) = mu0 (T / T0)^(3/2) (T0 + S) / (T + S), S = 110.4 K
+    (ViscousFluxGPU.h:20-31): mu(T0) = mu0 and the law at other temperatures, read through the
+    heat flux of a fluid at rest (as above, gamma mu(T_f) / Pr grad(rho E) . n L) whose face
+    temperature T_f = mean(rho E) / Cv spans about 330-530 K over the channel."""
+    m = ns_mesh()
+    x, y = m.arrays()["element_xy"]
+    Rgas, g, mu0, Pr, S = 287.06, 1.4, 1.8e-5, 0.72, 110.4
+    Cv = Rgas / (g - 1)
+    Ex, Ey = 100.0 * Cv, 20.0 * Cv
+    U = np.array([np.ones_like(x), np.zeros_like(x), np.zeros_like(x), 330.0 * Cv + Ex * x + Ey * y])
+    Fv, e, ar = viscous_part(m, U, viscosity_model=F.MU_SUTHERLAND, mu0=mu0, T0=T0, Pr_viscous=Pr)
+    L, R = ar["edge_elementL"][e], ar["edge_elementR"][e]
+    Tf = 0.5 * (U[3][L] + U[3][R]) / Cv
+    assert Tf.min() < 360 and Tf.max() > 500
+    mu = mu0 * (Tf / T0) ** 1.5 * (T0 + S) / (Tf + S)
+    nx, ny, ln = ar["edge_normal"][0][e], ar["edge_normal"][1][e], ar["edge_length"][e]
+    expect = g * mu / Pr * (Ex * nx + Ey * ny) * ln
+    assert np.abs(Fv[3][e] - expect).max() < 1e-12 * np.abs(expect).max()
+    # at T = T0 the law gives mu0 itself
+    assert mu0 * (T0 / T0) ** 1.5 * (T0 + S) / (T0 + S) == mu0
+
+
+def periodic_shift_map(m, n, L):
+    """For every cell of the periodic square, the cell one quad further in x and in y (same
+    triangle shape, centroid + (h, h) modulo L)."""
+    h = L / n
+    x, y = m.arrays()["element_xy"]
+    kx, ky = np.rint(3 * x / h).astype(int), np.rint(3 * y / h).astype(int)
+    index = {(a, b): c for c, (a, b) in enumerate(zip(kx, ky))}
+    return np.array([index[((a + 3) % (3 * n), (b + 3) % (3 * n))] for a, b in zip(kx, ky)])
+
+
+def test_green_gauss_periodic_faces_follow_the_translation():
+    """On the periodic square the mesh is invariant under a shift by one quad, so the Green-Gauss
+    gradient of a periodic field shifted by one quad is the gradient of the field, shifted.
+    Cells on the periodic boundary see their neighbour across the periodic face through the
+    shift of its centroid by the face midpoint difference (GradientGPU.cu:332-339); a wrong
+    shift changes the Green-Gauss weight of those faces only, which breaks the invariance."""
+    n, Lbox = 12, 10.0
+    m = F.Mesh.generate(F.GEN_PERIODIC_SQUARE, n, n, b=Lbox)
+    x, y = m.arrays()["element_xy"]
+    k = 2 * np.pi / Lbox
+    U = np.array([1.0 + 0.2 * np.sin(k * x) * np.cos(k * y), 0.3 + 0.1 * np.cos(k * x),
+                  -0.2 + 0.1 * np.sin(k * (x + y)), 2.5 + 0.3 * np.cos(k * x) * np.sin(k * y)])
+    nxt = periodic_shift_map(m, n, Lbox)
+    U2 = np.empty_like(U)
+    U2[:, nxt] = U
+    p = F.default_params(inf_ruvp=[1, 1, 1, 1], gradient=F.GRA_GREENGAUSS, limiter=F.LIM_NONE)
+    s = O.OracleSolver(m, p)
+    s.set_state(U)
+    gx, gy = s.gradient()
+    s.set_state(U2)
+    gx2, gy2 = s.gradient()
+    scale = max(np.abs(gx).max(), np.abs(gy).max())
+    assert np.abs(gx2[:, nxt] - gx).max() < 1e-12 * scale
+    assert np.abs(gy2[:, nxt] - gy).max() < 1e-12 * scale
+    # and the weights matter: the field is not linear, so the gradients vary cell to cell
+    assert np.abs(gx - gx.mean(axis=1, keepdims=True)).max() > 0.1 * scale
+
+
+@pytest.mark.parametrize("K", [0.0, 3.0])
+def test_venkatakrishnan_published_form(K):
+    """Venkatakrishnan's limiter (AIAA 93-0880, eq. 9 with eps^2 = (K h)^3, h = sqrt(area)) per
+    vertex j of a cell: with D- = grad U . (x_j - x_c) and D+ = U_max - U_c (D- > 0) or
+    U_min - U_c (D- < 0), the extremes over the cell and its element neighbours,
+    phi_j = [(D+^2 + eps^2) D- + 2 D-^2 D+] / [(D+^2 + 2 D-^2 + D+ D- + eps^2) D-], phi_j = 1 where
+    D- = 0; the gradient is scaled by min(1, min_j phi_j).  On a random field: phi = limited /
+    unlimited gradient of the oracle in every cell and variable."""
+    m = F.Mesh.generate(F.GEN_RECT_CHANNEL, 14, 8, a=2.0, b=1.0, jitter=0.15, seed=21)
+    ar = m.arrays()
+    rng = np.random.default_rng(5)
+    nc = m.num_element
+    U = np.array([1.0, 0.4, -0.2, 2.5])[:, None] + 0.1 * rng.standard_normal((4, nc))
+    grads = {}
+    for lim in (F.LIM_NONE, F.LIM_VENKATAKRISHNAN):
+        p = F.default_params(gradient=F.GRA_LEASTSQUARE, limiter=lim, venkat_K=K)
+        s = O.OracleSolver(m, p)
+        s.set_state(U)
+        grads[lim] = s.gradient()
+    gx, gy = grads[F.LIM_NONE]
+    lx, ly = grads[F.LIM_VENKATAKRISHNAN]
+    xc, yc = ar["element_xy"]
+    nb = ar["element_neighbors"][:3]
+    eps2 = (K * np.sqrt(ar["element_volume"])) ** 3
+    checked = 0
+    for k in range(4):
+        vals = np.concatenate([U[k][None], np.where(nb >= 0, U[k][np.maximum(nb, 0)], U[k][None])])
+        hi, lo = vals.max(axis=0), vals.min(axis=0)
+        phi = np.ones(nc)
+        for j in range(3):
+            nd = ar["element_nodes"][j]
+            dm = gx[k] * (ar["node_xy"][0][nd] - xc) + gy[k] * (ar["node_xy"][1][nd] - yc)
+            dp = np.where(dm > 0, hi - U[k], lo - U[k])
+            with np.errstate(divide="ignore", invalid="ignore"):
+                pj = ((dp * dp + eps2) * dm + 2 * dm * dm * dp) / ((dp * dp + 2 * dm * dm + dp * dm + eps2) * dm)
+            phi = np.minimum(phi, np.where(dm != 0, pj, 1.0))
+        ok = np.abs(gx[k]) > 1e-8
+        assert np.allclose(lx[k][ok] / gx[k][ok], phi[ok], rtol=1e-10, atol=1e-12)
+        assert np.allclose(ly[k], gy[k] * phi, rtol=1e-10, atol=1e-14)
+        checked += ok.sum()
+        assert (phi < 0.9).sum() > 10    # the limiter is active in many cells
+    assert checked > 3 * nc
+
+
+def test_isothermal_wall_ghost():
+    """No-slip isothermal wall at T_w (new; the reference declares the code, FVMR/src/global/
+    Constexpr.h): the ghost state has the wall's pressure p_R = p_L and the reversed velocity, so
+    the face mean velocity is zero, and T_R = 2 T_w - T_L, so the face mean temperature is T_w
+    (T = p / (rho R)); the ghost temperature is held at T_w / 2 or above when T_L > 1.5 T_w."""
+    Rgas, g = 287.06, 1.4
+    for rho, u, v, p, Tw in [(1.2, 30.0, -12.0, 1.0e5, 300.0), (0.8, -5.0, 40.0, 9.0e4, 420.0),
+                             (1.0, 1.0, 1.0, 2.0e5, 300.0)]:
+        UL = state(rho, u, v, p, g)
+        UR = O.isothermal(UL, Tw, Rgas, g)
+        rl, rr = O.u2ruvp(UL, g), O.u2ruvp(UR, g)
+        TL, TR = rl[3] / (rl[0] * Rgas), rr[3] / (rr[0] * Rgas)
+        assert np.isclose(rr[3], p, rtol=1e-14)
+        assert np.allclose(rr[1:3], [-u, -v], rtol=1e-13)
+        if TL < 1.5 * Tw:
+            assert abs(0.5 * (TL + TR) - Tw) < 1e-12 * Tw
+        else:
+            assert abs(TR - 0.5 * Tw) < 1e-12 * Tw
