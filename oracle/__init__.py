@@ -32,6 +32,7 @@ _SIG = {
     "orc_field_init": (C.c_int, [C.POINTER(MeshSoA), C.POINTER(Params), C.c_int, _dp, _dp * 4]),
     "orc_solver_create": (C.c_void_p, [C.POINTER(MeshSoA), C.POINTER(Params)]),
     "orc_solver_free": (None, [C.c_void_p]),
+    "orc_set_threads": (C.c_int, [C.c_int]),
     "orc_set_state": (None, [C.c_void_p, _dp * 4]),
     "orc_get_state": (None, [C.c_void_p, _dp * 4]),
     "orc_gradient": (C.c_int, [C.c_void_p, _dp * 4, _dp * 4]),
@@ -54,29 +55,70 @@ _SIG = {
     "orc_vertex_colors": (None, [C.POINTER(MeshSoA), _dp * 4, C.c_double, C.c_int, C.c_float, C.c_float, C.c_int,
                                  C.POINTER(C.c_float), C.POINTER(C.c_float)]),
 }
-_lib = None
+_libs = {}
 
 
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def _cpu_tag():
+    """A short tag of this host's CPU (model and flags): the fast build is -march=native."""
+    import hashlib
+    info = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith(("model name", "flags")):
+                    info += line
+                if line.strip() == "" and info:
+                    break
+    except OSError:
+        pass
+    return hashlib.sha1(info.encode()).hexdigest()[:12]
+
+
+def fast_lib_path():
+    return os.path.join(HERE, "_build", "fast_" + _cpu_tag(), "libfvmr_oracle_fast.so")
+
+
+def build_fast():
+    """The -O3 -march=native -fopenmp build for this host's CPU (the benchmark's CPU baseline)."""
+    subprocess.run(["make", "-s", "-C", HERE, "fast", "FAST_LIB=" + fast_lib_path()], check=True)
+
+
+def _load(path):
+    L = C.CDLL(path)
+    for k, (r, a) in _SIG.items():
+        f = getattr(L, k)
+        f.restype = r
+        f.argtypes = a
+    return L
+
+
 def lib():
-    global _lib
-    if _lib is None:
+    """The plain build (one thread), which the tests use."""
+    if "plain" not in _libs:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        for k, (r, a) in _SIG.items():
-            f = getattr(L, k)
-            f.restype = r
-            f.argtypes = a
-        _lib = L
-    return _lib
+        _libs["plain"] = _load(LIB_PATH)
+    return _libs["plain"]
 
 
-def _err(what):
-    raise RuntimeError(f"oracle {what} failed: {lib().orc_last_error().decode()}")
+def fast_lib(threads=None):
+    """The OpenMP build for this host (built on first use); threads: OpenMP threads to use."""
+    if "fast" not in _libs:
+        path = fast_lib_path()
+        if not os.path.exists(path):
+            build_fast()
+        _libs["fast"] = _load(path)
+    if threads is not None:
+        _libs["fast"].orc_set_threads(int(threads))
+    return _libs["fast"]
+
+
+def _err(what, L=None):
+    raise RuntimeError(f"oracle {what} failed: {(L or lib()).orc_last_error().decode()}")
 
 
 def _dptr(a):
@@ -169,61 +211,63 @@ def field_init(soa, params, init_type, args=()):
 class OracleSolver:
     """FP64 CPU restatement of CIteration::iteration_host (forward Euler) plus SSP-RK3."""
 
-    def __init__(self, mesh, params):
+    def __init__(self, mesh, params, fast=False):
+        """fast: the OpenMP build (fast_lib), with the same results."""
         self.mesh = mesh
         self.n = mesh.soa.num_element
         self.nf = mesh.soa.num_edge
-        self._h = C.c_void_p(lib().orc_solver_create(C.byref(mesh.soa), C.byref(params)))
+        self._L = fast_lib() if fast else lib()
+        self._h = C.c_void_p(self._L.orc_solver_create(C.byref(mesh.soa), C.byref(params)))
 
     def __del__(self):
         if getattr(self, "_h", None) and self._h.value:
-            lib().orc_solver_free(self._h)
+            self._L.orc_solver_free(self._h)
             self._h = C.c_void_p()
 
     def set_state(self, U):
         U, rows = _rows(U)
-        lib().orc_set_state(self._h, rows)
+        self._L.orc_set_state(self._h, rows)
 
     def get_state(self):
         U, rows = _rows(np.zeros((4, self.n)))
-        lib().orc_get_state(self._h, rows)
+        self._L.orc_get_state(self._h, rows)
         return U
 
     def gradient(self):
         gx, rx = _rows(np.zeros((4, self.n)))
         gy, ry = _rows(np.zeros((4, self.n)))
-        if lib().orc_gradient(self._h, rx, ry):
-            _err("gradient")
+        if self._L.orc_gradient(self._h, rx, ry):
+            _err("gradient", self._L)
         return gx, gy
 
     def stage_rhs(self, t_rk=0.0):
         R, rows = _rows(np.zeros((4, self.n)))
-        if lib().orc_stage_rhs(self._h, t_rk, rows):
-            _err("stage_rhs")
+        if self._L.orc_stage_rhs(self._h, t_rk, rows):
+            _err("stage_rhs", self._L)
         return R
 
     def edge_flux(self):
         F, rows = _rows(np.zeros((4, self.nf)))
-        lib().orc_edge_flux(self._h, rows)
+        self._L.orc_edge_flux(self._h, rows)
         return F
 
     def global_dt(self):
-        return lib().orc_global_dt(self._h)
+        return self._L.orc_global_dt(self._h)
 
     def step(self, t, T, residual_norm=0):
         tt = C.c_double(t)
         info = StepInfo(residual_norm=residual_norm)
-        if lib().orc_step(self._h, C.byref(tt), T, C.byref(info)):
-            _err("step")
+        if self._L.orc_step(self._h, C.byref(tt), T, C.byref(info)):
+            _err("step", self._L)
         return tt.value, info
 
     def residual(self, norm=3):
         out = (C.c_double * 4)()
-        lib().orc_residual(self._h, norm, out)
+        self._L.orc_residual(self._h, norm, out)
         return np.array(out[:])
 
     def restrict(self):
-        return lib().orc_restrict(self._h)
+        return self._L.orc_restrict(self._h)
 
 
 def roe(UL, UR, nx, ny, length, gamma=1.4):

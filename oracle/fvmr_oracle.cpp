@@ -22,7 +22,11 @@
  * Parity status of each function is stated in the repository's design notes;
  * functions whose only check is a closed form say so in the tests.
  *
- * Build: see oracle/Makefile (g++ -O2 -ffp-contract=off, no fast-math).
+ * Build: see oracle/Makefile (g++ -O2 -ffp-contract=off, no fast-math).  The `fast` target
+ * (-O3 -march=native -fopenmp, still without contraction or fast-math) is the CPU baseline of the
+ * benchmark: its loops run over cells or edges with every write owned by one iteration, and the
+ * time-step sums become per-cell sums over the cell's edges in ascending edge order (the order in
+ * which the sequential edge loop adds them), so every thread count gives the same bits.
  */
 #include <cmath>
 #include <cstdio>
@@ -34,6 +38,9 @@
 #include <vector>
 #include <algorithm>
 #include <array>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "../include/fvmr.h"
 
@@ -51,6 +58,18 @@ constexpr double kP_MAX = 400 * 101325.0;
 constexpr double kP_MIN = kEPS;
 /* FVMR/src/space/viscous_flux/ViscousFluxGPU.h:11 */
 constexpr double kS_SUTHERLAND = 110.4;
+
+/* dst = src over n doubles (a loop the OpenMP build shares out; a copy in the plain build) */
+inline void copy_n(double *dst, const double *src, int n) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) dst[i] = src[i];
+}
+inline void copy_field(std::vector<double> *dst, const std::vector<double> *src, int n) {
+    for (int k = 0; k < 4; k++) {
+        dst[k].resize(n);
+        copy_n(dst[k].data(), src[k].data(), n);
+    }
+}
 
 inline double dmin(double a, double b) { return (a < b) ? a : b; }   /* BasicAlgorithm.h min */
 inline double dmax(double a, double b) { return (a > b) ? a : b; }   /* BasicAlgorithm.h max */
@@ -392,6 +411,8 @@ struct OSolver {
     std::vector<double> U[4], Ux[4], Uy[4], Flux[4], ruvp[4], Uold[4], alphaC, eFlux[4];
     double last_dt = 0;
     int n_restricted = 0;
+    std::vector<int> dt_off, dt_edge;     /* per cell, the edges adding to its alphaC, ascending */
+    std::vector<double> e_alpha, e_aV;    /* per edge: alpha and the viscous term */
 };
 
 /* FVMR/src/space/FluxGPU.cu:34-61: linear reconstruction with out-of-range fallback */
@@ -647,6 +668,7 @@ void edge_viscous(const OSolver &s, const double *const *U, int e, double F[4]) 
 
 /* GradientGreenGaussKernel (FVMR/src/space/gradient/GradientGPU.cu:223-425) */
 void gradient_gg(OSolver &s, const double *const *U) {
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < s.ne; c++) {
         double volumeC = s.evol[c];
         int eid[4], etype[4], esign[4] = {0, 0, 0, 0}, out[4] = {-1, -1, -1, -1};
@@ -721,10 +743,12 @@ void inv2x2(double *m, bool compat) {
 
 /* GradientLeastSquare_2 (FVMR/src/space/gradient/Gradient.cpp:160-246) */
 int gradient_lsq(OSolver &s, const double *const *U) {
+    int no_neighbour = 0;
+#pragma omp parallel for schedule(static) reduction(| : no_neighbour)
     for (int c = 0; c < s.ne; c++) {
         int vn[3], n = 0;
         for (int j = 0; j < 3; j++) if (s.enb[j][c] != -1) vn[n++] = s.enb[j][c];
-        if (n <= 0) return fail(FVMR_ERR_NUMERIC, "invalid numOfValidNeighbor");
+        if (n <= 0) { no_neighbour = 1; continue; }
         double dX[3][2], dU[3][4];
         for (int k = 0; k < n; k++) {
             dX[k][0] = s.ex[vn[k]] - s.ex[c];
@@ -757,11 +781,13 @@ int gradient_lsq(OSolver &s, const double *const *U) {
             s.Uy[v][c] = gy;
         }
     }
+    if (no_neighbour) return fail(FVMR_ERR_NUMERIC, "invalid numOfValidNeighbor");
     return FVMR_OK;
 }
 
 /* LimiterBarth (FVMR/src/space/gradient/Gradient.cpp:58-114; GradientGPU.cu:453-502) */
 void limiter_barth(OSolver &s, const double *const *U) {
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < s.ne; c++) {
         int vn[3], n = 0;
         for (int j = 0; j < 3; j++) if (s.enb[j][c] != -1) vn[n++] = s.enb[j][c];
@@ -787,6 +813,7 @@ void limiter_barth(OSolver &s, const double *const *U) {
 /* New: Venkatakrishnan (1993) limiter on the same neighbour/vertex sets as Barth,
    with eps^2 = (K h)^3, h = sqrt(area). */
 void limiter_venkat(OSolver &s, const double *const *U) {
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < s.ne; c++) {
         int vn[3], n = 0;
         for (int j = 0; j < 3; j++) if (s.enb[j][c] != -1) vn[n++] = s.enb[j][c];
@@ -833,12 +860,14 @@ int rhs(OSolver &s, const double *const *U, double t_rk) {
     int rc = gradient(s, U);
     if (rc) return rc;
     for (int k = 0; k < 4; k++) std::fill(s.Flux[k].begin(), s.Flux[k].end(), 0.0);
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < s.nf; e++) {
         double F[4];
         edge_flux(s, U, t_rk, e, F);
         for (int k = 0; k < 4; k++) s.eFlux[k][e] = F[k];
     }
     if (s.p.equation == FVMR_EQ_NS) {
+#pragma omp parallel for schedule(static)
         for (int e = 0; e < s.nf; e++) {
             double F[4];
             edge_viscous(s, U, e, F);
@@ -846,6 +875,7 @@ int rhs(OSolver &s, const double *const *U, double t_rk) {
         }
     }
     /* add_edge_flux_to_element_flux_kernel (FluxGPU.cu:285-323) */
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < s.ne; c++) {
         for (int i = 0; i < 4; i++) {
             int e = s.ee[i][c];
@@ -856,6 +886,7 @@ int rhs(OSolver &s, const double *const *U, double t_rk) {
     }
     /* vector_weighted_divide_kernel (BasicAlgorithmGPU.cu:4-9) with weight -1 */
     for (int k = 0; k < 4; k++)
+#pragma omp parallel for schedule(static)
         for (int c = 0; c < s.ne; c++) s.Flux[k][c] /= -1.0 * s.evol[c];
     return FVMR_OK;
 }
@@ -863,6 +894,7 @@ int rhs(OSolver &s, const double *const *U, double t_rk) {
 /* update_alphaC_host + get_minimus_dt_by_alphaC (FVMR/src/time/CalculateDt.cpp:10-85) */
 double global_dt(OSolver &s, const double *const *U) {
     for (int k = 0; k < 4; k++)
+#pragma omp parallel for schedule(static)
         for (int c = 0; c < s.ne; c++) {
             /* CIteration::update_host_ruvp_Uold (CIteration.cpp:163-180) */
             double Uc[4] = {U[0][c], U[1][c], U[2][c], U[3][c]}, r[4];
@@ -871,8 +903,9 @@ double global_dt(OSolver &s, const double *const *U) {
         }
     const double gamma = s.p.gamma;
     const double value_1 = dmax(4.0 / 3.0, gamma / s.p.Pr);
-    std::fill(s.alphaC.begin(), s.alphaC.end(), 0.0);
-    for (int i = 0; i < s.nf; i++) {
+    const bool ns = s.p.equation == FVMR_EQ_NS;
+    /* per edge: the spectral radius and the viscous term */
+    auto edge_terms = [&](int i, double &alpha, double &aV) {
         int L = s.gL[i], R = s.gR[i];
         double rho, u, v, p;
         if (R != -1) {
@@ -887,23 +920,75 @@ double global_dt(OSolver &s, const double *const *U) {
         double dy = s.gny[i] * s.glen[i];
         const double length = s.glen[i];
         double unormal = u * dx + v * dy;
-        double alpha = std::fabs(unormal) + std::sqrt(gamma * p / rho) * std::sqrt(length);
-        s.alphaC[L] += alpha;
-        if (R != -1) s.alphaC[R] += alpha;
-        if (s.p.equation == FVMR_EQ_NS) {
+        alpha = std::fabs(unormal) + std::sqrt(gamma * p / rho) * std::sqrt(length);
+        aV = 0.0;
+        if (ns) {
             double T = p / s.p.R / rho;
             double mu = mu_of_T(s, T);
-            double aV = 2.0 * length * value_1 * mu / rho;
-            s.alphaC[L] += aV / s.evol[L];
-            if (R != -1) s.alphaC[R] += aV / s.evol[R];
+            aV = 2.0 * length * value_1 * mu / rho;
+        }
+    };
+    int nthreads = 1;
+#ifdef _OPENMP
+    nthreads = omp_get_max_threads();
+#endif
+    if (nthreads == 1) {
+        std::fill(s.alphaC.begin(), s.alphaC.end(), 0.0);
+        for (int i = 0; i < s.nf; i++) {
+            int L = s.gL[i], R = s.gR[i];
+            double alpha, aV;
+            edge_terms(i, alpha, aV);
+            s.alphaC[L] += alpha;
+            if (R != -1) s.alphaC[R] += alpha;
+            if (ns) {
+                s.alphaC[L] += aV / s.evol[L];
+                if (R != -1) s.alphaC[R] += aV / s.evol[R];
+            }
+        }
+    } else {
+        /* the same sums, cell-owned: each cell adds its edges' terms in ascending edge order */
+        if (s.dt_off.empty()) {
+            s.dt_off.assign(s.ne + 1, 0);
+            for (int i = 0; i < s.nf; i++) {
+                s.dt_off[s.gL[i] + 1]++;
+                if (s.gR[i] != -1) s.dt_off[s.gR[i] + 1]++;
+            }
+            for (int c = 0; c < s.ne; c++) s.dt_off[c + 1] += s.dt_off[c];
+            s.dt_edge.assign(s.dt_off[s.ne], 0);
+            std::vector<int> fill(s.dt_off.begin(), s.dt_off.end() - 1);
+            for (int i = 0; i < s.nf; i++) {
+                s.dt_edge[fill[s.gL[i]]++] = i;
+                if (s.gR[i] != -1) s.dt_edge[fill[s.gR[i]]++] = i;
+            }
+            s.e_alpha.assign(s.nf, 0.0);
+            s.e_aV.assign(s.nf, 0.0);
+        }
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < s.nf; i++) edge_terms(i, s.e_alpha[i], s.e_aV[i]);
+#pragma omp parallel for schedule(static)
+        for (int c = 0; c < s.ne; c++) {
+            double a = 0.0;
+            for (int k = s.dt_off[c]; k < s.dt_off[c + 1]; k++) {
+                const int i = s.dt_edge[k];
+                a += s.e_alpha[i];
+                if (ns) a += s.e_aV[i] / s.evol[c];
+            }
+            s.alphaC[c] = a;
         }
     }
     /* the minimum skips NaN cells, as the device reduction's operator_min
        (BasicAlgorithmGPU.h:38-40, "if (a > b) a = b") does for a finite running value */
     double dt = 1e10;
-    for (int c = 0; c < s.ne; c++) {
-        double v = s.p.CFL * s.evol[c] / s.alphaC[c];
-        if (dt > v) dt = v;
+#pragma omp parallel
+    {
+        double dt_t = 1e10;
+#pragma omp for schedule(static) nowait
+        for (int c = 0; c < s.ne; c++) {
+            double v = s.p.CFL * s.evol[c] / s.alphaC[c];
+            if (dt_t > v) dt_t = v;
+        }
+#pragma omp critical
+        if (dt > dt_t) dt = dt_t;
     }
     return dt;
 }
@@ -911,8 +996,12 @@ double global_dt(OSolver &s, const double *const *U) {
 /* modifyElementFieldU2d (Restrict.cpp:7-90 / RestrictGPU.cu:6-84), Jacobi reads */
 int restrict_field(OSolver &s, double *const *U) {
     std::vector<double> snap[4];
-    for (int k = 0; k < 4; k++) snap[k].assign(U[k], U[k] + s.ne);
+    for (int k = 0; k < 4; k++) {
+        snap[k].resize(s.ne);
+        copy_n(snap[k].data(), U[k], s.ne);
+    }
     int count = 0;
+#pragma omp parallel for schedule(static) reduction(+ : count)
     for (int c = 0; c < s.ne; c++) {
         double Uc[4] = {snap[0][c], snap[1][c], snap[2][c], snap[3][c]}, r[4];
         U2ruvp(Uc, r, s.p.gamma);
@@ -1068,6 +1157,17 @@ void *orc_solver_create(const fvmr_mesh_soa *m, const fvmr_params *p) {
 }
 void orc_solver_free(void *h) { delete (OSolver *)h; }
 
+/* Threads of the OpenMP build (the plain build has one); returns the count in use. */
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
 void orc_set_state(void *h, const double *const *U) {
     OSolver *s = (OSolver *)h;
     for (int k = 0; k < 4; k++) std::copy(U[k], U[k] + s->ne, s->U[k].begin());
@@ -1117,7 +1217,7 @@ double orc_global_dt(void *h) {
 int orc_step(void *h, double *t, double T, fvmr_step_info *info) {
     OSolver *s = (OSolver *)h;
     const int ne = s->ne;
-    for (int k = 0; k < 4; k++) s->Uold[k] = s->U[k];
+    copy_field(s->Uold, s->U, ne);
     const double *Up[4] = {s->U[0].data(), s->U[1].data(), s->U[2].data(), s->U[3].data()};
     double dt = global_dt(*s, Up);
     if (*t + dt > T) dt = T - *t;
@@ -1129,9 +1229,10 @@ int orc_step(void *h, double *t, double T, fvmr_step_info *info) {
     int nres = 0;
     if (s->p.time_advance == FVMR_EVO_EXPLICIT) {
         if ((rc = rhs(*s, Up, t_rk))) return rc;
-        for (int k = 0; k < 4; k++) rhs1[k] = s->Flux[k];
+        copy_field(rhs1, s->Flux, ne);
         /* vector_weighted_add_kernel (BasicAlgorithmGPU.cu:18-23) */
         for (int k = 0; k < 4; k++)
+#pragma omp parallel for schedule(static)
             for (int c = 0; c < ne; c++) s->U[k][c] += dt * s->Flux[k][c];
         nres += restrict_field(*s, Uw);
     } else if (s->p.time_advance == FVMR_EVO_RK3) {
@@ -1139,8 +1240,9 @@ int orc_step(void *h, double *t, double T, fvmr_step_info *info) {
         const double b[3] = {1.0, 0.25, 2.0 / 3.0};
         for (int st = 0; st < 3; st++) {
             if ((rc = rhs(*s, Up, t_rk))) return rc;
-            if (st == 0) for (int k = 0; k < 4; k++) rhs1[k] = s->Flux[k];
+            if (st == 0) copy_field(rhs1, s->Flux, ne);
             for (int k = 0; k < 4; k++)
+#pragma omp parallel for schedule(static)
                 for (int c = 0; c < ne; c++) {
                     double stage = s->U[k][c] + dt * s->Flux[k][c];
                     s->U[k][c] = (st == 0) ? stage : a[st] * s->Uold[k][c] + b[st] * stage;
@@ -1151,7 +1253,7 @@ int orc_step(void *h, double *t, double T, fvmr_step_info *info) {
     } else {
         return fail(FVMR_ERR_UNSUPPORTED, "invalid temporal_choice");
     }
-    for (int k = 0; k < 4; k++) s->Flux[k] = rhs1[k];
+    copy_field(s->Flux, rhs1, ne);
     s->last_dt = dt;
     s->n_restricted = nres;
     if (info) {
