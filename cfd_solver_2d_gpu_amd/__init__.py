@@ -324,6 +324,18 @@ class GPUSolver2:
                                        _dptr(cy)), "fvmr_surface_force")
         return cx[:len(sets)], cy[:len(sets)]
 
+    def surface_force_noalloc(self, S_ref=1.0, sets=None):
+        """surface_force through fvmr_surface_force_noalloc: the same coefficients, bit for bit,
+        with no device allocation (at most 256 sets and 131072 edges per call)."""
+        if sets is None:
+            sets = [s[2] for s in self.mesh.boundary_sets()]
+        off, edges = self._sets(sets)
+        cx = np.zeros(max(1, len(sets)))
+        cy = np.zeros(max(1, len(sets)))
+        check(lib().fvmr_surface_force_noalloc(self._h, len(sets), _iptr(off), _iptr(edges), float(S_ref),
+                                               _dptr(cx), _dptr(cy)), "fvmr_surface_force_noalloc")
+        return cx[:len(sets)], cy[:len(sets)]
+
     # CudaGrid::CudaGrid's colour map (FVMR/src/gui/cuda/CudaGrid.cpp:33-39): {r, g, b, position}
     DEFAULT_COLORMAP = [(1.0, 0.0, 0.0, 0.00), (1.0, 1.0, 0.0, 0.25), (0.0, 1.0, 0.0, 0.50),
                         (0.0, 0.0, 1.0, 0.75), (1.0, 0.0, 1.0, 1.00)]

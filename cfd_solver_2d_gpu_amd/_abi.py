@@ -129,6 +129,7 @@ SIGNATURES = {
     "fvmr_batch_get_state": (C.c_int, [C.c_void_p, C.c_int, _dp * 4]),
     "fvmr_batch_advance": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "fvmr_surface_force": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, C.c_double, _dp, _dp]),
+    "fvmr_surface_force_noalloc": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, C.c_double, _dp, _dp]),
     "fvmr_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "fvmr_profile_query": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip]),
     "fvmr_profile_reset": (C.c_int, [C.c_void_p]),

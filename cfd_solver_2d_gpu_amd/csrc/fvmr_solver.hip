@@ -30,6 +30,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <type_traits>
@@ -1477,14 +1478,10 @@ __global__ __launch_bounds__(TileCfg<G>::kThreads) void k_tile_dt(TileArgs<G> ta
 // update_nodeField_ruvp + update_nodeField_other_variables (FieldWriter.cpp:371-452): one
 // thread per node, the cells of the node in the caller's element order, U2ruvp as the
 // reference writes it (PhysicalKernelGPU.h:11-16), no contraction.  out: ruvp (4 nv), Cp, Ma.
-__global__ __launch_bounds__(kBlock) void k_node_field(int nv, const int *__restrict__ off,
-                                                        const int *__restrict__ cells, const S4 *__restrict__ U,
-                                                        double gamma, double p_inf, double p_dyn,
-                                                        double *__restrict__ out) {
+__device__ __forceinline__ void node_mean_ruvp(int v, const int *__restrict__ off, const int *__restrict__ cells,
+                                               const S4 *__restrict__ U, double gamma, double sum[4]) {
 #pragma clang fp contract(off)
-    const int v = blockIdx.x * kBlock + threadIdx.x;
-    if (v >= nv) return;
-    double sum[4] = {0.0, 0.0, 0.0, 0.0};
+    sum[0] = sum[1] = sum[2] = sum[3] = 0.0;
     const int k0 = off[v], k1 = off[v + 1];
     for (int k = k0; k < k1; k++) {
         const S4 u = U[cells[k]];
@@ -1498,9 +1495,25 @@ __global__ __launch_bounds__(kBlock) void k_node_field(int nv, const int *__rest
     const int n = k1 - k0;
     if (n > 0)
         for (int q = 0; q < 4; q++) sum[q] /= n;
+}
+// Pressure coefficient of a node mean pressure (update_nodeField_other_variables, FieldWriter.cpp:420-452).
+__device__ __forceinline__ double node_cp(double p, double p_inf, double p_dyn) {
+#pragma clang fp contract(off)
+    return (p - p_inf) / p_dyn;
+}
+
+__global__ __launch_bounds__(kBlock) void k_node_field(int nv, const int *__restrict__ off,
+                                                        const int *__restrict__ cells, const S4 *__restrict__ U,
+                                                        double gamma, double p_inf, double p_dyn,
+                                                        double *__restrict__ out) {
+#pragma clang fp contract(off)
+    const int v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= nv) return;
+    double sum[4];
+    node_mean_ruvp(v, off, cells, U, gamma, sum);
     for (int q = 0; q < 4; q++) out[(size_t)q * nv + v] = sum[q];
     const double rho = sum[0], u = sum[1], w = sum[2], p = sum[3];
-    out[(size_t)4 * nv + v] = (p - p_inf) / p_dyn;
+    out[(size_t)4 * nv + v] = node_cp(p, p_inf, p_dyn);
     const double V2 = u * u + w * w;
     const double a2 = gamma * p / rho;
     out[(size_t)5 * nv + v] = sqrt(V2 / a2);
@@ -1591,6 +1604,44 @@ __global__ void k_surface_force(int nsets, const int *__restrict__ off, const in
     }
     out[2 * s] = cd;
     out[2 * s + 1] = cl;
+}
+
+// k_surface_force without the node-field buffer: the caller's edge ids of set s,
+// edges[off[s] .. off[s+1]-1], are looked up in the handle's boundary-edge table (bref
+// ascending, nb entries; every id was checked on the host to be there) and the Cp of each
+// edge node is formed where it is used, with the arithmetic of k_node_field, so the sums
+// equal k_surface_force's bit for bit.  Results: cx[s], cy[s].
+__global__ void k_surface_force_direct(int nsets, const int *__restrict__ off, const int *__restrict__ edges, int nb,
+                                       const int *__restrict__ bref, const int2 *__restrict__ bnodes,
+                                       const V4<double> *__restrict__ bgeo, const int *__restrict__ noff,
+                                       const int *__restrict__ ncells, const S4 *__restrict__ U, double gamma,
+                                       double p_inf, double p_dyn, double S_ref, double *__restrict__ cx,
+                                       double *__restrict__ cy) {
+#pragma clang fp contract(off)
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsets) return;
+    const double one_on_Sref = 1 / S_ref;
+    double cd = 0.0, cl = 0.0;
+    for (int k = off[s]; k < off[s + 1]; k++) {
+        const int e = edges[k];
+        int lo = 0, hi = nb - 1;   // bref[lo] <= e <= bref[hi] holds for every listed id
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (bref[mid] < e) lo = mid + 1;
+            else hi = mid;
+        }
+        const int2 nd = bnodes[lo];
+        const V4<double> g = bgeo[lo];
+        double m0[4], m1[4];
+        node_mean_ruvp(nd.x, noff, ncells, U, gamma, m0);
+        node_mean_ruvp(nd.y, noff, ncells, U, gamma, m1);
+        const double cp = 0.5 * (node_cp(m0[3], p_inf, p_dyn) + node_cp(m1[3], p_inf, p_dyn));
+        const double dC = cp * one_on_Sref * g.z;
+        cd += dC * g.x;
+        cl += dC * g.y;
+    }
+    cx[s] = cd;
+    cy[s] = cl;
 }
 
 // Device order -> the caller's order (out: 4 x nc, component-major) and back, for the nc cells
@@ -1838,6 +1889,45 @@ struct Prof {
 
 }  // namespace
 
+// Staging of fvmr_surface_force_noalloc, which allocates nothing: per call the set offsets,
+// the caller's edge ids and the per-set results.  One copy per device, held until the process
+// ends; every call on a device shares it (calls that overlap on one device race).  Host code
+// never uses this name as an address: it uses the device address taken once per device at
+// fvmr_create (scratch_ptr).
+constexpr int kNoAllocMaxSets = 256;
+constexpr int kNoAllocMaxEdges = 131072;
+constexpr size_t kScratchOffInts = 0;                                   // int[kNoAllocMaxSets + 1]
+constexpr size_t kScratchEdgeInts = 512;                                // int[kNoAllocMaxEdges]
+constexpr size_t kScratchOutDoubles = (kScratchEdgeInts + kNoAllocMaxEdges) / 2;   // double[2 kNoAllocMaxSets]
+constexpr size_t kScratchDoubles = kScratchOutDoubles + 2 * kNoAllocMaxSets;
+static_assert(kNoAllocMaxSets + 1 <= kScratchEdgeInts && (kScratchEdgeInts + kNoAllocMaxEdges) % 2 == 0,
+              "scratch regions overlap");
+static __device__ __align__(256) double scratch[kScratchDoubles];   // 1 MiB + 8 KiB
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, taken once per device.
+int scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return set_error(FVMR_ERR_UNSUPPORTED, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return FVMR_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_error(FVMR_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (double *)p;
+    return FVMR_OK;
+}
+double *scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
 struct fvmr_solver {
     int nc = 0, nf = 0, nv = 0, nb = 0;
     fvmr_params p{};
@@ -1885,6 +1975,9 @@ struct fvmr_solver {
     std::vector<int> bedge_ref;               // caller ids of boundary-set edges, ascending
     std::vector<int> bedge_nodes;             // 2 per boundary edge
     std::vector<double> bedge_geo;            // nx, ny, L per boundary edge
+    int *bedge_ref_dev = nullptr;             // the same three tables on the device
+    int2 *bedge_nodes_dev = nullptr;
+    V4<double> *bedge_geo_dev = nullptr;
     std::vector<int> enodes;                  // 2 per device edge: its nodes in the caller's order
     int bc_flags = 0;                         // boundary kinds present (periodic, DSR, isothermal)
     void *adj = nullptr;                      // steady-solve / adjoint workspace
@@ -2162,7 +2255,8 @@ void free_all(fvmr_solver *h) {
     void *ptrs[] = {h->grad, h->rhs, h->fix, h->cgeo, h->nxy, h->egeo, h->emid, h->cslot, h->cnode, h->dtcode,
                     h->eint, h->partial, h->norm_out, h->bad_list, h->sc, h->t_info, h->t_halo, h->t_lnodes,
                     h->t_nxy, h->t_gslot, h->t_face, h->t_gcoef, h->t_dtrec,
-                    h->nc_off, h->nc_cells, h->node_buf, h->cbc, h->tile_case_begin, h->perm_dev, h->io_soa};
+                    h->nc_off, h->nc_cells, h->node_buf, h->cbc, h->tile_case_begin, h->perm_dev, h->io_soa,
+                    h->bedge_ref_dev, h->bedge_nodes_dev, h->bedge_geo_dev};
     for (void *q : ptrs) if (q) (void)hipFree(q);
     for (int k = 0; k < kNumProf; k++)
         for (auto e : h->pending[k]) (void)hipEventDestroy(e);
@@ -2373,6 +2467,19 @@ static int create_impl(const fvmr_mesh_soa *m, const fvmr_params *p, int ncases,
             h->bedge_geo.push_back(m->edge_normal[1][e]);
             h->bedge_geo.push_back(m->edge_length[e]);
         }
+        const size_t nbe = h->bedge_ref.size();
+        std::vector<int2> bn(nbe);
+        std::vector<V4<double>> bg(nbe);
+        for (size_t b = 0; b < nbe; b++) {
+            bn[b] = make_int2(h->bedge_nodes[2 * b], h->bedge_nodes[2 * b + 1]);
+            bg[b] = V4<double>{h->bedge_geo[3 * b], h->bedge_geo[3 * b + 1], h->bedge_geo[3 * b + 2], 0.0};
+        }
+        if ((rc = upload(&h->bedge_ref_dev, h->bedge_ref, h->stream)) || (rc = upload(&h->bedge_nodes_dev, bn, h->stream)) ||
+            (rc = upload(&h->bedge_geo_dev, bg, h->stream)))
+            return bail(rc);
+        if (hipStreamSynchronize(h->stream) != hipSuccess) return bail(set_error(FVMR_ERR_HIP, "boundary table upload failed"));
+        // the allocation-free entry point's staging array on this device
+        if ((rc = scratch_init(h->device))) return bail(rc);
     }
     // tile records
     fvmr::TileMesh tm;
@@ -2915,6 +3022,46 @@ int fvmr_surface_force(fvmr_handle *h, int nsets, const int *set_offsets, const 
     release();
     if (e != hipSuccess) return set_error(FVMR_ERR_HIP, std::string("surface force: ") + hipGetErrorString(e));
     for (int s = 0; s < nsets; s++) { Cx[s] = out[2 * s]; Cy[s] = out[2 * s + 1]; }
+    return FVMR_OK;
+}
+
+// fvmr_surface_force with no allocation: the caller's set offsets and edge ids are copied into
+// the device's `scratch` array (through scratch_ptr, taken at fvmr_create), the node Cp values
+// are formed inside the kernel, and the results come back from the same array.
+int fvmr_surface_force_noalloc(fvmr_handle *h, int nsets, const int *set_offsets, const int *set_edges,
+                               double S_ref, double *Cx, double *Cy) {
+    if (!h || nsets < 0 || (nsets > 0 && (!set_offsets || !Cx || !Cy))) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
+    if (nsets == 0) return FVMR_OK;
+    if (nsets > kNoAllocMaxSets)
+        return set_error(FVMR_ERR_INVALID_ARG, "fvmr_surface_force_noalloc: at most " + std::to_string(kNoAllocMaxSets) +
+                                                   " sets per call (asked for " + std::to_string(nsets) + ")");
+    if (set_offsets[0] != 0) return set_error(FVMR_ERR_INVALID_ARG, "set_offsets[0] must be 0");
+    for (int s = 0; s < nsets; s++)
+        if (set_offsets[s + 1] < set_offsets[s]) return set_error(FVMR_ERR_INVALID_ARG, "set_offsets must not decrease");
+    const int ne = set_offsets[nsets];
+    if (ne > kNoAllocMaxEdges)
+        return set_error(FVMR_ERR_INVALID_ARG, "fvmr_surface_force_noalloc: at most " + std::to_string(kNoAllocMaxEdges) +
+                                                   " edges per call (asked for " + std::to_string(ne) + ")");
+    if (ne > 0 && !set_edges) return set_error(FVMR_ERR_INVALID_ARG, "null set_edges");
+    for (int k = 0; k < ne; k++)
+        if (!std::binary_search(h->bedge_ref.begin(), h->bedge_ref.end(), set_edges[k]))
+            return set_error(FVMR_ERR_INVALID_ARG, "surface force: edge " + std::to_string(set_edges[k]) + " is not in a boundary set");
+    double *base = scratch_ptr(h->device);
+    if (!base) return set_error(FVMR_ERR_HIP, "fvmr_surface_force_noalloc: scratch of this device not initialised");
+    int *d_off = reinterpret_cast<int *>(base) + kScratchOffInts;
+    int *d_edges = reinterpret_cast<int *>(base) + kScratchEdgeInts;
+    double *d_cx = base + kScratchOutDoubles, *d_cy = d_cx + kNoAllocMaxSets;
+    HIP_OK(hipMemcpyAsync(d_off, set_offsets, (size_t)(nsets + 1) * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    if (ne > 0) HIP_OK(hipMemcpyAsync(d_edges, set_edges, (size_t)ne * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    const double *inf = h->p.inf_ruvp;
+    const double V2 = inf[1] * inf[1] + inf[2] * inf[2];
+    k_surface_force_direct<<<(nsets + 63) / 64, 64, 0, h->stream>>>(
+        nsets, d_off, d_edges, (int)h->bedge_ref.size(), h->bedge_ref_dev, h->bedge_nodes_dev, h->bedge_geo_dev,
+        h->nc_off, h->nc_cells, h->Ubuf[h->cur], h->p.gamma, inf[3], 0.5 * inf[0] * V2, S_ref, d_cx, d_cy);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(Cx, d_cx, (size_t)nsets * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipMemcpyAsync(Cy, d_cy, (size_t)nsets * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
     return FVMR_OK;
 }
 

@@ -303,6 +303,15 @@ int fvmr_node_field(fvmr_handle *h, double *const ruvp[4], double *Cp, double *M
  * Cy[s] likewise with ny (CL).  Edges must belong to a boundary set. */
 int fvmr_surface_force(fvmr_handle *h, int nsets, const int *set_offsets, const int *set_edges, double S_ref,
                        double *Cx, double *Cy);
+/* fvmr_surface_force with the same arguments and results, bit for bit, that allocates nothing
+ * (no hipMalloc, hipFree or allocating library call): no counterpart in the reference; it is for
+ * callers that cannot allocate device memory (a lift evaluated inside a captured or
+ * allocation-free loop).  Its staging (offsets, edge ids, results) lives in a fixed device array
+ * of the library, one per device, held until the process ends: at most 256 sets and 131072 edges
+ * per call (larger calls return FVMR_ERR_INVALID_ARG).  Calls on one device share that array,
+ * so calls that overlap in time on one device (other threads or streams) race. */
+int fvmr_surface_force_noalloc(fvmr_handle *h, int nsets, const int *set_offsets, const int *set_edges,
+                               double S_ref, double *Cx, double *Cy);
 
 /* GUI vertex colours of a cell quantity (GPU::setVerticesColor, CudaGrid_kernels.cu:185-232, with
  * getColor_device :22-65 and the kernels :77-123): per node the largest of q_c * n over its n
