@@ -4,17 +4,24 @@ Workload: M = 2 Euler flow over a 15 deg ramp in a [0,3]x[0,1] channel, 1024x512
 split into 1,048,576 triangles, interior nodes jittered by +-10 % of the local spacing
 (splitmix64 seed 2024), inlet/outlet/slip walls, Roe flux, Green-Gauss gradient + Barth
 limiter, linear reconstruction, SSP-RK3, CFL 0.5, FP64.  A step is one SSP-RK3 step
-(3 stages); value = cells x 3 x steps / wall time of the timed steps.
+(3 stages).  After an untimed clock warm-up (1000 steps) and the --warmup steps, --windows
+(at least 5) windows of exactly --steps steps are timed, each between two device
+synchronisations; value = cells x 3 x steps / the median window's wall time, with the
+windows' spread reported beside it.
 
-Per step the tile kernels run k_tile_dt + k_dt_finish (global dt) and 3 x (k_tile +
-k_restrict); k_tile is one whole RK stage (gradients, fluxes, update).  Its roofline counts
+Per step the tile kernels run k_tile_dt + k_dt_finish (global dt), 3 x k_tile and one
+k_restrict; k_tile is one whole RK stage (gradients, fluxes, update).  Its roofline counts
 the algorithmic bytes of one RK stage, B_stage = n_c(12 s_U + 16 s_G + 3 s_X + 24) +
-n_f(8 + 5 s_X) + n_v 2 s_X, against its average launch time from HIP events; the measured
-HBM traffic per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated) is reported beside.
+n_f(8 + 5 s_X) + n_v 2 s_X, against its average launch time from HIP events on the handle's
+stream in a window right after the timed ones; the measured HBM traffic per launch (rocprofv3
+FETCH_SIZE/WRITE_SIZE, copy-calibrated) is reported beside.
 
-Also reported: the same run in FP32-mixed precision (speed-up and relative L2 density
-error against FP64 after the same steps), the LLF flux, the cell-kernel family on the same
-workload, and the CPU oracle timed on a bounded sample.
+Also reported: the same windows in FP32-mixed precision and a fixed 1000-step run of both
+precisions from the same field (speed-up, relative L2/Linf of rho, rho u, rho v, rho E against
+FP64), the LLF flux, the cell-kernel family, the C1 vortex (100 and 1000 steps), the reference's
+quick_start Neuvillette case, SU2 ingest at 1M cells, the C3 Navier-Stokes cylinder, the C4
+batch (tile size from the library's tuner, mixed-vs-FP64 errors on sampled cases), the C5
+adjoint, and the CPU oracle (-O3 -march=native -fopenmp) on one thread and on all threads.
 """
 import argparse
 import json
@@ -69,21 +76,45 @@ KERNEL_SLOTS = {  # profile slot -> kernel name, per kernel family
 }
 
 
-def run_gpu(scheme, precision, steps, warmup, profile, kernels=None):
+CLOCK_WARMUP_STEPS = 1000
+
+
+def run_gpu(scheme, precision, steps, warmup, profile, kernels=None, reps=1, clock_warmup=CLOCK_WARMUP_STEPS):
+    """clock_warmup + warmup untimed steps, then `reps` timed windows of `steps` steps each (the
+    device is synchronised before and after every window), then, right after them, one window
+    with HIP events around every launch.  The GPU raises its clocks over tens of milliseconds of
+    sustained load and lowers them within milliseconds of idling (the stage kernel takes 72 us at
+    the start of a run and 58-60 us after about 30 ms), so the untimed clock warm-up comes first
+    and the profiled window follows the timed ones without a pause.  Returns the mesh, the
+    window wall times, the state after the timed windows and the per-kernel event times."""
     import cfd_solver_2d_gpu_amd as F
     m, p, U = make_case(scheme, precision, kernels)
     g = F.GPUSolver2(m, p)
     g.set_state(U)
     t = 0.0
+    if clock_warmup:
+        t, _ = g.advance(clock_warmup, t, 1e9)
     if warmup:
         t, _ = g.advance(warmup, t, 1e9)
     g.synchronize()
-    t0 = time.perf_counter()
-    t, _ = g.advance(steps, t, 1e9)
-    g.synchronize()
-    wall = time.perf_counter() - t0
-    state = g.get_state()
+    walls = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        t, _ = g.advance(steps, t, 1e9)
+        g.synchronize()
+        walls.append(time.perf_counter() - t0)
     kern = {}
+    if profile:
+        g.profile(True)
+        g.profile_reset()
+        t, _ = g.advance(steps, t, 1e9)
+        g.synchronize()
+        names = KERNEL_SLOTS["cell" if kernels == F.KERNELS_CELL else "tile"]
+        for k, name in names.items():
+            ms, n = g.profile_query(k)
+            kern[name] = (ms, n)
+        g.profile(False)
+    state = g.get_state()
     # the boundary takes and returns host arrays (caller's element order): one state each way
     t1 = time.perf_counter()
     g.set_state(state)
@@ -92,19 +123,36 @@ def run_gpu(scheme, precision, steps, warmup, profile, kernels=None):
     t1 = time.perf_counter()
     g.get_state()
     kern["_d2h_s"] = time.perf_counter() - t1
-    if profile:
-        # second pass over the same number of steps with HIP events around every launch
-        g.profile(True)
-        g.profile_reset()
-        g.advance(steps, t, 1e9)
-        g.synchronize()
-        names = KERNEL_SLOTS["cell" if kernels == F.KERNELS_CELL else "tile"]
-        for k, name in names.items():
-            ms, n = g.profile_query(k)
-            kern[name] = (ms, n)
-        g.profile(False)
     g.close()
-    return m, wall, state, kern
+    return m, walls, state, kern
+
+
+def window_stats(walls, steps):
+    """Median and spread of the timed windows, in ms per step."""
+    ms = sorted(w / steps * 1e3 for w in walls)
+    med = float(np.median(ms))
+    return {"count": len(ms), "steps_each": steps, "ms_per_step_sorted": [round(x, 5) for x in ms],
+            "median_ms_per_step": med, "min_ms_per_step": ms[0], "max_ms_per_step": ms[-1],
+            "spread_pct": round(100.0 * (ms[-1] - ms[0]) / med, 2)}
+
+
+def accuracy_run(scheme, precisions, steps, kernels=None):
+    """`steps` steps of the same initial field in each precision, each as one timed advance:
+    (ms per step, state) per precision."""
+    import cfd_solver_2d_gpu_amd as F
+    out = {}
+    for prec in precisions:
+        m, p, U = make_case(scheme, prec, kernels)
+        g = F.GPUSolver2(m, p)
+        g.set_state(U)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.advance(steps, 0.0, 1e9)
+        g.synchronize()
+        wall = time.perf_counter() - t0
+        out[prec] = (wall / steps * 1e3, g.get_state())
+        g.close()
+    return out
 
 
 def traffic_from_profiles(kernel="k_tile"):
@@ -158,20 +206,24 @@ def config_c1():
     import cfd_solver_2d_gpu_amd as F
     m = F.Mesh.generate(F.GEN_PERIODIC_SQUARE, 64, 64, a=0.0, b=10.0)
     res = {"cells": m.num_element}
-    outs = {}
-    for prec in (F.PREC_FP64, F.PREC_MIXED):
-        p = F.default_params(inf_ruvp=[1, 1, 1, 1], CFL=0.5, gradient=F.GRA_LEASTSQUARE, limiter=F.LIM_BARTH,
-                             flux_scheme=F.SOL_ROE, time_advance=F.EVO_RK3, precision=prec)
-        U0 = F.field_init(m, p, F.INIT_ISENTROPIC_VORTEX, (5.0, 5.0, 5.0))
-        g = F.GPUSolver2(m, p)
-        g.set_state(U0)
-        t, _ = g.advance(100, 0.0, 1e9)
-        outs[prec] = (t, g.get_state())
-        g.close()
-    t, U = outs[F.PREC_FP64]
-    res["t_end"] = t
-    res["fp64_vs_analytic"] = var_errors(U, vortex_analytic(m, t))
-    res["mixed_vs_fp64"] = var_errors(outs[F.PREC_MIXED][1], U)
+    for steps in (100, 1000):
+        outs = {}
+        for prec in (F.PREC_FP64, F.PREC_MIXED):
+            p = F.default_params(inf_ruvp=[1, 1, 1, 1], CFL=0.5, gradient=F.GRA_LEASTSQUARE, limiter=F.LIM_BARTH,
+                                 flux_scheme=F.SOL_ROE, time_advance=F.EVO_RK3, precision=prec)
+            U0 = F.field_init(m, p, F.INIT_ISENTROPIC_VORTEX, (5.0, 5.0, 5.0))
+            g = F.GPUSolver2(m, p)
+            g.set_state(U0)
+            t, _ = g.advance(steps, 0.0, 1e9)
+            outs[prec] = (t, g.get_state())
+            g.close()
+        t, U = outs[F.PREC_FP64]
+        if steps == 100:
+            res["t_end"] = t
+            res["fp64_vs_analytic"] = var_errors(U, vortex_analytic(m, t))
+            res["mixed_vs_fp64"] = var_errors(outs[F.PREC_MIXED][1], U)
+        else:
+            res["steps_1000"] = {"t_end": t, "mixed_vs_fp64": var_errors(outs[F.PREC_MIXED][1], U)}
     return res
 
 
@@ -284,6 +336,16 @@ def config_c4(ncases=256, steps=500, seed=0):
         params.append(p)
         states.append(F.field_init(m, p, F.INIT_UNIFORM))
     total = sum(m.num_element for m in meshes)
+    # the library's tuner on every 8th case (about 1/8 of the cells; a batch handle per
+    # candidate), then the whole batch with the tile size it picks
+    t0 = time.perf_counter()
+    tune_cases = list(range(0, ncases, 8))
+    best_tc, best_ms, timings = F.autotune_batch([meshes[c] for c in tune_cases], [params[c] for c in tune_cases],
+                                                 steps=10)
+    tune_s = time.perf_counter() - t0
+    default_tc = 300   # the FP32-mixed shape's default tile size
+    for q in params:
+        q.tile_cells = best_tc
     t0 = time.perf_counter()
     b = F.BatchSolver(meshes, params)
     setup = time.perf_counter() - t0
@@ -292,9 +354,25 @@ def config_c4(ncases=256, steps=500, seed=0):
     t = b.advance(2, np.zeros(ncases), np.full(ncases, 1e9))
     b.synchronize()
     t0 = time.perf_counter()
-    b.advance(steps, t, np.full(ncases, 1e9))
+    t = b.advance(steps, t, np.full(ncases, 1e9))
     b.synchronize()
     wall = time.perf_counter() - t0
+    # FP32-mixed against FP64 on a sample of cases spread over the sizes: the batch's state after
+    # its 2 + steps steps against FP64 on a handle of the case's own (a batch case equals its
+    # separate handle bit for bit)
+    sample = list(range(0, ncases, max(1, ncases // 8)))[:8]
+    errs = []
+    for c in sample:
+        Umx = b.get_state(c)
+        q = F.Params.from_buffer_copy(params[c])
+        q.precision = F.PREC_FP64
+        g = F.GPUSolver2(meshes[c], q)
+        g.set_state(states[c])
+        g.advance(2 + steps, 0.0, 1e9)
+        U64 = g.get_state()
+        g.close()
+        e = var_errors(Umx, U64)
+        errs.append({"case": c, "cells": meshes[c].num_element, **{k: v for k, v in e.items()}})
     b.profile(True)
     b.profile_reset()
     b.advance(5, t, np.full(ncases, 1e9))
@@ -322,6 +400,13 @@ def config_c4(ncases=256, steps=500, seed=0):
     return {"cases": ncases, "cells": total, "steps": steps, "dtype": "f32-mixed",
             "value": total * 3 * steps / wall, "unit": "cell-RK-stage updates/s", "ms_per_step": wall / steps * 1e3,
             "setup_s": setup, "kernels_ms": kern,
+            "autotune": {"cases": len(tune_cases), "cells": sum(meshes[c].num_element for c in tune_cases),
+                         "steps": 10, "tile_cells": best_tc, "ms_per_step": best_ms,
+                         "ms_per_step_by_tile_cells": timings, "default_tile_cells": default_tc,
+                         "gain_vs_default": (timings.get(default_tc, best_ms) / best_ms) if best_ms else None,
+                         "seconds": tune_s},
+            "mixed_vs_fp64_sample": {"steps": 2 + steps, "cases": errs,
+                                     "max_rel_l2": {k: max(e[k]["l2"] for e in errs) for k in ("rho", "rhou", "rhov", "rhoE")}},
             "separate_handles": {"cases": sub, "cells": sub_cells, "steps": sub_steps,
                                  "value": sub_cells * 3 * sub_steps / wall_sep}}
 
@@ -389,19 +474,39 @@ def config_c5(primal_iters=260000, adjoint_iters=150000, fd_iters=140000, h=1e-6
             "fd_solves": {"iterations_each": fd_iters, "seconds": t_fd}}
 
 
-def cpu_baseline(steps):
-    """The CPU oracle (single thread, FP64) on the same mesh for a bounded number of steps."""
+def cpu_baseline(threads, runs=5, steps_one=3, steps_all=10):
+    """The CPU oracle (FP64, the restatement of the reference's CPU path) built for this host with
+    -O3 -march=native -fopenmp (no contraction, no fast-math; its loops own their writes, so every
+    thread count gives the plain build's bits) on the same mesh and initial field: after one
+    warm-up step, `runs` windows of `steps_one` SSP-RK3 steps on one thread (the reference's CPU
+    path is single-threaded: its OpenMP pragmas are commented out) and of `steps_all` steps on
+    `threads` threads; the median window, in cell-RK-stage updates/s."""
     import cfd_solver_2d_gpu_amd as F
     import oracle as O
     m, p, U = make_case(F.SOL_ROE, F.PREC_FP64)
-    o = O.OracleSolver(m, p)
-    o.set_state(U)
-    t = 0.0
     t0 = time.perf_counter()
-    for _ in range(steps):
-        t, _ = o.step(t, 1e9)
-    wall = time.perf_counter() - t0
-    return m.num_element * 3 * steps / wall, wall
+    O.fast_lib()
+    build_s = time.perf_counter() - t0
+    res = {}
+    total = 0.0
+    for name, nth, steps in (("one_thread", 1, steps_one), ("all_threads", threads, steps_all)):
+        used = O.fast_lib(nth).orc_set_threads(0)
+        o = O.OracleSolver(m, p, fast=True)
+        o.set_state(U)
+        t, _ = o.step(0.0, 1e9)
+        walls = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                t, _ = o.step(t, 1e9)
+            walls.append(time.perf_counter() - t0)
+        total += sum(walls)
+        med = float(np.median(walls))
+        res[name] = {"value": m.num_element * 3 * steps / med, "threads": used, "steps_per_window": steps,
+                     "windows": runs, "median_s": med, "spread_pct": round(100 * (max(walls) - min(walls)) / med, 2)}
+    res["build_s"] = build_s
+    res["cpu_seconds"] = total
+    return res
 
 
 def main():
@@ -409,7 +514,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--windows", type=int, default=7,
+                    help="timed windows of --steps steps each (at least 5); value is their median")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="FP64 Roe only")
     ap.add_argument("--no-c5", action="store_true", help="skip the adjoint leg (about two minutes)")
@@ -441,11 +547,13 @@ def main():
         del src, dst
 
     cell = args.kernels == "cell"
-    m, wall, U64, kern = run_gpu(F.SOL_ROE, F.PREC_FP64, args.steps, args.warmup, profile=True,
-                                 kernels=F.KERNELS_CELL if cell else None)
+    reps = max(5, args.windows)
+    m, walls, U64, kern = run_gpu(F.SOL_ROE, F.PREC_FP64, args.steps, args.warmup, profile=True,
+                                  kernels=F.KERNELS_CELL if cell else None, reps=reps)
     nc, nf, nv = m.num_element, m.num_edge, m.num_node
-    value = nc * 3 * args.steps / wall
-    ms_step = wall / args.steps * 1e3
+    win = window_stats(walls, args.steps)
+    ms_step = win["median_ms_per_step"]
+    value = nc * 3 / (ms_step * 1e-3)
 
     st_ms, st_n = kern["k_stage" if cell else "k_tile"]
     b_launch = stage_bytes(nc, nf, nv, 8, 8, 8)
@@ -463,31 +571,43 @@ def main():
         "value": value, "unit": "cell-RK-stage updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "clock_warmup_steps": CLOCK_WARMUP_STEPS,
         "config": {"workload": "wedge M=2, 1024x512x2 = 1,048,576 triangles (jitter 10%, seed 2024), "
                                "Roe, Green-Gauss + Barth, linear reconstruction, SSP-RK3, CFL 0.5, FP64",
                    "cells": nc, "edges": nf, "nodes": nv},
         "roofline": roofline,
+        # value and ms_per_step: the median of the timed windows of `steps` steps
+        "windows": win,
         "path_roofline": {"bytes_per_step": path_b, "achieved_GBs": round(path_b / (ms_step * 1e-3) / 1e9, 1),
                           "frac": round(path_b / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels_ms": {k: round(v[0], 5) for k, v in kern.items() if not k.startswith("_")},
+        "launches_per_step": round(sum(v[1] for k, v in kern.items() if not k.startswith("_")) / args.steps, 2),
         # host arrays across the boundary (PCIe, permutation to and from the device order):
         # the rate when the timed steps start from a host state and end in one
         "host_io": {"set_state_ms": round(kern["_h2d_s"] * 1e3, 3), "get_state_ms": round(kern["_d2h_s"] * 1e3, 3),
-                    "pcie_inclusive_value": nc * 3 * args.steps / (wall + kern["_h2d_s"] + kern["_d2h_s"])},
+                    "pcie_inclusive_value": nc * 3 * args.steps / (ms_step * 1e-3 * args.steps + kern["_h2d_s"]
+                                                                   + kern["_d2h_s"])},
     }
     if not args.no_extra:
-        _, wall_mx, Umx, kmx = run_gpu(F.SOL_ROE, F.PREC_MIXED, args.steps, args.warmup, profile=True)
-        err = float(np.linalg.norm(Umx[0] - U64[0]) / np.linalg.norm(U64[0]))
+        _, walls_mx, Umx, kmx = run_gpu(F.SOL_ROE, F.PREC_MIXED, args.steps, args.warmup, profile=True, reps=reps)
+        win_mx = window_stats(walls_mx, args.steps)
         kmx = {k: v for k, v in kmx.items() if not k.startswith("_")}
-        out["mixed"] = {"value": nc * 3 * args.steps / wall_mx, "speedup_vs_fp64": wall / wall_mx,
-                        "rel_l2_density_vs_fp64": err, "steps": args.steps,
-                        "errors_vs_fp64": var_errors(Umx, U64),
-                        "kernels_ms": {k: round(v[0], 5) for k, v in kmx.items()}}
-        _, wall_llf, _, _ = run_gpu(F.SOL_LLF, F.PREC_FP64, args.steps, args.warmup, profile=False)
-        out["llf_fp64_value"] = nc * 3 * args.steps / wall_llf
-        _, wall_cell, Ucell, kcell = run_gpu(F.SOL_ROE, F.PREC_FP64, args.steps, args.warmup, profile=True,
-                                             kernels=F.KERNELS_CELL)
-        out["cell_kernels"] = {"value": nc * 3 * args.steps / wall_cell,
+        acc = accuracy_run(F.SOL_ROE, (F.PREC_FP64, F.PREC_MIXED), 1000)
+        (ms64, A64), (msmx, Amx) = acc[F.PREC_FP64], acc[F.PREC_MIXED]
+        out["mixed"] = {"value": nc * 3 / (win_mx["median_ms_per_step"] * 1e-3),
+                        "speedup_vs_fp64": ms_step / win_mx["median_ms_per_step"],
+                        "windows": win_mx,
+                        "kernels_ms": {k: round(v[0], 5) for k, v in kmx.items()},
+                        "steps_1000": {"steps": 1000, "fp64_ms_per_step": ms64, "mixed_ms_per_step": msmx,
+                                       "speedup_vs_fp64": ms64 / msmx,
+                                       "rel_l2_density_vs_fp64": float(np.linalg.norm(Amx[0] - A64[0])
+                                                                       / np.linalg.norm(A64[0])),
+                                       "errors_vs_fp64": var_errors(Amx, A64)}}
+        _, walls_llf, _, _ = run_gpu(F.SOL_LLF, F.PREC_FP64, args.steps, args.warmup, profile=False, reps=reps)
+        out["llf_fp64_value"] = nc * 3 / (window_stats(walls_llf, args.steps)["median_ms_per_step"] * 1e-3)
+        _, walls_cell, Ucell, kcell = run_gpu(F.SOL_ROE, F.PREC_FP64, args.steps, args.warmup, profile=True,
+                                              kernels=F.KERNELS_CELL, reps=reps)
+        out["cell_kernels"] = {"value": nc * 3 / (window_stats(walls_cell, args.steps)["median_ms_per_step"] * 1e-3),
                                "rel_l2_density_vs_tile": float(np.linalg.norm(Ucell[0] - U64[0]) / np.linalg.norm(U64[0])),
                                "kernels_ms": {k: round(v[0], 5) for k, v in kcell.items() if not k.startswith("_")}}
     if not args.no_extra:
@@ -499,10 +619,19 @@ def main():
     if not args.no_extra and not args.no_c5:
         out["c5_adjoint"] = config_c5(*args.c5_iters)
     if not args.no_cpu:
-        rate, cw = cpu_baseline(args.cpu_steps)
-        out["cpu_baseline"] = {"value": rate, "unit": "cell-RK-stage updates/s", "cores": 1, "kind": "port",
-                               "sample": f"CPU oracle (FP64, 1 thread, g++ -O2) on the same 1,048,576-cell wedge, "
-                                         f"{args.cpu_steps} SSP-RK3 steps ({cw:.1f} s)"}
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+        cb = cpu_baseline(threads)
+        allc, one = cb["all_threads"], cb["one_thread"]
+        out["cpu_baseline"] = {
+            "value": allc["value"], "unit": "cell-RK-stage updates/s", "cores": allc["threads"], "kind": "port",
+            "sample": f"CPU oracle (FP64 restatement of the reference CPU path, g++ -O3 -march=native -fopenmp, "
+                      f"no contraction) on the same 1,048,576-cell wedge: median of {allc['windows']} windows of "
+                      f"{allc['steps_per_window']} SSP-RK3 steps on {allc['threads']} threads "
+                      f"({cb['cpu_seconds']:.1f} s of timed CPU work in all)",
+            "nproc": os.cpu_count(),
+            "one_thread": {"value": one["value"], "cores": 1, "windows": one["windows"],
+                           "steps_per_window": one["steps_per_window"], "spread_pct": one["spread_pct"]},
+            "all_threads_spread_pct": allc["spread_pct"], "build_s": round(cb["build_s"], 2)}
     print(json.dumps(out), flush=True)
 
 

@@ -18,7 +18,7 @@ from . import _abi
 from ._abi import *  # noqa: F401,F403  (codes)
 from ._abi import AdjointResult, FvmrError, GenParams, MeshSoA, Params, StepInfo, check, lib
 
-__all__ = ["Mesh", "GPUSolver2", "BatchSolver", "autotune", "field_init", "default_params", "ruvp_from_mach", "FvmrError",
+__all__ = ["Mesh", "GPUSolver2", "BatchSolver", "autotune", "autotune_batch", "autotune_candidates", "field_init", "default_params", "ruvp_from_mach", "FvmrError",
            "Params", "StepInfo", "MeshSoA", "lib"]
 
 
@@ -83,7 +83,7 @@ class Mesh:
 
     def write_continue(self, path, U, t, istep, CFL):
         """FieldWriter::writeContinueFile_1 layout with 17 significant digits."""
-        U, rows = _rows(U)
+        U, rows = _rows(U, self.num_element)
         check(lib().fvmr_write_continue(self._h, str(path).encode(), float(t), int(istep), float(CFL), rows),
               "fvmr_write_continue")
 
@@ -191,8 +191,12 @@ def soa_arrays(soa):
     }
 
 
-def _rows(U):
+def _rows(U, n=None):
+    """U as a contiguous float64 array and its four row pointers; with n, U must be 4 x n (the
+    library copies n doubles from each row)."""
     U = np.ascontiguousarray(U, dtype=np.float64)
+    if n is not None and U.shape != (4, n):
+        raise ValueError(f"state must be 4 x {n} (rho, rho u, rho v, rho E per cell), got shape {U.shape}")
     return U, (C.POINTER(C.c_double) * 4)(*[_dptr(U[k]) for k in range(4)])
 
 
@@ -202,6 +206,41 @@ def autotune(mesh, params, steps=20):
     ms = C.c_double()
     check(lib().fvmr_autotune(C.byref(mesh.soa), C.byref(params), steps, C.byref(best), C.byref(ms)), "fvmr_autotune")
     return best.value, ms.value
+
+
+def autotune_candidates(precision=PREC_FP64):
+    """The tile sizes the tuners try for a precision."""
+    n = C.c_int()
+    out = np.zeros(8, dtype=np.int32)
+    check(lib().fvmr_autotune_candidates(precision, C.byref(n), _iptr(out)), "fvmr_autotune_candidates")
+    return [int(x) for x in out[:n.value]]
+
+
+def autotune_batch(meshes, params, steps=10):
+    """fvmr_autotune_batch: (best tile_cells, its ms per step, {tile_cells: ms per step}) for a
+    batch of these cases."""
+    n = len(meshes)
+    soas = (C.POINTER(MeshSoA) * n)(*[C.pointer(m.soa) for m in meshes])
+    ps = (Params * n)(*params)
+    best = C.c_int()
+    ms = C.c_double()
+    cand = autotune_candidates(params[0].precision)
+    times = np.zeros(8)
+    check(lib().fvmr_autotune_batch(n, soas, ps, steps, C.byref(best), C.byref(ms), _dptr(times)),
+          "fvmr_autotune_batch")
+    return best.value, ms.value, {c: float(t) for c, t in zip(cand, times)}
+
+
+def device_division(a, b):
+    """(a / b, 1 / b) as the device kernels form them in FP64 (fvmr_device_division)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    if a.shape != b.shape or a.ndim != 1:
+        raise ValueError("a and b must be 1-D arrays of one length")
+    q = np.zeros_like(a)
+    r = np.zeros_like(a)
+    check(lib().fvmr_device_division(len(a), _dptr(a), _dptr(b), _dptr(q), _dptr(r)), "fvmr_device_division")
+    return q, r
 
 
 def field_init(mesh, params, init_type=INIT_UNIFORM, args=()):
@@ -244,7 +283,7 @@ class GPUSolver2:
         self.close()
 
     def set_state(self, U):
-        U, rows = _rows(U)
+        U, rows = _rows(U, self.n)
         check(lib().fvmr_set_state(self._h, rows), "fvmr_set_state")
 
     def get_state(self):
@@ -382,7 +421,7 @@ class GPUSolver2:
     def adjoint_vjp(self, psi, alpha=False, coords=False):
         """(dr/dU)^T psi at the current state, plus psi . dr/dalpha and psi . dr/dX (2 x num_node)
         when asked."""
-        P, prow = _rows(np.array(psi, dtype=np.float64, copy=True))
+        P, prow = _rows(np.array(psi, dtype=np.float64, copy=True), self.n)
         out, orow = _rows(np.zeros((4, self.n)))
         da = C.c_double()
         dxy = np.zeros(2 * self.mesh.soa.num_node)
@@ -460,7 +499,9 @@ class BatchSolver:
         return b
 
     def set_state(self, c, U):
-        U, rows = _rows(U)
+        if not 0 <= c < self.n:
+            raise ValueError(f"case {c} out of range [0, {self.n})")
+        U, rows = _rows(U, self.meshes[c].num_element)
         check(lib().fvmr_batch_set_state(self._h, c, rows), "fvmr_batch_set_state")
 
     def get_state(self, c):
@@ -478,6 +519,12 @@ class BatchSolver:
 
     def synchronize(self):
         check(lib().fvmr_synchronize(self._h), "fvmr_synchronize")
+
+    def step_info(self, c):
+        """Case c after the last advance: StepInfo with t, dt and n_restricted of its last step."""
+        info = StepInfo()
+        check(lib().fvmr_batch_step_info(self._h, int(c), C.byref(info)), "fvmr_batch_step_info")
+        return info
 
     def use_graphs(self, on=True):
         check(lib().fvmr_use_graphs(self._h, 1 if on else 0), "fvmr_use_graphs")

@@ -112,9 +112,13 @@ SIGNATURES = {
     "fvmr_global_dt": (C.c_int, [C.c_void_p, _dp]),
     "fvmr_residual": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "fvmr_permutation": (C.c_int, [C.c_void_p, _ip]),
+    "fvmr_device_division": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp]),
     "fvmr_node_field": (C.c_int, [C.c_void_p, _dp * 4, _dp, _dp]),
     "fvmr_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "fvmr_autotune": (C.c_int, [C.POINTER(MeshSoA), C.POINTER(Params), C.c_int, _ip, _dp]),
+    "fvmr_autotune_batch": (C.c_int, [C.c_int, C.POINTER(C.POINTER(MeshSoA)), C.POINTER(Params), C.c_int, _ip, _dp,
+                                      _dp]),
+    "fvmr_autotune_candidates": (C.c_int, [C.c_int, _ip, _ip]),
     "fvmr_set_boundary_states": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "fvmr_vertex_colors": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_int, C.POINTER(C.c_float),
                                      C.POINTER(C.c_float), C.c_int]),
@@ -128,6 +132,7 @@ SIGNATURES = {
     "fvmr_batch_set_state": (C.c_int, [C.c_void_p, C.c_int, _dp * 4]),
     "fvmr_batch_get_state": (C.c_int, [C.c_void_p, C.c_int, _dp * 4]),
     "fvmr_batch_advance": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
+    "fvmr_batch_step_info": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(StepInfo)]),
     "fvmr_surface_force": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, C.c_double, _dp, _dp]),
     "fvmr_surface_force_noalloc": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, C.c_double, _dp, _dp]),
     "fvmr_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),

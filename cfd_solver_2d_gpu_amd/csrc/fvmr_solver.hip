@@ -130,8 +130,8 @@ struct Scalars {
     double t, dt, T, t_rk;
     unsigned int counter;
     unsigned int spare;
-    int bad_count;              // cells flagged by the stage kernel
-    int n_restricted;           // cells repaired in the current step
+    int bad_count;              // sc[0]: length of the handle's list of flagged cells
+    int n_restricted;           // cells of this case restricted in the current step
 };
 
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous
@@ -236,6 +236,12 @@ struct GlobalCells {
     const S4 *__restrict__ U;
     const Grad<G> *__restrict__ grad;
     __device__ __forceinline__ void state(int c, double u[4]) const { ld(U, c, u); }
+    template <class C> __device__ __forceinline__ void state_c(int c, C u[4]) const {
+        double d[4];
+        ld(U, c, d);
+#pragma unroll
+        for (int k = 0; k < 4; k++) u[k] = (C)d[k];
+    }
     __device__ __forceinline__ V2<G> centroid(int c) const {
         const V4<G> g = m.cgeo[c];
         return V2<G>{g.x, g.y};
@@ -268,6 +274,10 @@ struct PairCells {
 #pragma unroll
         for (int k = 0; k < 4; k++) u[k] = (c == L) ? uL[k] : uR[k];
     }
+    template <class C> __device__ __forceinline__ void state_c(int c, C u[4]) const {
+#pragma unroll
+        for (int k = 0; k < 4; k++) u[k] = (C)((c == L) ? uL[k] : uR[k]);
+    }
     __device__ __forceinline__ V2<G> centroid(int c) const { return (c == L) ? cL : cR; }
     __device__ __forceinline__ Grad<G> gradient(int c) const { return (c == L) ? gL : gR; }
     __device__ __forceinline__ bool has(int c) const { return !(c < 0 || c >= nc); }
@@ -278,11 +288,8 @@ struct PairCells {
 template <class C, class G, class Cells>
 __device__ __forceinline__ void reconstruct(const Cells &cells, const Phys &ph, int c, C x, C y, FaceState<C> &s) {
     const C gamma = (C)ph.gamma;
-    double Ud[4];
-    cells.state(c, Ud);
     C Uc[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) Uc[i] = (C)Ud[i];
+    cells.state_c(c, Uc);
     if (ph.reconstruct == FVMR_REC_LINEAR) {
         const V2<G> g = cells.centroid(c);
         const Grad<G> gr = cells.gradient(c);
@@ -447,12 +454,12 @@ __device__ __forceinline__ void flux_viscous(const Cells &cells, const Phys &ph,
     C gx[4], gy[4], Um[4];
     {
         const Grad<G> gL = cells.gradient(L);
-        double uL[4];
-        cells.state(L, uL);
+        C uL[4];
+        cells.state_c(L, uL);
         if (hasR) {
             const Grad<G> gR = cells.gradient(R);
-            double uR[4];
-            cells.state(R, uR);
+            C uR[4];
+            cells.state_c(R, uR);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 gx[k] = C(0.5) * ((C)gL.x[k] + (C)gR.x[k]);
@@ -1017,7 +1024,10 @@ __global__ __launch_bounds__(kGroup3) FVMR_STAGE_ATTR void k_stage(Mesh<G> m, Ph
     for (int k = 0; k < 4; k++) bad = bad || isnan(rv[k]);
     if (bad) {
         const int slot = atomicAdd(&sc->bad_count, 1);
-        if (slot < nc) bad_list[slot] = c;
+        if (slot < nc) {
+            bad_list[slot] = c;
+            bad_list[nc + slot] = 0;   // the case of the cell (the cell family runs one case)
+        }
     }
 }
 
@@ -1123,12 +1133,24 @@ __device__ __forceinline__ int tile_btype(int4 fr) {
     return bc ? bc + fvmr::kBcBase : -1;
 }
 
-template <class G>
+// The staged state record of a tile kernel computing in C: FP64 in the plain path; in mixed
+// precision the FP32 values the arithmetic uses (converted once when staged; the RK update
+// takes each tile cell's FP64 state from the staging registers).
+template <class C> using TileState = typename std::conditional<std::is_same<C, float>::value, V4<float>, S4>::type;
+
+template <class G, class SU = S4>
 struct TileCells {
-    const S4 *sU;
+    const SU *sU;
     const V2<G> *sC;
     const Grad<G> *sGr;
-    __device__ __forceinline__ void state(int c, double u[4]) const { ld(sU, c, u); }
+    __device__ __forceinline__ void state(int c, double u[4]) const {
+        const SU v = sU[c];
+        u[0] = v.x; u[1] = v.y; u[2] = v.z; u[3] = v.w;
+    }
+    template <class C> __device__ __forceinline__ void state_c(int c, C u[4]) const {
+        const SU v = sU[c];
+        u[0] = (C)v.x; u[1] = (C)v.y; u[2] = (C)v.z; u[3] = (C)v.w;
+    }
     __device__ __forceinline__ V2<G> centroid(int c) const { return sC[c]; }
     __device__ __forceinline__ Grad<G> gradient(int c) const { return sGr[c]; }
     __device__ __forceinline__ bool has(int c) const { return c >= 0; }
@@ -1160,7 +1182,8 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
     constexpr int kTileThreads = Cfg::kThreads, kTileMaxG = Cfg::kMaxG, kTileMaxL = Cfg::kMaxL;
     constexpr int kTileMaxN = Cfg::kMaxN, kTileMaxF = Cfg::kMaxF, kTileMaxFF = Cfg::kMaxFF;
     static_assert(kTileMaxL <= kTileThreads && kTileMaxFF <= kTileThreads, "one local cell and one face per thread");
-    __shared__ S4 sU[kTileMaxL];
+    using SU = TileState<C>;
+    __shared__ SU sU[kTileMaxL];
     __shared__ V2<double> sN[kTileMaxN];
     __shared__ V2<G> sC[kTileMaxL];
     __shared__ int4 sFace[kTileMaxF];
@@ -1196,7 +1219,7 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
     if (tid < nN) sN[tid] = nd;
     if (tid < nF) sFace[tid] = fr0;
     if (tid + kTileThreads < nF) sFace[tid + kTileThreads] = fr1;
-    if (i < nL) sU[i] = uin;
+    if (i < nL) sU[i] = SU{(C)uin.x, (C)uin.y, (C)uin.z, (C)uin.w};
     __syncthreads();
     // 2. centroids and face normals
     if (i < nL) {
@@ -1214,11 +1237,11 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
     if (i < nG) {
         const int sw[3] = {gs.x, gs.y, gs.z};
         const C vol = (C)gvol;
-        double Ud[4];
-        ld(sU, i, Ud);
         C Uc[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) Uc[k] = (C)Ud[k];
+        {
+            const SU v = sU[i];
+            Uc[0] = (C)v.x; Uc[1] = (C)v.y; Uc[2] = (C)v.z; Uc[3] = (C)v.w;
+        }
         const C xc = (C)sC[i].x, yc = (C)sC[i].y;
         struct {
             C a[3][3], u[3][4], n[3][2];
@@ -1238,10 +1261,10 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
             V4<G> eg;
             V2<G> em;
             tile_face_geom_n<G>(sN, sNrm, f, fr, eg, em);
-            double Uo[4];
-            ld(sU, has_out ? out : i, Uo);
-#pragma unroll
-            for (int k = 0; k < 4; k++) sl.u[j][k] = (C)Uo[k];
+            {
+                const SU v = sU[has_out ? out : i];
+                sl.u[j][0] = (C)v.x; sl.u[j][1] = (C)v.y; sl.u[j][2] = (C)v.z; sl.u[j][3] = (C)v.w;
+            }
             const V2<double> nd = sN[lfield(nw, j)];
             sl.n[j][0] = (C)(G)nd.x - xc;
             sl.n[j][1] = (C)(G)nd.y - yc;
@@ -1277,7 +1300,7 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
     // 4. one canonical flux per face of the tile (nFF <= threads), written over the gradients
     const double t_rk = sc[cs].t_rk;
     const CaseBC &bc = ta.cbc[cs];
-    const TileCells<G> cells{sU, sC, sGr};
+    const TileCells<G, SU> cells{sU, sC, sGr};
     C F[4] = {C(0), C(0), C(0), C(0)};
     const int f = tid;
     if (f < nFF) {
@@ -1311,8 +1334,16 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
             acc[3] += s * (double)fv.w;
         }
         const double w = -1.0 * (double)gvol;
-        double Ui[4], Ub[4] = {0, 0, 0, 0}, r[4], Unew[4];
-        ld(sU, i, Ui);
+        // the cell's own FP64 state: from LDS in the plain path, from this thread's staging
+        // registers in mixed precision (its LDS copy is FP32)
+        double Ui[4];
+        if (std::is_same<SU, S4>::value) {
+            const SU v = sU[i];
+            Ui[0] = v.x; Ui[1] = v.y; Ui[2] = v.z; Ui[3] = v.w;
+        } else {
+            Ui[0] = uin.x; Ui[1] = uin.y; Ui[2] = uin.z; Ui[3] = uin.w;
+        }
+        double Ub[4] = {0, 0, 0, 0}, r[4], Unew[4];
         if (!first) { Ub[0] = un.x; Ub[1] = un.y; Ub[2] = un.z; Ub[3] = un.w; }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -1329,8 +1360,11 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
 #pragma unroll
         for (int k = 0; k < 4; k++) bad = bad || isnan(rv[k]);
         if (bad) {
-            const int slot = atomicAdd(&sc->bad_count, 1);
-            if (slot < nc) bad_list[slot] = c;
+            const int slot = atomicAdd(&sc->bad_count, 1);   // the handle's list: sc[0]
+            if (slot < nc) {
+                bad_list[slot] = c;
+                bad_list[nc + slot] = cs;
+            }
         }
     }
 }
@@ -1696,10 +1730,12 @@ __global__ __launch_bounds__(1024) void k_restrict(Mesh<G> m, S4 *__restrict__ U
         st(fix, i, res);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < count; i += blockDim.x) U[bad_list[i]] = fix[i];
+    for (int i = threadIdx.x; i < count; i += blockDim.x) {
+        U[bad_list[i]] = fix[i];
+        atomicAdd(&sc[bad_list[nc + i]].n_restricted, 1);   // counted per case of a batch
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        sc->n_restricted += count;
         sc->bad_count = 0;
     }
 }
@@ -1766,7 +1802,7 @@ __global__ __launch_bounds__(1024) void k_dt_finish_cases(const int *__restrict_
             if (t + d > sc[c].T) d = sc[c].T - t;
             sc[c].t_rk = t;
             sc[c].t = t + d;
-            if (c == 0) sc[0].n_restricted = 0;
+            sc[c].n_restricted = 0;
         }
         sc[c].dt = d;
     }
@@ -1816,6 +1852,16 @@ __global__ __launch_bounds__(kBlock) void k_norm_finish(int nparts, int norm, co
         if (threadIdx.x == 0) out[k] = (norm == FVMR_NORM_2) ? sqrt(s[0]) : s[0];
         __syncthreads();
     }
+}
+
+// The FP64 quotient and reciprocal of the face, gradient and update arithmetic, on n operand
+// pairs (a parity hook for those forms).
+__global__ __launch_bounds__(kBlock) void k_division(int n, const double *__restrict__ a, const double *__restrict__ b,
+                                                     double *__restrict__ q, double *__restrict__ r) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    q[i] = fdiv(a[i], b[i]);
+    r[i] = frcp(b[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1961,6 +2007,7 @@ struct fvmr_solver {
     int *tile_case_begin = nullptr;       // tiles of case c: [c] .. [c+1]-1
     int *perm_dev = nullptr;              // perm on the device (state transfers, first use)
     double *io_soa = nullptr;             // 4 x nc doubles in the caller's order (state transfers)
+    bool io_ready = false;                // perm_dev holds perm (set by ensure_io)
     // tile kernels (fvmr_tiles.h)
     bool tiles = false;
     int ntiles = 0;
@@ -2570,7 +2617,7 @@ static int create_impl(const fvmr_mesh_soa *m, const fvmr_params *p, int ncases,
     if (hipMalloc(&h->grad, (size_t)nc * gsz) != hipSuccess ||
         hipMalloc((void **)&h->rhs, (size_t)nc * sizeof(S4)) != hipSuccess ||
         hipMalloc((void **)&h->fix, (size_t)nc * sizeof(S4)) != hipSuccess ||
-        hipMalloc((void **)&h->bad_list, (size_t)nc * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&h->bad_list, 2 * (size_t)nc * sizeof(int)) != hipSuccess ||
         hipMalloc((void **)&h->partial, (size_t)nparts * sizeof(double)) != hipSuccess ||
         hipMalloc((void **)&h->norm_out, 4 * sizeof(double)) != hipSuccess ||
         hipMalloc((void **)&h->sc, ncases * sizeof(Scalars)) != hipSuccess)
@@ -2690,16 +2737,26 @@ int fvmr_destroy(fvmr_handle *h) {
 // device order as a gather (scatter) on the device.  perm is a permutation of 0 .. nc-1 (built
 // by fvmr_create), so every index stays inside the nc-long arrays.
 static int ensure_io(fvmr_handle *h) {
-    if (h->io_soa) return FVMR_OK;
+    if (h->io_ready) return FVMR_OK;
     const size_t nc = (size_t)h->nc;
-    if (hipMalloc((void **)&h->perm_dev, nc * sizeof(int)) != hipSuccess ||
-        hipMalloc((void **)&h->io_soa, 4 * nc * sizeof(double)) != hipSuccess) {
+    auto release = [&]() {
         if (h->perm_dev) (void)hipFree(h->perm_dev);
+        if (h->io_soa) (void)hipFree(h->io_soa);
         h->perm_dev = nullptr;
         h->io_soa = nullptr;
+    };
+    if (hipMalloc((void **)&h->perm_dev, nc * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&h->io_soa, 4 * nc * sizeof(double)) != hipSuccess) {
+        release();
         return set_error(FVMR_ERR_HIP, "hipMalloc state transfer buffers");
     }
-    HIP_OK(hipMemcpyAsync(h->perm_dev, h->perm.data(), nc * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    // ready only once the permutation is on the device: the transfer kernels index with it
+    const hipError_t e = hipMemcpy(h->perm_dev, h->perm.data(), nc * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        release();
+        return set_error(FVMR_ERR_HIP, std::string("permutation upload: ") + hipGetErrorString(e));
+    }
+    h->io_ready = true;
     return FVMR_OK;
 }
 
@@ -2824,6 +2881,36 @@ int fvmr_batch_advance(fvmr_handle *h, int nsteps, double *t, const double *T) {
     HIP_OK(hipStreamSynchronize(h->stream));
     if (h->profiling && (rc = prof_collect(h))) return rc;
     for (int c = 0; c < h->ncases; c++) t[c] = s[c].t;
+    return FVMR_OK;
+}
+
+int fvmr_batch_step_info(fvmr_handle *h, int c, fvmr_step_info *info) {
+    if (!h || !info || c < 0 || c >= h->ncases) return set_error(FVMR_ERR_INVALID_ARG, "invalid case or argument");
+    Scalars s{};
+    HIP_OK(hipMemcpyAsync(&s, h->sc + c, sizeof(Scalars), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    info->t = s.t;
+    info->dt = s.dt;
+    info->n_restricted = s.n_restricted;
+    return FVMR_OK;
+}
+
+int fvmr_device_division(int n, const double *a, const double *b, double *quotient, double *reciprocal) {
+    if (n < 0 || (n > 0 && (!a || !b || !quotient || !reciprocal))) return set_error(FVMR_ERR_INVALID_ARG, "invalid argument");
+    if (n == 0) return FVMR_OK;
+    const size_t bytes = (size_t)n * sizeof(double);
+    double *d = nullptr;
+    if (hipMalloc((void **)&d, 4 * bytes) != hipSuccess) return set_error(FVMR_ERR_HIP, "hipMalloc division operands");
+    hipError_t e = hipMemcpy(d, a, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + n, b, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        k_division<<<grid_of(n), kBlock>>>(n, d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(quotient, d + 2 * (size_t)n, bytes, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(reciprocal, d + 3 * (size_t)n, bytes, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return set_error(FVMR_ERR_HIP, std::string("fvmr_device_division: ") + hipGetErrorString(e));
     return FVMR_OK;
 }
 
@@ -3065,39 +3152,57 @@ int fvmr_surface_force_noalloc(fvmr_handle *h, int nsets, const int *set_offsets
     return FVMR_OK;
 }
 
-// Times `steps` SSP-RK3 / Euler steps of the mesh's uniform freestream for each candidate tile
-// size (both tile shapes' ranges, clipped to the LDS capacity) and returns the fastest.
-int fvmr_autotune(const fvmr_mesh_soa *m, const fvmr_params *p, int steps, int *best_tile_cells, double *best_ms) {
-    if (!m || !p || !best_tile_cells || steps < 1) return set_error(FVMR_ERR_INVALID_ARG, "invalid argument");
-    const bool mixed = p->precision == FVMR_PREC_MIXED;
+}  // extern "C"
+
+// Tile-size candidates of a precision: 0.6 - 1.15 x the shape's default, within its LDS capacity.
+static std::vector<int> tile_candidates(bool mixed) {
     const int dflt = mixed ? TileCfg<float>::kCells : TileCfg<double>::kCells;
     const int cap = mixed ? TileCfg<float>::kMaxG : TileCfg<double>::kMaxG;
     std::vector<int> cand;
-    for (double f : {0.6, 0.75, 0.9, 1.0, 1.15})
-        cand.push_back(std::min(cap, std::max(8, (int)std::lround(f * dflt))));
-    std::vector<double> U0[4];
-    for (auto &u : U0) u.resize(m->num_element);
-    double *Urows[4] = {U0[0].data(), U0[1].data(), U0[2].data(), U0[3].data()};
-    int rc = fvmr_field_init(m, p, FVMR_INIT_UNIFORM, nullptr, Urows);
-    if (rc) return rc;
-    const double *Uc[4] = {Urows[0], Urows[1], Urows[2], Urows[3]};
+    for (double f : {0.6, 0.75, 0.9, 1.0, 1.15}) {
+        const int c = std::min(cap, std::max(8, (int)std::lround(f * dflt)));
+        if (std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+    }
+    return cand;
+}
+
+// Times `steps` steps of the uniform freestream of each case on a handle made by `make` for
+// every candidate tile size (after 2 untimed steps) and returns the fastest; timings: the ms per
+// step of every candidate, in candidate order (may be NULL).
+template <class Make>
+static int autotune_tiles(int ncases, const fvmr_mesh_soa *const *meshes, const fvmr_params *params, int steps,
+                          Make make, int *best_tile_cells, double *best_ms, double *timings) {
+    const bool mixed = params[0].precision == FVMR_PREC_MIXED;
+    const std::vector<int> cand = tile_candidates(mixed);
+    // the cases' uniform freestream states
+    std::vector<std::vector<double>> U0(4 * (size_t)ncases);
+    for (int c = 0; c < ncases; c++) {
+        double *rows[4];
+        for (int k = 0; k < 4; k++) {
+            U0[4 * c + k].resize(meshes[c]->num_element);
+            rows[k] = U0[4 * c + k].data();
+        }
+        int rc = fvmr_field_init(meshes[c], &params[c], FVMR_INIT_UNIFORM, nullptr, rows);
+        if (rc) return rc;
+    }
     double best = 1e300;
-    int best_c = dflt;
-    for (int c : cand) {
-        fvmr_params q = *p;
-        q.tile_cells = c;
-        q.kernels = FVMR_KERNELS_TILE;
+    int best_c = mixed ? TileCfg<float>::kCells : TileCfg<double>::kCells;
+    for (size_t q = 0; q < cand.size(); q++) {
         fvmr_handle *h = nullptr;
-        if ((rc = fvmr_create(m, &q, &h))) return rc;
-        double t = 0.0;
+        int rc = make(cand[q], &h);
+        if (rc) return rc;
+        for (int c = 0; c < ncases && !rc; c++) {
+            const double *rows[4] = {U0[4 * c].data(), U0[4 * c + 1].data(), U0[4 * c + 2].data(), U0[4 * c + 3].data()};
+            rc = (ncases == 1) ? fvmr_set_state(h, rows) : fvmr_batch_set_state(h, c, rows);
+        }
+        std::vector<double> t(ncases, 0.0), T(ncases, 1e300);
         float ms = 0.0f;
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        rc = fvmr_set_state(h, Uc);
-        if (!rc) rc = fvmr_advance(h, 2, &t, 1e300, nullptr);
+        if (!rc) rc = fvmr_batch_advance(h, 2, t.data(), T.data());
         if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
             rc = set_error(FVMR_ERR_HIP, "hipEventCreate failed");
         if (!rc && hipEventRecord(e0, h->stream) != hipSuccess) rc = set_error(FVMR_ERR_HIP, "hipEventRecord failed");
-        if (!rc) rc = fvmr_advance(h, steps, &t, 1e300, nullptr);
+        if (!rc) rc = fvmr_batch_advance(h, steps, t.data(), T.data());
         if (!rc && (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                     hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
             rc = set_error(FVMR_ERR_HIP, "event timing failed");
@@ -3105,10 +3210,48 @@ int fvmr_autotune(const fvmr_mesh_soa *m, const fvmr_params *p, int steps, int *
         if (e1) (void)hipEventDestroy(e1);
         fvmr_destroy(h);
         if (rc) return rc;
-        if (ms < best) { best = ms; best_c = c; }
+        if (timings) timings[q] = ms / steps;
+        if (ms < best) { best = ms; best_c = cand[q]; }
     }
     *best_tile_cells = best_c;
     if (best_ms) *best_ms = best / steps;
+    return FVMR_OK;
+}
+
+extern "C" {
+
+// Times `steps` SSP-RK3 / Euler steps of the mesh's uniform freestream for each candidate tile
+// size (0.6 - 1.15 x the shape's default, clipped to the LDS capacity) and returns the fastest.
+int fvmr_autotune(const fvmr_mesh_soa *m, const fvmr_params *p, int steps, int *best_tile_cells, double *best_ms) {
+    if (!m || !p || !best_tile_cells || steps < 1) return set_error(FVMR_ERR_INVALID_ARG, "invalid argument");
+    const fvmr_mesh_soa *ms[1] = {m};
+    return autotune_tiles(1, ms, p, steps, [&](int tc, fvmr_handle **h) {
+        fvmr_params q = *p;
+        q.tile_cells = tc;
+        q.kernels = FVMR_KERNELS_TILE;
+        return fvmr_create(m, &q, h);
+    }, best_tile_cells, best_ms, nullptr);
+}
+
+// The same over the cases of a batch (one batch handle per candidate).
+int fvmr_autotune_batch(int ncases, const fvmr_mesh_soa *const *meshes, const fvmr_params *params, int steps,
+                        int *best_tile_cells, double *best_ms, double *timings) {
+    if (ncases < 1 || !meshes || !params || !best_tile_cells || steps < 1)
+        return set_error(FVMR_ERR_INVALID_ARG, "invalid argument");
+    std::vector<fvmr_params> q(params, params + ncases);
+    return autotune_tiles(ncases, meshes, params, steps, [&](int tc, fvmr_handle **h) {
+        for (auto &x : q) { x.tile_cells = tc; x.kernels = FVMR_KERNELS_TILE; }
+        return fvmr_create_batch(ncases, meshes, q.data(), h);
+    }, best_tile_cells, best_ms, timings);
+}
+
+// Candidate tile sizes of fvmr_autotune / fvmr_autotune_batch for a precision.
+int fvmr_autotune_candidates(int precision, int *n, int *tile_cells) {
+    if (!n || (precision != FVMR_PREC_FP64 && precision != FVMR_PREC_MIXED))
+        return set_error(FVMR_ERR_INVALID_ARG, "invalid argument");
+    const std::vector<int> cand = tile_candidates(precision == FVMR_PREC_MIXED);
+    *n = (int)cand.size();
+    if (tile_cells) std::copy(cand.begin(), cand.end(), tile_cells);
     return FVMR_OK;
 }
 

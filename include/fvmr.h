@@ -266,6 +266,12 @@ int fvmr_global_dt(fvmr_handle *h, double *dt);
 int fvmr_residual(fvmr_handle *h, int norm, double out[4]);
 /* The cell permutation used on the device: perm[new] = reference index. */
 int fvmr_permutation(fvmr_handle *h, int *perm);
+/* parity hook: the FP64 quotient a[i] / b[i] and reciprocal 1 / b[i] as the device kernels form
+ * them (hardware reciprocal refined by Newton-Raphson, the quotient corrected by its residual),
+ * on n operand pairs on the current device.  Equal to IEEE division on the solver's operands;
+ * b = 0 or b = +-inf give NaN where IEEE gives +-inf or 0 (no solver quotient has such a
+ * divisor on a valid state). */
+int fvmr_device_division(int n, const double *a, const double *b, double *quotient, double *reciprocal);
 
 /* ---------------------------------------------------------------------- */
 /* batches: many independent cases advanced together (no reference         */
@@ -284,6 +290,9 @@ int fvmr_batch_set_state(fvmr_handle *h, int c, const double *const U[4]);
 int fvmr_batch_get_state(fvmr_handle *h, int c, double *const U[4]);
 /* nsteps steps of every case: t[c] in/out, each case's dt clamped to T[c] - t[c]. */
 int fvmr_batch_advance(fvmr_handle *h, int nsteps, double *t, const double *T);
+/* Case c after the last fvmr_batch_advance: t, dt of its last step and the cells of the case
+ * that the restriction repaired in that step (the other fields are left as they are). */
+int fvmr_batch_step_info(fvmr_handle *h, int c, fvmr_step_info *info);
 
 /* ---------------------------------------------------------------------- */
 /* output quantities (FieldWriter, FVMR/src/output/FieldWriter.cpp)         */
@@ -374,6 +383,13 @@ int fvmr_use_graphs(fvmr_handle *h, int on);
  * of p on mesh m for several tile sizes and returns the fastest (set it as p->tile_cells) and
  * its ms per step (best_ms may be NULL). */
 int fvmr_autotune(const fvmr_mesh_soa *m, const fvmr_params *p, int steps, int *best_tile_cells, double *best_ms);
+/* The same for a batch (fvmr_create_batch's arguments): one batch handle per candidate tile
+ * size; timings (may be NULL) receives the ms per step of every candidate, in the order of
+ * fvmr_autotune_candidates. */
+int fvmr_autotune_batch(int ncases, const fvmr_mesh_soa *const *meshes, const fvmr_params *params, int steps,
+                        int *best_tile_cells, double *best_ms, double *timings);
+/* The candidate tile sizes of the tuners for a precision (FVMR_PREC_*): *n of them (at most 8). */
+int fvmr_autotune_candidates(int precision, int *n, int *tile_cells);
 /* 1: record HIP events around every kernel launch on the handle's stream. */
 int fvmr_profile_enable(fvmr_handle *h, int on);
 /* Average ms per launch and launch count of a kernel class since the last reset:

@@ -83,3 +83,18 @@ def test_invalid_params_rejected_before_device():
         with pytest.raises(F.FvmrError) as e:
             F.GPUSolver2(m, p)
         assert msg in str(e.value)
+
+
+def test_state_arrays_must_be_four_rows_of_the_cell_count(tmp_path):
+    """The library copies num_element doubles from each of the four rows: a state of another
+    shape is refused before any copy."""
+    import cfd_solver_2d_gpu_amd as FF
+    with pytest.raises(ValueError):
+        FF._rows(np.zeros((4, 5)), 6)
+    with pytest.raises(ValueError):
+        FF._rows(np.zeros((3, 6)), 6)
+    U, rows = FF._rows(np.zeros((4, 6)), 6)
+    assert U.shape == (4, 6)
+    m = FF.Mesh.generate(FF.GEN_PERIODIC_SQUARE, 4, 4, b=1.0)
+    with pytest.raises(ValueError):
+        m.write_continue(tmp_path / "pause_x[1].dat", np.ones((4, m.num_element - 1)), 0.0, 1, 0.5)

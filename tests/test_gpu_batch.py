@@ -89,3 +89,36 @@ def test_batch_refuses_mixed_settings():
     with pytest.raises(F.FvmrError) as e:
         F.BatchSolver([m for m, _, _ in cs], [p for _, p, _ in cs])
     assert "shared setting" in str(e.value)
+
+
+def test_batch_restriction_counts_per_case():
+    """A bad cell injected into one case (not the first) is repaired in that case and counted
+    there: the case's restriction count and state equal the same case on its own handle, and the
+    other cases count none."""
+    announce("batch: restriction of a bad cell of case 3 counted in case 3")
+    cs = cases()
+    k = 3
+    m, p, U = cs[k]
+    U = U.copy()
+    U[0, U.shape[1] // 2] = -1.0                  # negative density
+    cs[k] = (m, p, U)
+    b = F.BatchSolver([m for m, _, _ in cs], [p for _, p, _ in cs])
+    for c, (_, _, Uc) in enumerate(cs):
+        b.set_state(c, Uc)
+    T = np.full(len(cs), 1e-3)                     # a fixed end time: both runs clamp alike
+    t = b.advance(1, np.zeros(len(cs)), T)
+    counts = [b.step_info(c).n_restricted for c in range(len(cs))]
+    g = F.GPUSolver2(m, p)
+    g.set_state(U)
+    tg, ig = g.iteration_device(0.0, T[k])
+    assert ig.n_restricted >= 1
+    assert counts[k] == ig.n_restricted, counts
+    assert sum(counts) == counts[k], counts
+    assert b.step_info(k).t == tg == t[k]
+    Ub, Ug = b.get_state(k), g.get_state()
+    assert np.array_equal(np.isnan(Ub), np.isnan(Ug))
+    assert np.array_equal(Ub[~np.isnan(Ub)], Ug[~np.isnan(Ug)])
+    g.close()
+    # a further step without bad cells counts none anywhere
+    b.advance(1, t, np.full(len(cs), 1e9))
+    b.close()
