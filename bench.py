@@ -155,7 +155,7 @@ def accuracy_run(scheme, precisions, steps, kernels=None):
     return out
 
 
-def traffic_from_profiles(kernel="k_tile"):
+def traffic_from_profiles(kernel="k_tile", precision="fp64"):
     """HBM bytes per launch of the stage kernel (k_tile, or the cell family's k_stage) measured
     with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), scaled by the copy-kernel calibration;
     committed in profiles/ (None when absent)."""
@@ -163,7 +163,7 @@ def traffic_from_profiles(kernel="k_tile"):
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch_fp64:"):
+        if line.startswith(f"traffic_bytes_per_launch_{precision}:"):
             return float(line.split(":")[1])
     return None
 
@@ -514,6 +514,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--clock-warmup", type=int, default=CLOCK_WARMUP_STEPS,
+                    help="untimed steps before --warmup that bring the GPU clocks up (0 for counter runs)")
     ap.add_argument("--windows", type=int, default=7,
                     help="timed windows of --steps steps each (at least 5); value is their median")
     ap.add_argument("--no-cpu", action="store_true")
@@ -522,6 +524,9 @@ def main():
     ap.add_argument("--c5-only", action="store_true", help="run only the adjoint leg and print its JSON")
     ap.add_argument("--c5-iters", type=int, nargs=3, default=[260000, 150000, 140000],
                     metavar=("PRIMAL", "ADJOINT", "FD"), help="iteration budgets of the adjoint leg")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="arithmetic of the headline leg (mixed: FP32 flux/gradients, FP64 state; for profiling it "
+                         "alone; the FP32-mixed comparison leg runs in any case)")
     ap.add_argument("--kernels", choices=("tile", "cell"), default="tile",
                     help="kernel family of the FP64 headline leg (the cell family is the fallback for meshes "
                          "whose geometry the tile kernels cannot recompute)")
@@ -547,33 +552,38 @@ def main():
         del src, dst
 
     cell = args.kernels == "cell"
+    mixed_head = args.precision == "mixed"
     reps = max(5, args.windows)
-    m, walls, U64, kern = run_gpu(F.SOL_ROE, F.PREC_FP64, args.steps, args.warmup, profile=True,
-                                  kernels=F.KERNELS_CELL if cell else None, reps=reps)
+    m, walls, U64, kern = run_gpu(F.SOL_ROE, F.PREC_MIXED if mixed_head else F.PREC_FP64, args.steps, args.warmup,
+                                  profile=True, kernels=F.KERNELS_CELL if cell else None, reps=reps,
+                                  clock_warmup=args.clock_warmup)
     nc, nf, nv = m.num_element, m.num_edge, m.num_node
     win = window_stats(walls, args.steps)
     ms_step = win["median_ms_per_step"]
     value = nc * 3 / (ms_step * 1e-3)
 
     st_ms, st_n = kern["k_stage" if cell else "k_tile"]
-    b_launch = stage_bytes(nc, nf, nv, 8, 8, 8)
+    sg = 4 if mixed_head else 8
+    b_launch = stage_bytes(nc, nf, nv, 8, sg, sg)
     achieved = b_launch / (st_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles("k_stage" if cell else "k_tile")
+    traffic = traffic_from_profiles("k_stage" if cell else "k_tile", args.precision)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": ("k_stage<double,double> (one RK stage after k_gradient: fluxes, update; cell family)" if cell
-                           else "k_tile<double,double,false,false,1> (one RK stage: gradients, limiter, fluxes, "
-                                "update; Green-Gauss + Barth + Roe fixed at compile time)"),
+                           else "k_tile<%s,false,false,1> (one RK stage: gradients, limiter, fluxes, "
+                                "update; Green-Gauss + Barth + Roe fixed at compile time)"
+                                % ("float,float" if mixed_head else "double,double")),
                 "bytes_per_launch": int(b_launch), "avg_launch_ms": round(st_ms, 5), "launches": st_n}
-    path_b = path_bytes_per_step(nc, nf, nv, 8, 8, 8)
+    path_b = path_bytes_per_step(nc, nf, nv, 8, sg, sg)
     out = {
         "metric": "cell-RK-stage updates/s + % HBM peak; FP32-mixed vs FP64 at stated L2 error",
         "value": value, "unit": "cell-RK-stage updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "clock_warmup_steps": CLOCK_WARMUP_STEPS,
+        "vs_baseline": None, "dtype": "f32-mixed" if mixed_head else "f64", "data": "synthetic",
+        "clock_warmup_steps": args.clock_warmup,
         "config": {"workload": "wedge M=2, 1024x512x2 = 1,048,576 triangles (jitter 10%, seed 2024), "
-                               "Roe, Green-Gauss + Barth, linear reconstruction, SSP-RK3, CFL 0.5, FP64",
+                               "Roe, Green-Gauss + Barth, linear reconstruction, SSP-RK3, CFL 0.5, "
+                               + ("FP32-mixed" if mixed_head else "FP64"),
                    "cells": nc, "edges": nf, "nodes": nv},
         "roofline": roofline,
         # value and ms_per_step: the median of the timed windows of `steps` steps
@@ -589,7 +599,7 @@ def main():
                                                                    + kern["_d2h_s"])},
     }
     if not args.no_extra:
-        _, walls_mx, Umx, kmx = run_gpu(F.SOL_ROE, F.PREC_MIXED, args.steps, args.warmup, profile=True, reps=reps)
+        _, walls_mx, Umx, kmx = run_gpu(F.SOL_ROE, F.PREC_MIXED, args.steps, args.warmup, profile=True, reps=reps, clock_warmup=args.clock_warmup)
         win_mx = window_stats(walls_mx, args.steps)
         kmx = {k: v for k, v in kmx.items() if not k.startswith("_")}
         acc = accuracy_run(F.SOL_ROE, (F.PREC_FP64, F.PREC_MIXED), 1000)
@@ -603,10 +613,10 @@ def main():
                                        "rel_l2_density_vs_fp64": float(np.linalg.norm(Amx[0] - A64[0])
                                                                        / np.linalg.norm(A64[0])),
                                        "errors_vs_fp64": var_errors(Amx, A64)}}
-        _, walls_llf, _, _ = run_gpu(F.SOL_LLF, F.PREC_FP64, args.steps, args.warmup, profile=False, reps=reps)
+        _, walls_llf, _, _ = run_gpu(F.SOL_LLF, F.PREC_FP64, args.steps, args.warmup, profile=False, reps=reps, clock_warmup=args.clock_warmup)
         out["llf_fp64_value"] = nc * 3 / (window_stats(walls_llf, args.steps)["median_ms_per_step"] * 1e-3)
         _, walls_cell, Ucell, kcell = run_gpu(F.SOL_ROE, F.PREC_FP64, args.steps, args.warmup, profile=True,
-                                              kernels=F.KERNELS_CELL, reps=reps)
+                                              kernels=F.KERNELS_CELL, reps=reps, clock_warmup=args.clock_warmup)
         out["cell_kernels"] = {"value": nc * 3 / (window_stats(walls_cell, args.steps)["median_ms_per_step"] * 1e-3),
                                "rel_l2_density_vs_tile": float(np.linalg.norm(Ucell[0] - U64[0]) / np.linalg.norm(U64[0])),
                                "kernels_ms": {k: round(v[0], 5) for k, v in kcell.items() if not k.startswith("_")}}
