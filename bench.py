@@ -411,6 +411,17 @@ def config_c4(ncases=256, steps=500, seed=0):
                                  "value": sub_cells * 3 * sub_steps / wall_sep}}
 
 
+def adjoint_product_bytes(nc, nf, nv):
+    """Algorithmic bytes of one adjoint product (k_adj_faces, k_adj_vjp, k_adj_update), every array
+    once, intermediates written and read: per cell the state (32), the steady state's gradients
+    (64), psi in, psi of the stage start and psi out (3 x 32), cell geometry (32), face slots (24),
+    vertices (12), the per-cell share written and read (2 x 32), dJ/dU (32) and the pseudo-time
+    step (8); per face the L/R/BC record (16), normal and length (32), midpoint (16), the two
+    sides' face adjoints written and read (2 x 64) and the two per-side shares written and read
+    (2 x 64); per node its coordinates (16)."""
+    return nc * (32 + 64 + 3 * 32 + 32 + 24 + 12 + 2 * 32 + 32 + 8) + nf * (16 + 32 + 16 + 2 * 64 + 2 * 64) + nv * 16
+
+
 def config_c5(primal_iters=260000, adjoint_iters=150000, fd_iters=140000, h=1e-6):
     """Discrete adjoint on the NACA 0012 O-grid 512 x 384 x 2 = 393,216 triangles, farfield at 20
     chords (radial growth 1.01), M 0.5, AoA 2 deg, Euler, Roe, Green-Gauss, linear reconstruction
@@ -462,6 +473,8 @@ def config_c5(primal_iters=260000, adjoint_iters=150000, fd_iters=140000, h=1e-6
     print(f"[c5] dJ/dalpha adjoint {res.dJ_dalpha:.10f}, central differences {fd:.10f}, rel {rel:.2e}",
           file=sys.stderr, flush=True)
     nc = m.num_element
+    bprod = adjoint_product_bytes(nc, m.num_edge, m.num_node)
+    gbs = 3 * bprod / (t_adj / res.iters) / 1e9
     return {"workload": "NACA 0012 O-grid 512x384x2 = 393,216 triangles, farfield 20c, M 0.5, AoA 2 deg, Euler, "
                         "Roe, Green-Gauss, no limiter, SSP-RK3 pseudo-time CFL 2 (per-cell dt), FP64",
             "cells": nc, "J_lift": J0, "dJ_dalpha_adjoint": res.dJ_dalpha, "dJ_dalpha_central_diff": fd,
@@ -470,7 +483,10 @@ def config_c5(primal_iters=260000, adjoint_iters=150000, fd_iters=140000, h=1e-6
                        "ms_per_iteration": t_primal / it_p * 1e3},
             "adjoint": {"iterations": res.iters, "residual_first": res.res0, "residual_last": res.res,
                         "seconds": t_adj, "ms_per_iteration": t_adj / res.iters * 1e3,
-                        "cell_products_per_s": nc * 3 * res.iters / t_adj},
+                        "cell_products_per_s": nc * 3 * res.iters / t_adj,
+                        "roofline": {"bound": "hbm", "bytes_per_product": bprod, "products_per_iteration": 3,
+                                     "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": round(gbs / HBM_PEAK_GBS, 4)}},
             "fd_solves": {"iterations_each": fd_iters, "seconds": t_fd}}
 
 

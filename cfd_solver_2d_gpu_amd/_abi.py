@@ -7,7 +7,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libfvmr.so")
+# the in-tree build; FVMR_LIBRARY may name another build of the same sources (a variant built
+# with other compile-time settings by cfd_solver_2d_gpu_amd.build.build_library)
+LIB_PATH = os.environ.get("FVMR_LIBRARY") or os.path.join(HERE, "libfvmr.so")
 
 # status codes
 FVMR_OK = 0

@@ -71,6 +71,13 @@ constexpr int kGroup3 = 3 * kCellsPerGroup;     // one wave64 per face slot
 #ifndef FVMR_XCD_REMAP
 #define FVMR_XCD_REMAP 1
 #endif
+// 1: the stage kernel k_tile loads each local cell's centroid and each face's normal (packed
+// per tile on the host, equal bit for bit to what it would recompute) instead of recomputing
+// them from the node coordinates (about 1% faster at 1M cells).  The time-step kernel always
+// recomputes its normals: it is bound by its loads, and loading them cost it 6%.
+#ifndef FVMR_TILE_GEOM_LOADED
+#define FVMR_TILE_GEOM_LOADED 1
+#endif
 #if FVMR_STAGE_WPE > 0
 #define FVMR_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(FVMR_STAGE_WPE)))
 #else
@@ -1084,6 +1091,8 @@ struct TileArgs {
     const V4<G> *gcoef;       // per gradient cell: Green-Gauss weights of the 3 slots, volume
     const int4 *face;         // per face: nodes | BC code, L | R | pair face, float length, edge
     const DtRec *dtrec;       // per device cell
+    const V2<G> *lcxy;        // per local cell: centroid (FVMR_TILE_GEOM_LOADED)
+    const V2<G> *fnrm;        // per face: normal (FVMR_TILE_GEOM_LOADED)
     const CaseBC *cbc;        // per case of the batch (one case: the handle's)
     double *dtl;              // per-cell steps of a steady solve (k_tile_dt / k_tile with LDT), else NULL
 };
@@ -1216,10 +1225,21 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
     const int4 fr0 = (tid < nF) ? ta.face[offF + tid] : make_int4(0, 0, 0, 0);
     const int4 fr1 = (tid + kTileThreads < nF) ? ta.face[offF + tid + kTileThreads] : make_int4(0, 0, 0, 0);
     const S4 un = (!first && i < nT) ? Un[c0 + i] : S4{0.0, 0.0, 0.0, 0.0};
+#if FVMR_TILE_GEOM_LOADED
+    const V2<G> cc = (i < nL) ? ta.lcxy[offL + i] : V2<G>{G(0), G(0)};
+    const V2<G> nr0 = (tid < nF) ? ta.fnrm[offF + tid] : V2<G>{G(0), G(0)};
+    const V2<G> nr1 = (tid + kTileThreads < nF) ? ta.fnrm[offF + tid + kTileThreads] : V2<G>{G(0), G(0)};
+#endif
     if (tid < nN) sN[tid] = nd;
     if (tid < nF) sFace[tid] = fr0;
     if (tid + kTileThreads < nF) sFace[tid + kTileThreads] = fr1;
     if (i < nL) sU[i] = SU{(C)uin.x, (C)uin.y, (C)uin.z, (C)uin.w};
+#if FVMR_TILE_GEOM_LOADED
+    if (i < nL) sC[i] = cc;
+    if (tid < nF) sNrm[tid] = nr0;
+    if (tid + kTileThreads < nF) sNrm[tid + kTileThreads] = nr1;
+    __syncthreads();
+#else
     __syncthreads();
     // 2. centroids and face normals
     if (i < nL) {
@@ -1233,6 +1253,7 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
         sNrm[k] = V2<G>{eg.x, eg.y};
     }
     __syncthreads();
+#endif
     // 3. limited gradients of the tile and the first ring
     if (i < nG) {
         const int sw[3] = {gs.x, gs.y, gs.z};
@@ -1405,9 +1426,10 @@ __global__ __launch_bounds__(TileCfg<G>::kThreads) void k_tile_dt(TileArgs<G> ta
     if (tid < nFF) sFace[tid] = fr;
     if (tid < nG) sU[tid] = u;
     __syncthreads();
-    // faces of the tile cells: L and R (when present) are tile or first-ring cells
-    for (int f = tid; f < nFF; f += kTileThreads) {
-        const int4 fr = sFace[f];
+    // faces of the tile cells (one per thread): L and R (when present) are tile or first-ring cells
+    static_assert(kTileMaxFF <= kTileThreads, "one face per thread");
+    if (tid < nFF) {
+        const int f = tid;
         const int L = lfield(fr.y, 0), R = lfield(fr.y, 1);
         V4<G> eg;
         V2<G> em;
@@ -1483,7 +1505,8 @@ __global__ __launch_bounds__(TileCfg<G>::kThreads) void k_tile_dt(TileArgs<G> ta
                 // the periodic pair of face f, seen from this cell (the face's L) and its R
                 const int4 ff = sFace[f];
                 const int out = lfield(ff.y, 1);
-                const int4 fr = ta.face[offF + lfield(ff.y, 2)];
+                const int pf = lfield(ff.y, 2);
+                const int4 fr = ta.face[offF + pf];
                 V4<G> eg;
                 V2<G> em;
                 tile_face_geom<G>(sN, fr, eg, em);
@@ -2015,6 +2038,7 @@ struct fvmr_solver {
     V2<double> *t_nxy = nullptr;
     int4 *t_gslot = nullptr, *t_face = nullptr;
     void *t_gcoef = nullptr;
+    void *t_lcxy = nullptr, *t_fnrm = nullptr;   // V2<G> per local cell / per face (FVMR_TILE_GEOM_LOADED)
     DtRec *t_dtrec = nullptr;
     // output quantities: node -> cells (caller's element order), node buffers, boundary edges
     int *nc_off = nullptr, *nc_cells = nullptr;
@@ -2055,6 +2079,7 @@ TileArgs<G> tile_args(const fvmr_solver *h) {
     TileArgs<G> t;
     t.info = h->t_info; t.halo = h->t_halo; t.nxy = h->t_nxy; t.lnodes = h->t_lnodes;
     t.gslot = h->t_gslot; t.gcoef = (const V4<G> *)h->t_gcoef; t.face = h->t_face; t.dtrec = h->t_dtrec;
+    t.lcxy = (const V2<G> *)h->t_lcxy; t.fnrm = (const V2<G> *)h->t_fnrm;
     t.cbc = h->cbc;
     t.dtl = nullptr;
     return t;
@@ -2301,7 +2326,7 @@ void free_all(fvmr_solver *h) {
     for (auto &b : h->Ubuf) if (b) (void)hipFree(b);
     void *ptrs[] = {h->grad, h->rhs, h->fix, h->cgeo, h->nxy, h->egeo, h->emid, h->cslot, h->cnode, h->dtcode,
                     h->eint, h->partial, h->norm_out, h->bad_list, h->sc, h->t_info, h->t_halo, h->t_lnodes,
-                    h->t_nxy, h->t_gslot, h->t_face, h->t_gcoef, h->t_dtrec,
+                    h->t_nxy, h->t_gslot, h->t_face, h->t_gcoef, h->t_dtrec, h->t_lcxy, h->t_fnrm,
                     h->nc_off, h->nc_cells, h->node_buf, h->cbc, h->tile_case_begin, h->perm_dev, h->io_soa,
                     h->bedge_ref_dev, h->bedge_nodes_dev, h->bedge_geo_dev};
     for (void *q : ptrs) if (q) (void)hipFree(q);
@@ -2544,12 +2569,15 @@ static int create_impl(const fvmr_mesh_soa *m, const fvmr_params *p, int ncases,
                 elen[e] = m->edge_length[eperm[e]];
             }
             for (int i = 0; i < nc; i++) vol[i] = m->element_volume[h->perm[i]];
-            std::vector<double> ccx(nc), ccy(nc), emx(nf), emy(nf);
+            std::vector<double> ccx(nc), ccy(nc), emx(nf), emy(nf), enx(nf), eny(nf);
             for (int i = 0; i < nc; i++) { ccx[i] = m->element_xy[0][h->perm[i]]; ccy[i] = m->element_xy[1][h->perm[i]]; }
-            for (int e = 0; e < nf; e++) { emx[e] = m->edge_xy[0][eperm[e]]; emy[e] = m->edge_xy[1][eperm[e]]; }
+            for (int e = 0; e < nf; e++) {
+                emx[e] = m->edge_xy[0][eperm[e]]; emy[e] = m->edge_xy[1][eperm[e]];
+                enx[e] = m->edge_normal[0][eperm[e]]; eny[e] = m->edge_normal[1][eperm[e]];
+            }
             fvmr::TileInput ti{nc, nf, sx.data(), sy.data(), cnode.data(), eL.data(), eR.data(), et.data(), ep.data(),
                                en0.data(), en1.data(), elen.data(), m->node_xy[0], m->node_xy[1], vol.data(), dtcode.data(),
-                               ccx.data(), ccy.data(), emx.data(), emy.data(), ncases, case_begin};
+                               ccx.data(), ccy.data(), emx.data(), emy.data(), enx.data(), eny.data(), ncases, case_begin};
             auto caps_of = [&](auto cfg) {
                 using Cfg = decltype(cfg);
                 return fvmr::TileCaps{std::min(p->tile_cells ? p->tile_cells : Cfg::kCells, Cfg::kMaxG), Cfg::kMaxG,
@@ -2585,6 +2613,29 @@ static int create_impl(const fvmr_mesh_soa *m, const fvmr_params *p, int ncases,
                 V4<double> *d = nullptr;
                 if ((rc = upload(&d, gcd, h->stream))) return bail(rc);
                 h->t_gcoef = d;
+            }
+            if (FVMR_TILE_GEOM_LOADED) {
+                // centroids and normals as the kernels would recompute them (standard geometry:
+                // the caller's values), rounded to the kernels' geometry type
+                auto pack = [&](const std::vector<double> &xy, void **dst) -> int {
+                    if (h->mixed) {
+                        std::vector<V2<float>> v(xy.size() / 2);
+                        for (size_t k = 0; k < v.size(); k++) v[k] = V2<float>{(float)xy[2 * k], (float)xy[2 * k + 1]};
+                        V2<float> *d = nullptr;
+                        int r = upload(&d, v, h->stream);
+                        *dst = d;
+                        if (!r && hipStreamSynchronize(h->stream) != hipSuccess) r = set_error(FVMR_ERR_HIP, "tile upload failed");
+                        return r;
+                    }
+                    std::vector<V2<double>> v(xy.size() / 2);
+                    for (size_t k = 0; k < v.size(); k++) v[k] = V2<double>{xy[2 * k], xy[2 * k + 1]};
+                    V2<double> *d = nullptr;
+                    int r = upload(&d, v, h->stream);
+                    *dst = d;
+                    if (!r && hipStreamSynchronize(h->stream) != hipSuccess) r = set_error(FVMR_ERR_HIP, "tile upload failed");
+                    return r;
+                };
+                if ((rc = pack(tm.lcxy, &h->t_lcxy)) || (rc = pack(tm.fnrm, &h->t_fnrm))) return bail(rc);
             }
             std::vector<DtRec> dtr(nc);
             for (int i = 0; i < nc; i++)

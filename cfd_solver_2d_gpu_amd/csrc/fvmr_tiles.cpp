@@ -208,6 +208,8 @@ int build_tiles(const TileInput &in, const TileCaps &caps, TileMesh &out) {
         for (int v : nodes) { out.nxy.push_back(in.node_x[v]); out.nxy.push_back(in.node_y[v]); }
         for (int i = 0; i < nL; i++) {
             const int c = cells[i];
+            out.lcxy.push_back(in.cx[c]);
+            out.lcxy.push_back(in.cy[c]);
             int w = 0;
             for (int j = 0; j < 3; j++) w |= nid[in.cnode[(size_t)j * nc + c]] << (kLocalBits * j);
             out.lnodes.push_back(w);
@@ -246,6 +248,8 @@ int build_tiles(const TileInput &in, const TileCaps &caps, TileMesh &out) {
             out.face.push_back(loc(in.eL[e]) | (loc(in.eR[e]) << kLocalBits) | (pl << (2 * kLocalBits)));
             out.face.push_back(lbits);
             out.face.push_back(e);
+            out.fnrm.push_back(in.nx[e]);
+            out.fnrm.push_back(in.ny[e]);
         }
         out.max_nL = std::max(out.max_nL, nL); out.max_nG = std::max(out.max_nG, nG);
         out.max_nF = std::max(out.max_nF, nF); out.max_nFF = std::max(out.max_nFF, nFF);

@@ -48,6 +48,8 @@ struct TileMesh {
     std::vector<int> lnodes;      // per local cell: 3 local node ids (10 bits each)
     std::vector<int> gslot;       // per gradient cell: 3 slot words + dtcode (tile cells)
     std::vector<double> gcoef;    // per gradient cell: Green-Gauss weight of slots 0-2, volume
+    std::vector<double> lcxy;     // per local cell: centroid x, y (the caller's, as lnodes)
+    std::vector<double> fnrm;     // per face: normal x, y (the caller's, as face)
     std::vector<int> face;        // 4 ints per face
     std::vector<int> dtface;      // per device cell: its 3 slot faces, local ids in its tile
     std::vector<int> case_tile_begin;  // tiles of case c: [c] .. [c+1]-1
@@ -69,6 +71,7 @@ struct TileInput {
     const int *dtcode;    // per cell
     const double *cx, *cy;  // cell centroids
     const double *mx, *my;  // edge midpoints
+    const double *nx, *ny;  // edge normals
     int ncases;             // cases of a batch: device cells case_begin[c] .. [c+1]-1 (tiles never
     const int *case_begin;  // cross a case boundary)
 };
