@@ -340,11 +340,12 @@ def config_c4(ncases=256, steps=500, seed=0):
     # candidate), then the whole batch with the tile size it picks
     t0 = time.perf_counter()
     tune_cases = list(range(0, ncases, 8))
-    best_tc, best_ms, timings = F.autotune_batch([meshes[c] for c in tune_cases], [params[c] for c in tune_cases],
-                                                 steps=10)
+    best_th, best_tc, best_ms, timings = F.autotune_batch([meshes[c] for c in tune_cases],
+                                                          [params[c] for c in tune_cases], steps=10)
     tune_s = time.perf_counter() - t0
-    default_tc = 300   # the FP32-mixed shape's default tile size
+    default = F.autotune_candidates(F.PREC_MIXED)[0]   # the default work-group shape and tile size
     for q in params:
+        q.tile_threads = best_th
         q.tile_cells = best_tc
     t0 = time.perf_counter()
     b = F.BatchSolver(meshes, params)
@@ -401,9 +402,10 @@ def config_c4(ncases=256, steps=500, seed=0):
             "value": total * 3 * steps / wall, "unit": "cell-RK-stage updates/s", "ms_per_step": wall / steps * 1e3,
             "setup_s": setup, "kernels_ms": kern,
             "autotune": {"cases": len(tune_cases), "cells": sum(meshes[c].num_element for c in tune_cases),
-                         "steps": 10, "tile_cells": best_tc, "ms_per_step": best_ms,
-                         "ms_per_step_by_tile_cells": timings, "default_tile_cells": default_tc,
-                         "gain_vs_default": (timings.get(default_tc, best_ms) / best_ms) if best_ms else None,
+                         "steps": 10, "tile_threads": best_th, "tile_cells": best_tc, "ms_per_step": best_ms,
+                         "ms_per_step_by_threads_and_tile_cells": {f"{a}x{b}": v for (a, b), v in timings.items()},
+                         "default": {"tile_threads": default[0], "tile_cells": default[1]},
+                         "gain_vs_default": (timings.get(default, best_ms) / best_ms) if best_ms else None,
                          "seconds": tune_s},
             "mixed_vs_fp64_sample": {"steps": 2 + steps, "cases": errs,
                                      "max_rel_l2": {k: max(e[k]["l2"] for e in errs) for k in ("rho", "rhou", "rhov", "rhoE")}},

@@ -201,34 +201,37 @@ def _rows(U, n=None):
 
 
 def autotune(mesh, params, steps=20):
-    """fvmr_autotune: (best tile_cells, ms per step) over candidate tile sizes on this mesh."""
-    best = C.c_int()
+    """fvmr_autotune: (best tile_threads, best tile_cells, ms per step) over the candidate
+    work-group shapes and tile sizes on this mesh."""
+    th, best = C.c_int(), C.c_int()
     ms = C.c_double()
-    check(lib().fvmr_autotune(C.byref(mesh.soa), C.byref(params), steps, C.byref(best), C.byref(ms)), "fvmr_autotune")
-    return best.value, ms.value
+    check(lib().fvmr_autotune(C.byref(mesh.soa), C.byref(params), steps, C.byref(th), C.byref(best), C.byref(ms)),
+          "fvmr_autotune")
+    return th.value, best.value, ms.value
 
 
 def autotune_candidates(precision=PREC_FP64):
-    """The tile sizes the tuners try for a precision."""
+    """The (tile_threads, tile_cells) pairs the tuners try for a precision."""
     n = C.c_int()
-    out = np.zeros(8, dtype=np.int32)
-    check(lib().fvmr_autotune_candidates(precision, C.byref(n), _iptr(out)), "fvmr_autotune_candidates")
-    return [int(x) for x in out[:n.value]]
+    th = np.zeros(16, dtype=np.int32)
+    cells = np.zeros(16, dtype=np.int32)
+    check(lib().fvmr_autotune_candidates(precision, C.byref(n), _iptr(th), _iptr(cells)), "fvmr_autotune_candidates")
+    return [(int(a), int(b)) for a, b in zip(th[:n.value], cells[:n.value])]
 
 
 def autotune_batch(meshes, params, steps=10):
-    """fvmr_autotune_batch: (best tile_cells, its ms per step, {tile_cells: ms per step}) for a
-    batch of these cases."""
+    """fvmr_autotune_batch: (best tile_threads, best tile_cells, its ms per step,
+    {(tile_threads, tile_cells): ms per step}) for a batch of these cases."""
     n = len(meshes)
     soas = (C.POINTER(MeshSoA) * n)(*[C.pointer(m.soa) for m in meshes])
     ps = (Params * n)(*params)
-    best = C.c_int()
+    th, best = C.c_int(), C.c_int()
     ms = C.c_double()
     cand = autotune_candidates(params[0].precision)
-    times = np.zeros(8)
-    check(lib().fvmr_autotune_batch(n, soas, ps, steps, C.byref(best), C.byref(ms), _dptr(times)),
+    times = np.zeros(16)
+    check(lib().fvmr_autotune_batch(n, soas, ps, steps, C.byref(th), C.byref(best), C.byref(ms), _dptr(times)),
           "fvmr_autotune_batch")
-    return best.value, ms.value, {c: float(t) for c, t in zip(cand, times)}
+    return th.value, best.value, ms.value, {c: float(t) for c, t in zip(cand, times)}
 
 
 def device_division(a, b):

@@ -55,7 +55,7 @@ class Params(C.Structure):
         ("equation", C.c_int), ("flux_scheme", C.c_int), ("reconstruct", C.c_int), ("gradient", C.c_int),
         ("limiter", C.c_int), ("time_advance", C.c_int), ("viscosity_model", C.c_int), ("precision", C.c_int),
         ("renumber", C.c_int), ("compat_inv2x2", C.c_int),
-        ("kernels", C.c_int), ("tile_cells", C.c_int),
+        ("kernels", C.c_int), ("tile_cells", C.c_int), ("tile_threads", C.c_int), ("reserved", C.c_int),
         ("inf_ruvp", C.c_double * 4), ("inlet_ruvp", C.c_double * 4), ("outlet_ruvp", C.c_double * 4),
         ("wall_temperature", C.c_double), ("venkat_K", C.c_double),
         ("dsr_shock_x", C.c_double), ("dsr_shock_y", C.c_double), ("dsr_shock_speed", C.c_double),
@@ -117,10 +117,10 @@ SIGNATURES = {
     "fvmr_device_division": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp]),
     "fvmr_node_field": (C.c_int, [C.c_void_p, _dp * 4, _dp, _dp]),
     "fvmr_use_graphs": (C.c_int, [C.c_void_p, C.c_int]),
-    "fvmr_autotune": (C.c_int, [C.POINTER(MeshSoA), C.POINTER(Params), C.c_int, _ip, _dp]),
-    "fvmr_autotune_batch": (C.c_int, [C.c_int, C.POINTER(C.POINTER(MeshSoA)), C.POINTER(Params), C.c_int, _ip, _dp,
-                                      _dp]),
-    "fvmr_autotune_candidates": (C.c_int, [C.c_int, _ip, _ip]),
+    "fvmr_autotune": (C.c_int, [C.POINTER(MeshSoA), C.POINTER(Params), C.c_int, _ip, _ip, _dp]),
+    "fvmr_autotune_batch": (C.c_int, [C.c_int, C.POINTER(C.POINTER(MeshSoA)), C.POINTER(Params), C.c_int, _ip, _ip,
+                                      _dp, _dp]),
+    "fvmr_autotune_candidates": (C.c_int, [C.c_int, _ip, _ip, _ip]),
     "fvmr_set_boundary_states": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "fvmr_vertex_colors": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_int, C.POINTER(C.c_float),
                                      C.POINTER(C.c_float), C.c_int]),

@@ -1039,7 +1039,7 @@ int adj_grid(int n) { return (n + kAdjBlock - 1) / kAdjBlock; }
 // has tiles; both give the same values).
 int local_dt(fvmr_solver *h, const S4 *U) {
     AdjWork *w = (AdjWork *)h->adj;
-    if (h->tiles) {
+    if (h->tiles && h->shape == 0) {   // (other work-group shapes use the cell kernels here)
         TileArgs<double> t = tile_args<double>(h);
         t.dtl = w->dtl;
         if (h->ph.equation == FVMR_EQ_NS)
@@ -1070,7 +1070,7 @@ int steady_rhs(fvmr_solver *h, const S4 *U, S4 *rhs) {
 
 int steady_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, double a, double b, int first, S4 *rhs) {
     const AdjWork *w = (const AdjWork *)h->adj;
-    if (h->tiles) {   // the tile kernel with per-cell dt (the same residual as the cell kernels)
+    if (h->tiles && h->shape == 0) {   // the tile kernel with per-cell dt (the same residual as the cell kernels)
         TileArgs<double> t = tile_args<double>(h);
         t.dtl = w->dtl;
         if (h->ph.equation == FVMR_EQ_NS)

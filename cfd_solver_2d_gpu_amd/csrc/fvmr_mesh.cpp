@@ -542,6 +542,8 @@ int fvmr_params_default(fvmr_params *p) {
     p->compat_inv2x2 = 0;
     p->kernels = FVMR_KERNELS_AUTO;
     p->tile_cells = 0;
+    p->tile_threads = 0;
+    p->reserved = 0;
     p->wall_temperature = 288.16;
     p->venkat_K = 5.0;
     p->dsr_shock_speed = 20.0;

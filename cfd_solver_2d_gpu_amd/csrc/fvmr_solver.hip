@@ -1046,12 +1046,15 @@ __global__ __launch_bounds__(kGroup3) FVMR_STAGE_ATTR void k_stage(Mesh<G> m, Ph
 // and the first ring in LDS, evaluates the flux of every face of the tile once, and sums
 // the fluxes per cell in slot order.  Gradients never go to HBM.
 // ---------------------------------------------------------------------------
-// Tile shape per arithmetic precision, sized so that the LDS of a work-group lets as many
-// work-groups share a CU as the registers allow (FP64: 116 VGPRs -> 4 waves per SIMD ->
-// 4 work-groups of 4 waves with <= 40 KB LDS each; FP32: 75 VGPRs -> 6 waves per SIMD ->
-// 3 work-groups of 8 waves with <= 53 KB each).
-template <class C> struct TileCfg;
-template <> struct TileCfg<double> {
+// Work-group shapes of the tile kernels per arithmetic precision.  The default (S = 0) is sized
+// so that the LDS of a work-group lets as many work-groups share a CU as the registers allow
+// (FP64: about 104 VGPRs -> 4 waves per SIMD -> 4 work-groups of 4 waves with <= 40 KB LDS each;
+// FP32: about 74 VGPRs -> 6 waves per SIMD -> 3 work-groups of 8 waves with <= 53 KB each).
+// S = 1 halves the work-group (smaller tiles, more work-groups per CU), S = 2 doubles it (FP64
+// only: the FP32 shape's local ids would not fit their 10-bit fields).  A handle uses one shape
+// (fvmr_params.tile_threads); the tuners pick among them.
+template <class C, int S = 0> struct TileCfg;
+template <> struct TileCfg<double, 0> {
     static constexpr int kThreads = 256;
     static constexpr int kMaxG = 200;    // cells with a gradient (tile + first ring)
     static constexpr int kMaxL = 256;    // cells with a state (+ second ring)
@@ -1060,7 +1063,17 @@ template <> struct TileCfg<double> {
     static constexpr int kMaxFF = kThreads;   // faces of tile cells: one per thread
     static constexpr int kCells = 150;   // default tile size
 };
-template <> struct TileCfg<float> {
+template <> struct TileCfg<double, 1> {
+    static constexpr int kThreads = 128;
+    static constexpr int kMaxG = 100, kMaxL = 128, kMaxN = 96, kMaxF = 176, kMaxFF = kThreads;
+    static constexpr int kCells = 72;
+};
+template <> struct TileCfg<double, 2> {
+    static constexpr int kThreads = 512;
+    static constexpr int kMaxG = 400, kMaxL = 512, kMaxN = 352, kMaxF = 704, kMaxFF = kThreads;
+    static constexpr int kCells = 300;
+};
+template <> struct TileCfg<float, 0> {
     static constexpr int kThreads = 512;
     static constexpr int kMaxG = 400;
     static constexpr int kMaxL = 480;
@@ -1069,9 +1082,37 @@ template <> struct TileCfg<float> {
     static constexpr int kMaxFF = kThreads;
     static constexpr int kCells = 300;
 };
-constexpr int kTileMaxCells = 400;   // largest tile_cells accepted (the FP32 shape's kMaxG)
-static_assert(TileCfg<float>::kMaxL <= fvmr::kLocalNone && TileCfg<float>::kMaxN <= fvmr::kLocalNone &&
-              TileCfg<float>::kMaxF <= fvmr::kLocalNone, "local ids are 10-bit fields");
+template <> struct TileCfg<float, 1> {
+    static constexpr int kThreads = 256;
+    static constexpr int kMaxG = 200, kMaxL = 240, kMaxN = 160, kMaxF = 336, kMaxFF = kThreads;
+    static constexpr int kCells = 150;
+};
+constexpr int kTileMaxCells = 400;   // largest tile_cells accepted (the largest shapes' kMaxG)
+constexpr int kNumShapes[2] = {3, 2};   // shapes per precision: FP64, FP32-mixed
+template <class Cfg> constexpr bool tile_cfg_fits() {
+    return Cfg::kMaxL <= fvmr::kLocalNone && Cfg::kMaxN <= fvmr::kLocalNone && Cfg::kMaxF <= fvmr::kLocalNone &&
+           Cfg::kMaxG <= kTileMaxCells && Cfg::kMaxL <= Cfg::kThreads && Cfg::kMaxN <= Cfg::kThreads &&
+           Cfg::kMaxF <= 2 * Cfg::kThreads && Cfg::kMaxG <= Cfg::kMaxL;
+}
+static_assert(tile_cfg_fits<TileCfg<double, 0>>() && tile_cfg_fits<TileCfg<double, 1>>() &&
+              tile_cfg_fits<TileCfg<double, 2>>() && tile_cfg_fits<TileCfg<float, 0>>() &&
+              tile_cfg_fits<TileCfg<float, 1>>(), "tile shapes: 10-bit local ids, one cell and two faces per thread");
+// threads of shape S of a precision (mixed: FP32)
+inline int shape_threads(bool mixed, int S) {
+    if (mixed) return S == 0 ? TileCfg<float, 0>::kThreads : TileCfg<float, 1>::kThreads;
+    return S == 0 ? TileCfg<double, 0>::kThreads : (S == 1 ? TileCfg<double, 1>::kThreads : TileCfg<double, 2>::kThreads);
+}
+inline int shape_cells(bool mixed, int S) {
+    if (mixed) return S == 0 ? TileCfg<float, 0>::kCells : TileCfg<float, 1>::kCells;
+    return S == 0 ? TileCfg<double, 0>::kCells : (S == 1 ? TileCfg<double, 1>::kCells : TileCfg<double, 2>::kCells);
+}
+// the shape of a precision with `threads` work-items (0: the default), -1 if none
+inline int shape_of_threads(bool mixed, int threads) {
+    if (threads == 0) return 0;
+    for (int S = 0; S < kNumShapes[mixed ? 1 : 0]; S++)
+        if (shape_threads(mixed, S) == threads) return S;
+    return -1;
+}
 
 // Per cell, for the time-step kernel: alphaC code, the 3 slot faces (local ids in the
 // cell's tile), volume.
@@ -1175,8 +1216,8 @@ inline bool tile_scheme_fixed(const Phys &ph) {
 }
 
 // One RK stage over the tiles (the work of k_gradient + k_stage, see there).
-template <class C, class G, bool NS, bool LDT = false, int SCHEME = kTileAny>
-__global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, Phys ph, const S4 *__restrict__ Uin,
+template <class C, class G, bool NS, bool LDT = false, int SCHEME = kTileAny, int SHAPE = 0>
+__global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs<G> ta, Phys ph, const S4 *__restrict__ Uin,
                                                         const S4 *__restrict__ Un, S4 *__restrict__ Uout,
                                                         S4 *__restrict__ rhs_out, Scalars *__restrict__ sc,
                                                         int *__restrict__ bad_list, int nc, double a, double b,
@@ -1187,7 +1228,7 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
         ph.scheme = FVMR_SOL_ROE;
         ph.reconstruct = FVMR_REC_LINEAR;
     }
-    using Cfg = TileCfg<C>;
+    using Cfg = TileCfg<C, SHAPE>;
     constexpr int kTileThreads = Cfg::kThreads, kTileMaxG = Cfg::kMaxG, kTileMaxL = Cfg::kMaxL;
     constexpr int kTileMaxN = Cfg::kMaxN, kTileMaxF = Cfg::kMaxF, kTileMaxFF = Cfg::kMaxFF;
     static_assert(kTileMaxL <= kTileThreads && kTileMaxFF <= kTileThreads, "one local cell and one face per thread");
@@ -1396,11 +1437,12 @@ __global__ __launch_bounds__(TileCfg<C>::kThreads) void k_tile(TileArgs<G> ta, P
 // LDT: the per-cell pseudo-time step of a steady solve instead, written to ta.dtl (the
 // consistent spectral radius (|u.n| + a) L of the face-mean primitives summed over the cell's
 // three faces, with k_local_dt's operation order).
-template <class G, bool NS, bool LDT = false>
-__global__ __launch_bounds__(TileCfg<G>::kThreads) void k_tile_dt(TileArgs<G> ta, Phys ph, const S4 *__restrict__ U,
-                                                           double *__restrict__ partial) {
+template <class G, bool NS, bool LDT = false, int SHAPE = 0>
+__global__ __launch_bounds__((TileCfg<G, SHAPE>::kThreads)) void k_tile_dt(TileArgs<G> ta, Phys ph,
+                                                                          const S4 *__restrict__ U,
+                                                                          double *__restrict__ partial) {
     if (!NS) ph.equation = FVMR_EQ_EULER;   // lets the compiler drop the viscous terms
-    using Cfg = TileCfg<G>;
+    using Cfg = TileCfg<G, SHAPE>;
     constexpr int kTileThreads = Cfg::kThreads, kTileMaxG = Cfg::kMaxG, kTileMaxN = Cfg::kMaxN;
     constexpr int kTileMaxFF = Cfg::kMaxFF;
     __shared__ S4 sU[kTileMaxG];
@@ -1724,9 +1766,10 @@ __global__ __launch_bounds__(kBlock) void k_from_caller(int nc, int base, const 
 
 // modifyElementFieldU2d_device_kernel (RestrictGPU.cu:6-84) with Jacobi reads: one work-group,
 // phase 1 computes all repairs from the unrepaired field, phase 2 writes them.
+// The body of k_restrict (every thread of the work-group calls it).
 template <class G>
-__global__ __launch_bounds__(1024) void k_restrict(Mesh<G> m, S4 *__restrict__ U, Scalars *__restrict__ sc,
-                                                   const int *__restrict__ bad_list, S4 *__restrict__ fix) {
+__device__ __forceinline__ void restrict_cells(const Mesh<G> &m, S4 *__restrict__ U, Scalars *__restrict__ sc,
+                                               const int *__restrict__ bad_list, S4 *__restrict__ fix) {
     const int nc = m.nc;
     const int count = min(sc->bad_count, nc);
     if (count == 0) return;   // the usual case: nothing flagged, nothing to write
@@ -1761,6 +1804,11 @@ __global__ __launch_bounds__(1024) void k_restrict(Mesh<G> m, S4 *__restrict__ U
     if (threadIdx.x == 0) {
         sc->bad_count = 0;
     }
+}
+template <class G>
+__global__ __launch_bounds__(1024) void k_restrict(Mesh<G> m, S4 *__restrict__ U, Scalars *__restrict__ sc,
+                                                   const int *__restrict__ bad_list, S4 *__restrict__ fix) {
+    restrict_cells<G>(m, U, sc, bad_list, fix);
 }
 
 // Minimum of the block partials of k_gradient<DT>; then the clamp to T - t and the t
@@ -2034,6 +2082,7 @@ struct fvmr_solver {
     // tile kernels (fvmr_tiles.h)
     bool tiles = false;
     int ntiles = 0;
+    int shape = 0;                        // work-group shape of the tile kernels (TileCfg<., shape>)
     int *t_info = nullptr, *t_halo = nullptr, *t_lnodes = nullptr;
     V2<double> *t_nxy = nullptr;
     int4 *t_gslot = nullptr, *t_face = nullptr;
@@ -2083,6 +2132,15 @@ TileArgs<G> tile_args(const fvmr_solver *h) {
     t.cbc = h->cbc;
     t.dtl = nullptr;
     return t;
+}
+
+template <class G, class F>
+int with_shape(const fvmr_solver *h, F &&f) {
+    if (h->shape == 1) return f(std::integral_constant<int, 1>{});
+    if constexpr (std::is_same<G, double>::value) {
+        if (h->shape == 2) return f(std::integral_constant<int, 2>{});
+    }
+    return f(std::integral_constant<int, 0>{});
 }
 
 int prof_begin(fvmr_solver *h, int k, hipEvent_t *out) {
@@ -2176,10 +2234,16 @@ int launch_tile_dt(fvmr_solver *h, const S4 *U, int dt_mode) {
     hipEvent_t e0{};
     int rc = prof_begin(h, 1, &e0);
     if (rc) return rc;
-    if (h->ph.equation == FVMR_EQ_NS)
-        k_tile_dt<G, true><<<h->ntiles, TileCfg<G>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, U, h->partial);
-    else
-        k_tile_dt<G, false><<<h->ntiles, TileCfg<G>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, U, h->partial);
+    with_shape<G>(h, [&](auto sh) {
+        constexpr int S = decltype(sh)::value;
+        if (h->ph.equation == FVMR_EQ_NS)
+            k_tile_dt<G, true, false, S><<<h->ntiles, TileCfg<G, S>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, U,
+                                                                                               h->partial);
+        else
+            k_tile_dt<G, false, false, S><<<h->ntiles, TileCfg<G, S>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, U,
+                                                                                                h->partial);
+        return FVMR_OK;
+    });
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 1, e0))) return rc;
     if ((rc = prof_begin(h, 2, &e0))) return rc;
@@ -2195,10 +2259,14 @@ int launch_tile_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, dou
     int rc = prof_begin(h, 0, &e0);
     if (rc) return rc;
     const bool ns = h->ph.equation == FVMR_EQ_NS, fixed = tile_scheme_fixed(h->ph);
-    auto *kern = ns ? (fixed ? k_tile<C, G, true, false, kTileGgBarthRoe> : k_tile<C, G, true>)
-                    : (fixed ? k_tile<C, G, false, false, kTileGgBarthRoe> : k_tile<C, G, false>);
-    kern<<<h->ntiles, TileCfg<C>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, Uin, Un, Uout, rhs_out, h->sc,
-                                                            h->bad_list, h->nc, a, b, first, last);
+    with_shape<C>(h, [&](auto sh) {
+        constexpr int S = decltype(sh)::value;
+        auto *kern = ns ? (fixed ? k_tile<C, G, true, false, kTileGgBarthRoe, S> : k_tile<C, G, true, false, kTileAny, S>)
+                        : (fixed ? k_tile<C, G, false, false, kTileGgBarthRoe, S> : k_tile<C, G, false, false, kTileAny, S>);
+        kern<<<h->ntiles, TileCfg<C, S>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, Uin, Un, Uout, rhs_out, h->sc,
+                                                                   h->bad_list, h->nc, a, b, first, last);
+        return FVMR_OK;
+    });
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 0, e0))) return rc;
     return last ? launch_restrict<G>(h, Uout) : FVMR_OK;
@@ -2351,6 +2419,10 @@ int validate_params(const fvmr_params *p) {
         return set_error(FVMR_ERR_INVALID_ARG, "invalid renumber");
     if (p->kernels < FVMR_KERNELS_AUTO || p->kernels > FVMR_KERNELS_TILE) return set_error(FVMR_ERR_INVALID_ARG, "invalid kernels");
     if (p->tile_cells < 0 || p->tile_cells > kTileMaxCells) return set_error(FVMR_ERR_INVALID_ARG, "tile_cells out of [0, 400]");
+    if (shape_of_threads(p->precision == FVMR_PREC_MIXED, p->tile_threads) < 0)
+        return set_error(FVMR_ERR_INVALID_ARG, p->precision == FVMR_PREC_MIXED
+                                                   ? "tile_threads must be 0, 512 or 256 in mixed precision"
+                                                   : "tile_threads must be 0, 256, 128 or 512 in FP64");
     return FVMR_OK;
 }
 
@@ -2583,7 +2655,11 @@ static int create_impl(const fvmr_mesh_soa *m, const fvmr_params *p, int ncases,
                 return fvmr::TileCaps{std::min(p->tile_cells ? p->tile_cells : Cfg::kCells, Cfg::kMaxG), Cfg::kMaxG,
                                       Cfg::kMaxL, Cfg::kMaxN, Cfg::kMaxF, Cfg::kMaxFF};
             };
-            const fvmr::TileCaps caps = h->mixed ? caps_of(TileCfg<float>{}) : caps_of(TileCfg<double>{});
+            h->shape = shape_of_threads(h->mixed, p->tile_threads);
+            fvmr::TileCaps caps;
+            if (h->mixed) caps = (h->shape == 1) ? caps_of(TileCfg<float, 1>{}) : caps_of(TileCfg<float, 0>{});
+            else caps = (h->shape == 1) ? caps_of(TileCfg<double, 1>{})
+                                        : ((h->shape == 2) ? caps_of(TileCfg<double, 2>{}) : caps_of(TileCfg<double, 0>{}));
             if ((rc = fvmr::build_tiles(ti, caps, tm))) return bail(rc);
             h->tiles = true;
             h->ntiles = tm.ntiles;
@@ -2702,7 +2778,8 @@ static bool same_shared_params(const fvmr_params &a, const fvmr_params &b) {
            a.reconstruct == b.reconstruct && a.gradient == b.gradient && a.limiter == b.limiter &&
            a.time_advance == b.time_advance && a.viscosity_model == b.viscosity_model && a.precision == b.precision &&
            a.renumber == b.renumber && a.compat_inv2x2 == b.compat_inv2x2 && a.kernels == b.kernels &&
-           a.tile_cells == b.tile_cells && a.wall_temperature == b.wall_temperature && a.venkat_K == b.venkat_K &&
+           a.tile_cells == b.tile_cells && a.tile_threads == b.tile_threads && a.wall_temperature == b.wall_temperature &&
+           a.venkat_K == b.venkat_K &&
            a.dsr_shock_x == b.dsr_shock_x && a.dsr_shock_y == b.dsr_shock_y && a.dsr_shock_speed == b.dsr_shock_speed;
 }
 
@@ -3205,26 +3282,31 @@ int fvmr_surface_force_noalloc(fvmr_handle *h, int nsets, const int *set_offsets
 
 }  // extern "C"
 
-// Tile-size candidates of a precision: 0.6 - 1.15 x the shape's default, within its LDS capacity.
-static std::vector<int> tile_candidates(bool mixed) {
-    const int dflt = mixed ? TileCfg<float>::kCells : TileCfg<double>::kCells;
-    const int cap = mixed ? TileCfg<float>::kMaxG : TileCfg<double>::kMaxG;
-    std::vector<int> cand;
-    for (double f : {0.6, 0.75, 0.9, 1.0, 1.15}) {
-        const int c = std::min(cap, std::max(8, (int)std::lround(f * dflt)));
-        if (std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+// Candidates of the tuners for a precision: every work-group shape, each with its default tile
+// size and 0.8 and 1.15 of it (within the shape's LDS capacity).
+struct TileCand {
+    int threads, cells;
+};
+static std::vector<TileCand> tile_candidates(bool mixed) {
+    std::vector<TileCand> cand;
+    for (int S = 0; S < kNumShapes[mixed ? 1 : 0]; S++) {
+        const int dflt = shape_cells(mixed, S), th = shape_threads(mixed, S);
+        for (double f : {1.0, 0.8, 1.15}) {
+            const int c = std::min(kTileMaxCells, std::max(8, (int)std::lround(f * dflt)));
+            cand.push_back(TileCand{th, c});
+        }
     }
     return cand;
 }
 
 // Times `steps` steps of the uniform freestream of each case on a handle made by `make` for
-// every candidate tile size (after 2 untimed steps) and returns the fastest; timings: the ms per
-// step of every candidate, in candidate order (may be NULL).
+// every candidate (after 2 untimed steps) and returns the fastest; timings: the ms per step of
+// every candidate, in candidate order (may be NULL).
 template <class Make>
 static int autotune_tiles(int ncases, const fvmr_mesh_soa *const *meshes, const fvmr_params *params, int steps,
-                          Make make, int *best_tile_cells, double *best_ms, double *timings) {
+                          Make make, int *best_tile_threads, int *best_tile_cells, double *best_ms, double *timings) {
     const bool mixed = params[0].precision == FVMR_PREC_MIXED;
-    const std::vector<int> cand = tile_candidates(mixed);
+    const std::vector<TileCand> cand = tile_candidates(mixed);
     // the cases' uniform freestream states
     std::vector<std::vector<double>> U0(4 * (size_t)ncases);
     for (int c = 0; c < ncases; c++) {
@@ -3237,7 +3319,7 @@ static int autotune_tiles(int ncases, const fvmr_mesh_soa *const *meshes, const 
         if (rc) return rc;
     }
     double best = 1e300;
-    int best_c = mixed ? TileCfg<float>::kCells : TileCfg<double>::kCells;
+    TileCand best_c = cand[0];
     for (size_t q = 0; q < cand.size(); q++) {
         fvmr_handle *h = nullptr;
         int rc = make(cand[q], &h);
@@ -3264,45 +3346,47 @@ static int autotune_tiles(int ncases, const fvmr_mesh_soa *const *meshes, const 
         if (timings) timings[q] = ms / steps;
         if (ms < best) { best = ms; best_c = cand[q]; }
     }
-    *best_tile_cells = best_c;
+    if (best_tile_threads) *best_tile_threads = best_c.threads;
+    *best_tile_cells = best_c.cells;
     if (best_ms) *best_ms = best / steps;
     return FVMR_OK;
 }
 
 extern "C" {
 
-// Times `steps` SSP-RK3 / Euler steps of the mesh's uniform freestream for each candidate tile
-// size (0.6 - 1.15 x the shape's default, clipped to the LDS capacity) and returns the fastest.
-int fvmr_autotune(const fvmr_mesh_soa *m, const fvmr_params *p, int steps, int *best_tile_cells, double *best_ms) {
+int fvmr_autotune(const fvmr_mesh_soa *m, const fvmr_params *p, int steps, int *best_tile_threads,
+                  int *best_tile_cells, double *best_ms) {
     if (!m || !p || !best_tile_cells || steps < 1) return set_error(FVMR_ERR_INVALID_ARG, "invalid argument");
     const fvmr_mesh_soa *ms[1] = {m};
-    return autotune_tiles(1, ms, p, steps, [&](int tc, fvmr_handle **h) {
+    return autotune_tiles(1, ms, p, steps, [&](TileCand tc, fvmr_handle **h) {
         fvmr_params q = *p;
-        q.tile_cells = tc;
+        q.tile_threads = tc.threads;
+        q.tile_cells = tc.cells;
         q.kernels = FVMR_KERNELS_TILE;
         return fvmr_create(m, &q, h);
-    }, best_tile_cells, best_ms, nullptr);
+    }, best_tile_threads, best_tile_cells, best_ms, nullptr);
 }
 
-// The same over the cases of a batch (one batch handle per candidate).
 int fvmr_autotune_batch(int ncases, const fvmr_mesh_soa *const *meshes, const fvmr_params *params, int steps,
-                        int *best_tile_cells, double *best_ms, double *timings) {
+                        int *best_tile_threads, int *best_tile_cells, double *best_ms, double *timings) {
     if (ncases < 1 || !meshes || !params || !best_tile_cells || steps < 1)
         return set_error(FVMR_ERR_INVALID_ARG, "invalid argument");
     std::vector<fvmr_params> q(params, params + ncases);
-    return autotune_tiles(ncases, meshes, params, steps, [&](int tc, fvmr_handle **h) {
-        for (auto &x : q) { x.tile_cells = tc; x.kernels = FVMR_KERNELS_TILE; }
+    return autotune_tiles(ncases, meshes, params, steps, [&](TileCand tc, fvmr_handle **h) {
+        for (auto &x : q) { x.tile_threads = tc.threads; x.tile_cells = tc.cells; x.kernels = FVMR_KERNELS_TILE; }
         return fvmr_create_batch(ncases, meshes, q.data(), h);
-    }, best_tile_cells, best_ms, timings);
+    }, best_tile_threads, best_tile_cells, best_ms, timings);
 }
 
-// Candidate tile sizes of fvmr_autotune / fvmr_autotune_batch for a precision.
-int fvmr_autotune_candidates(int precision, int *n, int *tile_cells) {
+int fvmr_autotune_candidates(int precision, int *n, int *tile_threads, int *tile_cells) {
     if (!n || (precision != FVMR_PREC_FP64 && precision != FVMR_PREC_MIXED))
         return set_error(FVMR_ERR_INVALID_ARG, "invalid argument");
-    const std::vector<int> cand = tile_candidates(precision == FVMR_PREC_MIXED);
+    const std::vector<TileCand> cand = tile_candidates(precision == FVMR_PREC_MIXED);
     *n = (int)cand.size();
-    if (tile_cells) std::copy(cand.begin(), cand.end(), tile_cells);
+    for (size_t k = 0; k < cand.size(); k++) {
+        if (tile_threads) tile_threads[k] = cand[k].threads;
+        if (tile_cells) tile_cells[k] = cand[k].cells;
+    }
     return FVMR_OK;
 }
 

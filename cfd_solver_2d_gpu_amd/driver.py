@@ -163,6 +163,7 @@ class Config:
         self.renumber = F.RENUMBER_RCM
         self.kernels = F.KERNELS_AUTO
         self.tile_cells = 0        # 0: library default, -1: autotune at start
+        self.tile_threads = 0      # work-group size of the tile kernels, 0: library default
         self.use_graphs = False
         self.venkat_K = None
         self.wall_temperature = None
@@ -173,7 +174,7 @@ class Config:
                   equation=self.equation, flux_scheme=self.flux_conservation_scheme,
                   reconstruct=self.flag_reconstruct, gradient=self.flag_gradient, limiter=self.flux_limiter,
                   time_advance=self.time_advance, precision=self.precision, renumber=self.renumber,
-                  kernels=self.kernels, tile_cells=max(0, self.tile_cells),
+                  kernels=self.kernels, tile_cells=max(0, self.tile_cells), tile_threads=self.tile_threads,
                   inf_ruvp=self.inf_ruvp, inlet_ruvp=self.inlet_ruvp, outlet_ruvp=self.outlet_ruvp,
                   dsr_shock_x=self.dsr[0], dsr_shock_y=self.dsr[1])
         if self.venkat_K is not None:
@@ -359,6 +360,7 @@ def config_from_tree(tree):
         c.tile_cells = -1
     else:
         c.tile_cells = g("device.tile_cells", int, c.tile_cells)
+    c.tile_threads = g("device.tile_threads", int, c.tile_threads)
     c.use_graphs = g("device.use_graphs", bool, c.use_graphs)
     c.venkat_K = g("device.venkat_K", float, None)
     c.wall_temperature = g("device.wall_temperature", float, None)
@@ -583,7 +585,7 @@ class Driver:
         c = self.cfg
         self.read_field()
         if c.tile_cells == -1:
-            self.params.tile_cells, _ = autotune(self.mesh, self.params)
+            self.params.tile_threads, self.params.tile_cells, _ = autotune(self.mesh, self.params)
         a = self.mesh.arrays()
         self.node_xy, self.element_nodes, self.edge_nodes = a["node_xy"], a["element_nodes"], a["edge_nodes"]
         self.sets = self.mesh.boundary_sets()
@@ -694,7 +696,8 @@ def run_case(config_path, max_steps=None, log=print):
 # scheme settings every case of a batch shares (fvmr_create_batch: one kernel configuration)
 _BATCH_SHARED = ("gamma", "R", "Pr", "Pr_viscous", "T0", "mu0", "equation", "flux_scheme", "reconstruct", "gradient",
                  "limiter", "time_advance", "viscosity_model", "precision", "renumber", "compat_inv2x2", "kernels",
-                 "tile_cells", "wall_temperature", "venkat_K", "dsr_shock_x", "dsr_shock_y", "dsr_shock_speed")
+                 "tile_cells", "tile_threads", "wall_temperature", "venkat_K", "dsr_shock_x", "dsr_shock_y",
+                 "dsr_shock_speed")
 
 
 def run_batch(config_paths, log=print):

@@ -149,8 +149,12 @@ typedef struct fvmr_params {
     int    renumber;         /* device cell order: FVMR_RENUMBER_* */
     int    compat_inv2x2;    /* 1: reproduce the reference inv_2x2 determinant bug (LSQ) */
     int    kernels;          /* FVMR_KERNELS_*: device kernel family */
-    int    tile_cells;       /* cells per tile of the tile kernels (0: default, 150 in FP64 and 300 in
-                                mixed precision; at most 400, capped by the kernel's LDS shape) */
+    int    tile_cells;       /* cells per tile of the tile kernels (0: the work-group shape's default, 150
+                                in FP64 and 300 in mixed precision for the default shapes; at most 400,
+                                capped by the shape's LDS) */
+    int    tile_threads;     /* work-items per work-group of the tile kernels (0: the default, 256 in FP64
+                                and 512 in mixed precision; 128 or 512 in FP64, 256 in mixed precision) */
+    int    reserved;         /* 0 */
     double inf_ruvp[4];      /* boundaryCondition.2D.inf    (rho,u,v,p) */
     double inlet_ruvp[4];    /* boundaryCondition.2D.inlet  */
     double outlet_ruvp[4];   /* boundaryCondition.2D.outlet */
@@ -379,17 +383,19 @@ int fvmr_adjoint_state(fvmr_handle *h, double *const psi[4]);
  * (default off; never while profiling).  No reference counterpart: the reference launches
  * every kernel from the host each step. */
 int fvmr_use_graphs(fvmr_handle *h, int on);
-/* Tile-size autotuner (no reference counterpart): times `steps` steps of the uniform freestream
- * of p on mesh m for several tile sizes and returns the fastest (set it as p->tile_cells) and
- * its ms per step (best_ms may be NULL). */
-int fvmr_autotune(const fvmr_mesh_soa *m, const fvmr_params *p, int steps, int *best_tile_cells, double *best_ms);
-/* The same for a batch (fvmr_create_batch's arguments): one batch handle per candidate tile
- * size; timings (may be NULL) receives the ms per step of every candidate, in the order of
- * fvmr_autotune_candidates. */
+/* Autotuner of the tile kernels (no reference counterpart): times `steps` steps of the uniform
+ * freestream of p on mesh m for every candidate work-group shape and tile size (fvmr_autotune_
+ * candidates) and returns the fastest pair (set them as p->tile_threads, p->tile_cells) and its
+ * ms per step (best_ms may be NULL). */
+int fvmr_autotune(const fvmr_mesh_soa *m, const fvmr_params *p, int steps, int *best_tile_threads,
+                  int *best_tile_cells, double *best_ms);
+/* The same for a batch (fvmr_create_batch's arguments): one batch handle per candidate; timings
+ * (may be NULL) receives the ms per step of every candidate, in fvmr_autotune_candidates' order. */
 int fvmr_autotune_batch(int ncases, const fvmr_mesh_soa *const *meshes, const fvmr_params *params, int steps,
-                        int *best_tile_cells, double *best_ms, double *timings);
-/* The candidate tile sizes of the tuners for a precision (FVMR_PREC_*): *n of them (at most 8). */
-int fvmr_autotune_candidates(int precision, int *n, int *tile_cells);
+                        int *best_tile_threads, int *best_tile_cells, double *best_ms, double *timings);
+/* The tuners' candidates for a precision (FVMR_PREC_*): *n pairs (at most 16) of work-group size
+ * and tile size. */
+int fvmr_autotune_candidates(int precision, int *n, int *tile_threads, int *tile_cells);
 /* 1: record HIP events around every kernel launch on the handle's stream. */
 int fvmr_profile_enable(fvmr_handle *h, int on);
 /* Average ms per launch and launch count of a kernel class since the last reset:

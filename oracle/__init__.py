@@ -99,8 +99,7 @@ def _load(path):
 def lib():
     """The plain build (one thread), which the tests use."""
     if "plain" not in _libs:
-        if not os.path.exists(LIB_PATH):
-            build()
+        build()   # make: rebuilds only when the source or the shared header is newer
         _libs["plain"] = _load(LIB_PATH)
     return _libs["plain"]
 
@@ -109,8 +108,7 @@ def fast_lib(threads=None):
     """The OpenMP build for this host (built on first use); threads: OpenMP threads to use."""
     if "fast" not in _libs:
         path = fast_lib_path()
-        if not os.path.exists(path):
-            build_fast()
+        build_fast()
         _libs["fast"] = _load(path)
     if threads is not None:
         _libs["fast"].orc_set_threads(int(threads))

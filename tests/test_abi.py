@@ -29,8 +29,8 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_struct_sizes_match_header_layout():
-    # fvmr_params: 7 doubles, 12 ints, 3x4 doubles, 5 doubles
-    assert C.sizeof(F.Params) == 7 * 8 + 12 * 4 + 12 * 8 + 5 * 8
+    # fvmr_params: 7 doubles, 14 ints, 3x4 doubles, 5 doubles
+    assert C.sizeof(F.Params) == 7 * 8 + 14 * 4 + 12 * 8 + 5 * 8
     assert C.sizeof(F.StepInfo) == 6 * 8 + 4 * 4
     assert C.sizeof(F.MeshSoA) == 16 + 8 * (2 + 4 * 3 + 2 + 1 + 2 + 5 + 2 + 2 + 1)
 

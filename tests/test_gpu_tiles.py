@@ -105,12 +105,38 @@ def test_nonstandard_geometry_falls_back_to_cells():
 
 
 def test_autotune_returns_a_working_tile_size():
-    announce("autotune: wedge 128x64 Roe, FP64 and mixed")
+    announce("autotune: wedge 128x64 Roe, FP64 and mixed, work-group shapes and tile sizes")
     for prec in (F.PREC_FP64, F.PREC_MIXED):
         m, p, U = wedge_case(128, 64, scheme=F.SOL_ROE)
         p.precision = prec
-        best, ms = F.autotune(m, p, steps=5)
-        assert 8 <= best <= 400 and ms > 0
+        cand = F.autotune_candidates(prec)
+        assert len({th for th, _ in cand}) == (3 if prec == F.PREC_FP64 else 2)
+        th, best, ms = F.autotune(m, p, steps=5)
+        assert (th, best) in cand and ms > 0
         Ud, _ = run(m, p, U, 3, F.KERNELS_TILE)
+        p.tile_threads = th
         Ub, _ = run(m, p, U, 3, F.KERNELS_TILE, tile_cells=best)
         assert rel_l2(Ub, Ud) < (1e-13 if prec == F.PREC_FP64 else 1e-6)
+
+
+@pytest.mark.parametrize("precision,threads", [(F.PREC_FP64, 128), (F.PREC_FP64, 512), (F.PREC_MIXED, 256)])
+def test_work_group_shapes(precision, threads):
+    """Every work-group shape of the tile kernels gives the default shape's results (FP64 to
+    1e-13, the same arithmetic in other tiles; mixed to 1e-6) and the oracle's to 1e-12 (FP64)."""
+    announce(f"tile work-group shape: {threads} threads, precision={precision}")
+    m, p, U = wedge_case(96, 48, scheme=F.SOL_ROE)
+    p.precision = precision
+    Ud, _ = run(m, p, U, 4, F.KERNELS_TILE)
+    p.tile_threads = threads
+    Us, _ = run(m, p, U, 4, F.KERNELS_TILE)
+    assert rel_l2(Us, Ud) < (1e-13 if precision == F.PREC_FP64 else 1e-6)
+    if precision == F.PREC_FP64:
+        o = O.OracleSolver(m, p)
+        o.set_state(U)
+        t = 0.0
+        for _ in range(4):
+            t, _ = o.step(t, 1e9)
+        assert rel_l2(Us, o.get_state()) < 1e-12
+    with pytest.raises(F.FvmrError):
+        p.tile_threads = 1024
+        F.GPUSolver2(m, p)
