@@ -1122,6 +1122,34 @@ struct DtRec {
     double vol;
 };
 
+// The repair of a flagged cell c in modifyElementFieldU2d_device_kernel (RestrictGPU.cu:6-84):
+// the volume-weighted mean of its element neighbours' states (periodic neighbours excluded), with
+// the volume sum taken once per variable (the reference's quirk: the mean is a quarter of it), or
+// its own state when it has no neighbour.  No contraction, so that the restriction kernel and the
+// time-step kernel's deferred repair form the same bits.
+template <class G>
+__device__ __forceinline__ void repaired_state(const int2 *__restrict__ cslot, const V4<G> *__restrict__ cgeo,
+                                               const S4 *__restrict__ U, int nc, int c, double res[4]) {
+#pragma clang fp contract(off)
+    int nvalid = 0;
+    double Usum[4] = {0, 0, 0, 0}, vs = 0.0;
+    for (int j = 0; j < 3; j++) {
+        const int2 sl = cslot[j * nc + c];
+        const int nb = (sl.x & kSlotPeriodic) ? -1 : sl.y;   // periodic neighbours are not element neighbours
+        if (nb == -1) continue;
+        nvalid++;
+        const double v = (double)cgeo[nb].z;
+        double Unb[4];
+        ld(U, nb, Unb);
+        for (int k = 0; k < 4; k++) {
+            Usum[k] += Unb[k] * v;
+            vs += v;
+        }
+    }
+    if (nvalid == 0) ld(U, c, res);
+    else for (int k = 0; k < 4; k++) res[k] = Usum[k] / vs;
+}
+
 template <class G>
 struct TileArgs {
     const int *info;          // fvmr::kTileInfo ints per tile
@@ -1136,6 +1164,13 @@ struct TileArgs {
     const V2<G> *fnrm;        // per face: normal (FVMR_TILE_GEOM_LOADED)
     const CaseBC *cbc;        // per case of the batch (one case: the handle's)
     double *dtl;              // per-cell steps of a steady solve (k_tile_dt / k_tile with LDT), else NULL
+    // the deferred restriction of an SSP-RK3 step's flagged cells (k_tile_dt repairs the states it
+    // stages, k_dt_finish_cases writes them)
+    const int2 *cslot;        // [3][nc] face slots (Mesh::cslot)
+    const V4<G> *cgeo;        // per cell: centroid, volume (Mesh::cgeo)
+    int *badidx;              // per cell: its entry in the list of flagged cells, -1 none
+    S4 *fix;                  // per entry of that list: the repaired state
+    int nc;
 };
 
 __device__ __forceinline__ int lfield(int w, int k) { return (w >> (fvmr::kLocalBits * k)) & fvmr::kLocalMask; }
@@ -1221,7 +1256,7 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
                                                         const S4 *__restrict__ Un, S4 *__restrict__ Uout,
                                                         S4 *__restrict__ rhs_out, Scalars *__restrict__ sc,
                                                         int *__restrict__ bad_list, int nc, double a, double b,
-                                                        int first, int last) {
+                                                        int first, int last, int reset_bad) {
     if (SCHEME == kTileGgBarthRoe) {
         ph.gradient = FVMR_GRA_GREENGAUSS;
         ph.limiter = FVMR_LIM_BARTH;
@@ -1244,6 +1279,10 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
     Grad<G> *sGr = reinterpret_cast<Grad<G> *>(sGrFl);
     V4<C> *sFl = reinterpret_cast<V4<C> *>(sGrFl);
     const int tid = threadIdx.x;
+    // reset_bad (the first stage of an SSP-RK3 step): the previous step's flagged cells are
+    // repaired by now (k_tile_dt, k_dt_finish_cases); the list starts empty for this step's last
+    // stage
+    if (reset_bad && blockIdx.x == 0 && tid == 0) sc->bad_count = 0;
     const int *ti = ta.info + (size_t)xcd_block() * fvmr::kTileInfo;
     const int c0 = ti[fvmr::kTiC0], nT = ti[fvmr::kTiNT], nG = ti[fvmr::kTiNG], nL = ti[fvmr::kTiNL];
     const int nN = ti[fvmr::kTiNN], nF = ti[fvmr::kTiNF], nFF = ti[fvmr::kTiNFF];
@@ -1426,6 +1465,7 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
             if (slot < nc) {
                 bad_list[slot] = c;
                 bad_list[nc + slot] = cs;
+                ta.badidx[c] = slot;
             }
         }
     }
@@ -1440,7 +1480,8 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
 template <class G, bool NS, bool LDT = false, int SHAPE = 0>
 __global__ __launch_bounds__((TileCfg<G, SHAPE>::kThreads)) void k_tile_dt(TileArgs<G> ta, Phys ph,
                                                                           const S4 *__restrict__ U,
-                                                                          double *__restrict__ partial) {
+                                                                          double *__restrict__ partial,
+                                                                          const Scalars *__restrict__ sc) {
     if (!NS) ph.equation = FVMR_EQ_EULER;   // lets the compiler drop the viscous terms
     using Cfg = TileCfg<G, SHAPE>;
     constexpr int kTileThreads = Cfg::kThreads, kTileMaxG = Cfg::kMaxG, kTileMaxN = Cfg::kMaxN;
@@ -1463,7 +1504,20 @@ __global__ __launch_bounds__((TileCfg<G, SHAPE>::kThreads)) void k_tile_dt(TileA
     const V2<double> nd = (tid < nN) ? ta.nxy[offN + tid] : V2<double>{0.0, 0.0};
     const int4 fr = (tid < nFF) ? ta.face[offF + tid] : make_int4(0, 0, 0, 0);
     const int src = (tid < nG) ? (tid < nT ? c0 + tid : ta.halo[offH + tid - nT]) : -1;
-    const S4 u = (src >= 0) ? U[src] : S4{0.0, 0.0, 0.0, 0.0};
+    S4 u = (src >= 0) ? U[src] : S4{0.0, 0.0, 0.0, 0.0};
+    // cells the previous step's last stage flagged (rare): each work-group repairs the states it
+    // stages from the unrepaired field (nothing writes U during this kernel, so every copy of a
+    // repair has the same bits) and the tile that owns the cell leaves its repair for
+    // k_dt_finish_cases to write
+    if (!LDT && sc && sc->bad_count > 0 && src >= 0) {
+        const int bi = ta.badidx[src];
+        if (bi >= 0) {
+            double r[4];
+            repaired_state<G>(ta.cslot, ta.cgeo, U, ta.nc, src, r);
+            u = S4{r[0], r[1], r[2], r[3]};
+            if (tid < nT) ta.fix[bi] = u;
+        }
+    }
     if (tid < nN) sN[tid] = nd;
     if (tid < nFF) sFace[tid] = fr;
     if (tid < nG) sU[tid] = u;
@@ -1765,39 +1819,30 @@ __global__ __launch_bounds__(kBlock) void k_from_caller(int nc, int base, const 
 }
 
 // modifyElementFieldU2d_device_kernel (RestrictGPU.cu:6-84) with Jacobi reads: one work-group,
-// phase 1 computes all repairs from the unrepaired field, phase 2 writes them.
+// phase 1 computes all repairs from the unrepaired field, phase 2 writes them and clears the
+// cells' entries in badidx.  reset_counts: every case's count of restricted cells starts from 0
+// (the restriction that ends an fvmr_advance call, whose steps deferred theirs).
 // The body of k_restrict (every thread of the work-group calls it).
 template <class G>
 __device__ __forceinline__ void restrict_cells(const Mesh<G> &m, S4 *__restrict__ U, Scalars *__restrict__ sc,
-                                               const int *__restrict__ bad_list, S4 *__restrict__ fix) {
+                                               const int *__restrict__ bad_list, S4 *__restrict__ fix,
+                                               int *__restrict__ badidx, int reset_counts, int ncases) {
     const int nc = m.nc;
+    if (reset_counts) {
+        for (int c = threadIdx.x; c < ncases; c += blockDim.x) sc[c].n_restricted = 0;
+        __syncthreads();
+    }
     const int count = min(sc->bad_count, nc);
     if (count == 0) return;   // the usual case: nothing flagged, nothing to write
     for (int i = threadIdx.x; i < count; i += blockDim.x) {
-        const int c = bad_list[i];
-        int nvalid = 0;
-        double Usum[4] = {0, 0, 0, 0}, vs = 0.0;
-        for (int j = 0; j < 3; j++) {
-            const int2 sl = m.cslot[j * nc + c];
-            const int nb = (sl.x & kSlotPeriodic) ? -1 : sl.y;   // periodic neighbours are not element neighbours
-            if (nb == -1) continue;
-            nvalid++;
-            const double v = (double)m.cgeo[nb].z;
-            double Unb[4];
-            ld(U, nb, Unb);
-            for (int k = 0; k < 4; k++) {
-                Usum[k] += Unb[k] * v;
-                vs += v;
-            }
-        }
         double res[4];
-        if (nvalid == 0) ld(U, c, res);
-        else for (int k = 0; k < 4; k++) res[k] = Usum[k] / vs;
+        repaired_state<G>(m.cslot, m.cgeo, U, nc, bad_list[i], res);
         st(fix, i, res);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < count; i += blockDim.x) {
         U[bad_list[i]] = fix[i];
+        badidx[bad_list[i]] = -1;
         atomicAdd(&sc[bad_list[nc + i]].n_restricted, 1);   // counted per case of a batch
     }
     __syncthreads();
@@ -1807,8 +1852,9 @@ __device__ __forceinline__ void restrict_cells(const Mesh<G> &m, S4 *__restrict_
 }
 template <class G>
 __global__ __launch_bounds__(1024) void k_restrict(Mesh<G> m, S4 *__restrict__ U, Scalars *__restrict__ sc,
-                                                   const int *__restrict__ bad_list, S4 *__restrict__ fix) {
-    restrict_cells<G>(m, U, sc, bad_list, fix);
+                                                   const int *__restrict__ bad_list, S4 *__restrict__ fix,
+                                                   int *__restrict__ badidx, int reset_counts, int ncases) {
+    restrict_cells<G>(m, U, sc, bad_list, fix, badidx, reset_counts, ncases);
 }
 
 // Minimum of the block partials of k_gradient<DT>; then the clamp to T - t and the t
@@ -1844,10 +1890,17 @@ __global__ __launch_bounds__(1024) void k_dt_finish(int nparts, const double *__
 }
 
 // k_dt_finish per case of a batch: block c takes the minimum over the partials of its tiles
-// tile_begin[c] .. tile_begin[c+1]-1 and clamps / advances the times of sc[c].
+// tile_begin[c] .. tile_begin[c+1]-1 and clamps / advances the times of sc[c].  apply (SSP-RK3
+// steps of the tile kernels, whose restriction is deferred to the next step): then it writes the
+// repairs k_tile_dt left in fix for the cells of case c that the previous step's last stage
+// flagged, clears their badidx entries and counts them (the next step's first stage empties the
+// list).
 __global__ __launch_bounds__(1024) void k_dt_finish_cases(const int *__restrict__ tile_begin,
                                                           const double *__restrict__ partial,
-                                                          Scalars *__restrict__ sc, int advance) {
+                                                          Scalars *__restrict__ sc, int advance, int apply,
+                                                          const int *__restrict__ bad_list,
+                                                          const S4 *__restrict__ fix, int *__restrict__ badidx,
+                                                          S4 *__restrict__ U, int nc) {
     __shared__ double s[16];
     const int c = blockIdx.x;
     const int beg = tile_begin[c], end = tile_begin[c + 1];
@@ -1876,6 +1929,18 @@ __global__ __launch_bounds__(1024) void k_dt_finish_cases(const int *__restrict_
             sc[c].n_restricted = 0;
         }
         sc[c].dt = d;
+    }
+    if (apply) {
+        const int count = min(sc[0].bad_count, nc);
+        if (count == 0) return;
+        __syncthreads();   // n_restricted cleared above
+        for (int i = threadIdx.x; i < count; i += blockDim.x) {
+            if (bad_list[nc + i] != c) continue;
+            const int cell = bad_list[i];
+            U[cell] = fix[i];
+            badidx[cell] = -1;
+            atomicAdd(&sc[c].n_restricted, 1);
+        }
     }
 }
 
@@ -2076,6 +2141,7 @@ struct fvmr_solver {
     std::vector<int> case_cell_begin;
     CaseBC *cbc = nullptr;
     int *tile_case_begin = nullptr;       // tiles of case c: [c] .. [c+1]-1
+    int *badidx = nullptr;                // per cell: its entry in bad_list, -1 none
     int *perm_dev = nullptr;              // perm on the device (state transfers, first use)
     double *io_soa = nullptr;             // 4 x nc doubles in the caller's order (state transfers)
     bool io_ready = false;                // perm_dev holds perm (set by ensure_io)
@@ -2131,6 +2197,11 @@ TileArgs<G> tile_args(const fvmr_solver *h) {
     t.lcxy = (const V2<G> *)h->t_lcxy; t.fnrm = (const V2<G> *)h->t_fnrm;
     t.cbc = h->cbc;
     t.dtl = nullptr;
+    t.cslot = h->cslot;
+    t.cgeo = (const V4<G> *)h->cgeo;
+    t.badidx = h->badidx;
+    t.fix = h->fix;
+    t.nc = h->nc;
     return t;
 }
 
@@ -2200,13 +2271,16 @@ int launch_gradient(fvmr_solver *h, const S4 *U, int dt_mode) {
 }
 
 // The restriction after the step's last stage (CIteration.cpp:79 and :152 restrict once per
-// iteration, after the evolve).
+// iteration, after the evolve).  SSP-RK3 steps of the tile kernels defer it to the next step's
+// time-step kernels (defer_restriction) and the call that runs them restricts once after its last
+// step (reset_counts: the cases' counts then hold that step's repairs alone).
 template <class G>
-int launch_restrict(fvmr_solver *h, S4 *U) {
+int launch_restrict(fvmr_solver *h, S4 *U, int reset_counts = 0) {
     hipEvent_t e0{};
     int rc = prof_begin(h, 3, &e0);
     if (rc) return rc;
-    k_restrict<G><<<1, 1024, 0, h->stream>>>(mesh_args<G>(h), U, h->sc, h->bad_list, h->fix);
+    k_restrict<G><<<1, 1024, 0, h->stream>>>(mesh_args<G>(h), U, h->sc, h->bad_list, h->fix, h->badidx,
+                                             reset_counts, h->ncases);
     HIP_OK(hipGetLastError());
     return prof_end(h, 3, e0);
 }
@@ -2228,9 +2302,12 @@ int launch_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, double a
     return last ? launch_restrict<G>(h, Uout) : FVMR_OK;
 }
 
-// Global dt over the tiles (dt_mode as launch_gradient).
+bool defer_restriction(const fvmr_solver *h) { return h->tiles && h->p.time_advance == FVMR_EVO_RK3; }
+
+// Global dt over the tiles (dt_mode as launch_gradient); apply: the finish writes the repairs of
+// the cells the previous step flagged (deferred restriction).
 template <class C, class G>
-int launch_tile_dt(fvmr_solver *h, const S4 *U, int dt_mode) {
+int launch_tile_dt(fvmr_solver *h, const S4 *U, int dt_mode, int apply) {
     hipEvent_t e0{};
     int rc = prof_begin(h, 1, &e0);
     if (rc) return rc;
@@ -2238,23 +2315,26 @@ int launch_tile_dt(fvmr_solver *h, const S4 *U, int dt_mode) {
         constexpr int S = decltype(sh)::value;
         if (h->ph.equation == FVMR_EQ_NS)
             k_tile_dt<G, true, false, S><<<h->ntiles, TileCfg<G, S>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, U,
-                                                                                               h->partial);
+                                                                                               h->partial, h->sc);
         else
             k_tile_dt<G, false, false, S><<<h->ntiles, TileCfg<G, S>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, U,
-                                                                                                h->partial);
+                                                                                                h->partial, h->sc);
         return FVMR_OK;
     });
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 1, e0))) return rc;
     if ((rc = prof_begin(h, 2, &e0))) return rc;
-    k_dt_finish_cases<<<h->ncases, 1024, 0, h->stream>>>(h->tile_case_begin, h->partial, h->sc, dt_mode);
+    k_dt_finish_cases<<<h->ncases, 1024, 0, h->stream>>>(h->tile_case_begin, h->partial, h->sc, dt_mode, apply,
+                                                         h->bad_list, h->fix, h->badidx, const_cast<S4 *>(U), h->nc);
     HIP_OK(hipGetLastError());
     return prof_end(h, 2, e0);
 }
 
+// restrict_after: the restriction follows a last stage (not when deferred); reset_bad: the
+// first stage of a step whose restriction was deferred empties the list of flagged cells.
 template <class C, class G>
 int launch_tile_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, double a, double b, int first, int last,
-                      S4 *rhs_out) {
+                      S4 *rhs_out, int restrict_after, int reset_bad) {
     hipEvent_t e0{};
     int rc = prof_begin(h, 0, &e0);
     if (rc) return rc;
@@ -2264,12 +2344,12 @@ int launch_tile_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, dou
         auto *kern = ns ? (fixed ? k_tile<C, G, true, false, kTileGgBarthRoe, S> : k_tile<C, G, true, false, kTileAny, S>)
                         : (fixed ? k_tile<C, G, false, false, kTileGgBarthRoe, S> : k_tile<C, G, false, false, kTileAny, S>);
         kern<<<h->ntiles, TileCfg<C, S>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, Uin, Un, Uout, rhs_out, h->sc,
-                                                                   h->bad_list, h->nc, a, b, first, last);
+                                                                   h->bad_list, h->nc, a, b, first, last, reset_bad);
         return FVMR_OK;
     });
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 0, e0))) return rc;
-    return last ? launch_restrict<G>(h, Uout) : FVMR_OK;
+    return (last && restrict_after) ? launch_restrict<G>(h, Uout) : FVMR_OK;
 }
 
 // One step on the device with the tile kernels.
@@ -2278,12 +2358,15 @@ int device_step_tiles(fvmr_solver *h, bool store_rhs) {
     int rc;
     const int a = h->cur, b = (a + 1) % 3, c = (a + 2) % 3;
     S4 *Ua = h->Ubuf[a], *Ub = h->Ubuf[b], *Uc = h->Ubuf[c];
-    if ((rc = launch_tile_dt<C, G>(h, Ua, 1))) return rc;
+    // SSP-RK3: the restriction of this step's flagged cells happens in the next step's time-step
+    // kernels (or after the call's last step), which saves its launch per step
     const int rk3 = h->p.time_advance == FVMR_EVO_RK3;
-    if ((rc = launch_tile_stage<C, G>(h, Ua, Ua, Ub, 0.0, 1.0, 1, !rk3, store_rhs ? h->rhs : nullptr))) return rc;
+    if ((rc = launch_tile_dt<C, G>(h, Ua, 1, rk3))) return rc;
+    if ((rc = launch_tile_stage<C, G>(h, Ua, Ua, Ub, 0.0, 1.0, 1, !rk3, store_rhs ? h->rhs : nullptr, !rk3, rk3)))
+        return rc;
     if (rk3) {
-        if ((rc = launch_tile_stage<C, G>(h, Ub, Ua, Uc, 0.75, 0.25, 0, 0, nullptr))) return rc;
-        if ((rc = launch_tile_stage<C, G>(h, Uc, Ua, Ub, 1.0 / 3.0, 2.0 / 3.0, 0, 1, nullptr))) return rc;
+        if ((rc = launch_tile_stage<C, G>(h, Ub, Ua, Uc, 0.75, 0.25, 0, 0, nullptr, 0, 0))) return rc;
+        if ((rc = launch_tile_stage<C, G>(h, Uc, Ua, Ub, 1.0 / 3.0, 2.0 / 3.0, 0, 1, nullptr, 0, 0))) return rc;
     }
     h->cur = b;
     if (store_rhs) h->rhs_valid = true;
@@ -2350,6 +2433,8 @@ int run_steps(fvmr_solver *h, int n, bool store_rhs_last) {
     }
     for (; done < n; done++)
         if ((rc = run_step(h, store_rhs_last && done == n - 1))) return rc;
+    if (n > 0 && defer_restriction(h))   // the last step's flagged cells
+        return h->mixed ? launch_restrict<float>(h, h->Ubuf[h->cur], 1) : launch_restrict<double>(h, h->Ubuf[h->cur], 1);
     return FVMR_OK;
 }
 
@@ -2395,7 +2480,7 @@ void free_all(fvmr_solver *h) {
     void *ptrs[] = {h->grad, h->rhs, h->fix, h->cgeo, h->nxy, h->egeo, h->emid, h->cslot, h->cnode, h->dtcode,
                     h->eint, h->partial, h->norm_out, h->bad_list, h->sc, h->t_info, h->t_halo, h->t_lnodes,
                     h->t_nxy, h->t_gslot, h->t_face, h->t_gcoef, h->t_dtrec, h->t_lcxy, h->t_fnrm,
-                    h->nc_off, h->nc_cells, h->node_buf, h->cbc, h->tile_case_begin, h->perm_dev, h->io_soa,
+                    h->nc_off, h->nc_cells, h->node_buf, h->cbc, h->tile_case_begin, h->badidx, h->perm_dev, h->io_soa,
                     h->bedge_ref_dev, h->bedge_nodes_dev, h->bedge_geo_dev};
     for (void *q : ptrs) if (q) (void)hipFree(q);
     for (int k = 0; k < kNumProf; k++)
@@ -2745,6 +2830,7 @@ static int create_impl(const fvmr_mesh_soa *m, const fvmr_params *p, int ncases,
         hipMalloc((void **)&h->rhs, (size_t)nc * sizeof(S4)) != hipSuccess ||
         hipMalloc((void **)&h->fix, (size_t)nc * sizeof(S4)) != hipSuccess ||
         hipMalloc((void **)&h->bad_list, 2 * (size_t)nc * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&h->badidx, (size_t)nc * sizeof(int)) != hipSuccess ||
         hipMalloc((void **)&h->partial, (size_t)nparts * sizeof(double)) != hipSuccess ||
         hipMalloc((void **)&h->norm_out, 4 * sizeof(double)) != hipSuccess ||
         hipMalloc((void **)&h->sc, ncases * sizeof(Scalars)) != hipSuccess)
@@ -2756,6 +2842,7 @@ static int create_impl(const fvmr_mesh_soa *m, const fvmr_params *p, int ncases,
     if (rc) return bail(rc);
     std::vector<Scalars> s0(ncases);
     if (hipMemsetAsync(h->Ubuf[0], 0, (size_t)nc * sizeof(S4), h->stream) != hipSuccess ||
+        hipMemsetAsync(h->badidx, 0xff, (size_t)nc * sizeof(int), h->stream) != hipSuccess ||
         hipMemcpyAsync(h->sc, s0.data(), ncases * sizeof(Scalars), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
         return bail(set_error(FVMR_ERR_HIP, "initial copy failed"));
@@ -3054,8 +3141,8 @@ int fvmr_stage_rhs(fvmr_handle *h, double *const rhs[4]) {
     HIP_OK(hipMemcpyAsync(h->sc, &z, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
     const int a = h->cur, b = (a + 1) % 3;
     if (h->tiles) {
-        rc = h->mixed ? launch_tile_stage<float, float>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, 0, h->rhs)
-                      : launch_tile_stage<double, double>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, 0, h->rhs);
+        rc = h->mixed ? launch_tile_stage<float, float>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, 0, h->rhs, 0, 0)
+                      : launch_tile_stage<double, double>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, 0, h->rhs, 0, 0);
         if (rc) return rc;
     } else if (h->mixed) {
         if ((rc = launch_gradient<float, float>(h, h->Ubuf[a], -1)) ||
@@ -3101,8 +3188,8 @@ int fvmr_global_dt(fvmr_handle *h, double *dt) {
     if (!h || !dt) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
     int rc;
     if (h->tiles)
-        rc = h->mixed ? launch_tile_dt<float, float>(h, h->Ubuf[h->cur], 0)
-                      : launch_tile_dt<double, double>(h, h->Ubuf[h->cur], 0);
+        rc = h->mixed ? launch_tile_dt<float, float>(h, h->Ubuf[h->cur], 0, 0)
+                      : launch_tile_dt<double, double>(h, h->Ubuf[h->cur], 0, 0);
     else
         rc = h->mixed ? launch_gradient<float, float>(h, h->Ubuf[h->cur], 0)
                       : launch_gradient<double, double>(h, h->Ubuf[h->cur], 0);

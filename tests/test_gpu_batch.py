@@ -118,7 +118,45 @@ def test_batch_restriction_counts_per_case():
     Ub, Ug = b.get_state(k), g.get_state()
     assert np.array_equal(np.isnan(Ub), np.isnan(Ug))
     assert np.array_equal(Ub[~np.isnan(Ub)], Ug[~np.isnan(Ug)])
+    # a further step: case k repairs what its own handle repairs, the other cases nothing
+    t = b.advance(1, t, np.full(len(cs), 1e9))
+    tg, ig = g.iteration_device(tg, 1e9)
+    counts = [b.step_info(c).n_restricted for c in range(len(cs))]
+    assert counts[k] == ig.n_restricted and sum(counts) == counts[k], counts
+    assert t[k] == tg
+    Ub, Ug = b.get_state(k), g.get_state()
+    assert np.array_equal(np.isnan(Ub), np.isnan(Ug))
+    assert np.array_equal(Ub[~np.isnan(Ub)], Ug[~np.isnan(Ug)])
     g.close()
-    # a further step without bad cells counts none anywhere
-    b.advance(1, t, np.full(len(cs), 1e9))
+    b.close()
+
+
+def test_batch_restriction_deferred_within_a_call():
+    """Dense cells (flagged by the first step) in two cases other than the first: four steps of
+    the batch in one call, whose restriction happens in the next step's time-step kernels, equal
+    four one-step calls of each case on its own handle bit for bit."""
+    announce("batch: deferred restriction of dense cells in cases 2 and 5, 4 steps in one call")
+    cs = cases()
+    for k in (2, 5):
+        m, p, U = cs[k]
+        U = U.copy()
+        n = U.shape[1]
+        for c in (n // 3, n // 3 + 1, (2 * n) // 3):
+            U[:, c] *= 400.0              # density above the allowed 300
+        cs[k] = (m, p, U)
+    b = F.BatchSolver([m for m, _, _ in cs], [p for _, p, _ in cs])
+    for c, (_, _, Uc) in enumerate(cs):
+        b.set_state(c, Uc)
+    t = b.advance(4, np.zeros(len(cs)), np.full(len(cs), 1e9))
+    for c, (m, p, Uc) in enumerate(cs):
+        g = F.GPUSolver2(m, p)
+        g.set_state(Uc)
+        tg, first = g.iteration_device(0.0, 1e9)
+        for _ in range(3):
+            tg, ig = g.iteration_device(tg, 1e9)
+        assert first.n_restricted == (3 if c in (2, 5) else 0), (c, first.n_restricted)
+        assert b.step_info(c).n_restricted == ig.n_restricted
+        assert t[c] == tg
+        assert np.array_equal(b.get_state(c), g.get_state()), c
+        g.close()
     b.close()

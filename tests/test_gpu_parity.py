@@ -117,6 +117,43 @@ def test_time_clamp():
     g.close()
 
 
+@pytest.mark.parametrize("precision", [F.PREC_FP64, F.PREC_MIXED])
+def test_restriction_deferred_within_a_call(precision):
+    """SSP-RK3 steps of the tile kernels repair the cells a step flags in the next step's
+    time-step kernels, and after the call's last step: six steps in one call equal six one-step
+    calls bit for bit, with the same restriction counts as the oracle (six dense cells, three of
+    them neighbours, flagged by the first step) and, in FP64, the oracle's state."""
+    announce(f"deferred restriction: 6 steps in one call vs 6 calls, precision={precision}")
+    m, p, U = vortex_case(32, advance=F.EVO_RK3, precision=precision)
+    U = U.copy()
+    for c in (100, 101, 102, 500, 777, 1500):
+        U[:, c] *= 400.0                 # density 400 > the allowed 300
+    g1 = F.GPUSolver2(m, p)
+    g1.set_state(U)
+    t1, i1 = g1.advance(6, 0.0, 1e9)
+    g2 = F.GPUSolver2(m, p)
+    g2.set_state(U)
+    t2, counts = 0.0, []
+    for _ in range(6):
+        t2, i2 = g2.iteration_device(t2, 1e9)
+        counts.append(i2.n_restricted)
+    o = O.OracleSolver(m, p)
+    o.set_state(U)
+    to, ocounts = 0.0, []
+    for _ in range(6):
+        to, io = o.step(to, 1e9)
+        ocounts.append(io.n_restricted)
+    assert counts == ocounts and counts[0] == 6, (counts, ocounts)
+    assert i1.n_restricted == counts[-1] and t1 == t2
+    U1, U2 = g1.get_state(), g2.get_state()
+    assert np.array_equal(U1, U2)
+    if precision == F.PREC_FP64:
+        assert abs(t1 / to - 1) < TOL
+        assert rel_l2(U1, o.get_state()) < TOL
+    g1.close()
+    g2.close()
+
+
 def test_residual_norms():
     announce("residual norms L1/L2/Linf")
     m, p, U = vortex_case(32)
