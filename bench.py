@@ -336,10 +336,11 @@ def config_c4(ncases=256, steps=500, seed=0):
         params.append(p)
         states.append(F.field_init(m, p, F.INIT_UNIFORM))
     total = sum(m.num_element for m in meshes)
-    # the library's tuner on every 8th case (about 1/8 of the cells; a batch handle per
-    # candidate), then the whole batch with the tile size it picks
+    # the library's tuner on the whole batch (a batch handle per candidate: a sample of the cases
+    # would fit the last-level cache and favour another shape than the batch itself), then the
+    # batch with the work-group shape and tile size it picks
     t0 = time.perf_counter()
-    tune_cases = list(range(0, ncases, 8))
+    tune_cases = list(range(ncases))
     best_th, best_tc, best_ms, timings = F.autotune_batch([meshes[c] for c in tune_cases],
                                                           [params[c] for c in tune_cases], steps=10)
     tune_s = time.perf_counter() - t0

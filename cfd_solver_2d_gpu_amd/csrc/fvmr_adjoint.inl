@@ -198,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_steady_stage(Mesh<double> m, Phys ph
         const double s = Ui[k] + dt * r[k];
         Unew[k] = first ? s : a * Ub[k] + b * s;
     }
-    st(Uout, c, Unew);
+    st_stream(Uout, c, Unew);
 }
 
 // ---------------------------------------------------------------------------
@@ -1043,9 +1043,9 @@ int local_dt(fvmr_solver *h, const S4 *U) {
         TileArgs<double> t = tile_args<double>(h);
         t.dtl = w->dtl;
         if (h->ph.equation == FVMR_EQ_NS)
-            k_tile_dt<double, true, true><<<h->ntiles, TileCfg<double>::kThreads, 0, h->stream>>>(t, h->ph, U, nullptr, nullptr);
+            k_tile_dt<double, true, true><<<h->ntiles, TileCfg<double>::kThreads, 0, h->stream>>>(t, h->ph, U, nullptr);
         else
-            k_tile_dt<double, false, true><<<h->ntiles, TileCfg<double>::kThreads, 0, h->stream>>>(t, h->ph, U, nullptr, nullptr);
+            k_tile_dt<double, false, true><<<h->ntiles, TileCfg<double>::kThreads, 0, h->stream>>>(t, h->ph, U, nullptr);
     } else {
         k_local_dt<<<grid_of(h->nc), kBlock, 0, h->stream>>>(mesh_args<double>(h), h->ph, U, w->dtl);
     }
@@ -1075,10 +1075,10 @@ int steady_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, double a
         t.dtl = w->dtl;
         if (h->ph.equation == FVMR_EQ_NS)
             k_tile<double, double, true, true><<<h->ntiles, TileCfg<double>::kThreads, 0, h->stream>>>(
-                t, h->ph, Uin, Un, Uout, rhs, h->sc, h->bad_list, h->nc, a, b, first, 0, 0);
+                t, h->ph, Uin, Un, Uout, rhs, h->sc, h->bad_list, h->nc, a, b, first, 0);
         else
             k_tile<double, double, false, true><<<h->ntiles, TileCfg<double>::kThreads, 0, h->stream>>>(
-                t, h->ph, Uin, Un, Uout, rhs, h->sc, h->bad_list, h->nc, a, b, first, 0, 0);
+                t, h->ph, Uin, Un, Uout, rhs, h->sc, h->bad_list, h->nc, a, b, first, 0);
         HIP_OK(hipGetLastError());
         return FVMR_OK;
     }

@@ -190,6 +190,17 @@ __device__ __forceinline__ void st(S4 *p, int i, const double u[4]) {
     v.x = u[0]; v.y = u[1]; v.z = u[2]; v.w = u[3];
     p[i] = v;
 }
+// The same store with the nontemporal hint, for the FP64 stage kernels' new state: 61.1 -> 59.5
+// us per stage kernel at 1M cells (2 % per step; the kernel boundary has less to write back).
+// Not used where it measured slower: the FP32-mixed stage kernel (+1.5 %) and the adjoint's
+// products (0.353 -> 0.425 ms per iteration), whose outputs the next kernel gathers.
+__device__ __forceinline__ void st_stream(S4 *p, int i, const double u[4]) {
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    d2v *q = reinterpret_cast<d2v *>(p + i);
+    const d2v lo = {u[0], u[1]}, hi = {u[2], u[3]};
+    __builtin_nontemporal_store(lo, q);
+    __builtin_nontemporal_store(hi, q + 1);
+}
 
 // ruvp2U (PhysicalKernelGPU.h:17-22)
 template <class T>
@@ -1022,7 +1033,8 @@ __global__ __launch_bounds__(kGroup3) FVMR_STAGE_ATTR void k_stage(Mesh<G> m, Ph
         Unew[k] = first ? s : a * Ub[k] + b * s;
     }
     if (rhs_out) st(rhs_out, c, r);
-    st(Uout, c, Unew);
+    if (std::is_same<C, double>::value) st_stream(Uout, c, Unew);
+    else st(Uout, c, Unew);
     if (!last) return;
     double rv[4];
     prim(Unew, rv, ph.gamma);
@@ -1122,34 +1134,6 @@ struct DtRec {
     double vol;
 };
 
-// The repair of a flagged cell c in modifyElementFieldU2d_device_kernel (RestrictGPU.cu:6-84):
-// the volume-weighted mean of its element neighbours' states (periodic neighbours excluded), with
-// the volume sum taken once per variable (the reference's quirk: the mean is a quarter of it), or
-// its own state when it has no neighbour.  No contraction, so that the restriction kernel and the
-// time-step kernel's deferred repair form the same bits.
-template <class G>
-__device__ __forceinline__ void repaired_state(const int2 *__restrict__ cslot, const V4<G> *__restrict__ cgeo,
-                                               const S4 *__restrict__ U, int nc, int c, double res[4]) {
-#pragma clang fp contract(off)
-    int nvalid = 0;
-    double Usum[4] = {0, 0, 0, 0}, vs = 0.0;
-    for (int j = 0; j < 3; j++) {
-        const int2 sl = cslot[j * nc + c];
-        const int nb = (sl.x & kSlotPeriodic) ? -1 : sl.y;   // periodic neighbours are not element neighbours
-        if (nb == -1) continue;
-        nvalid++;
-        const double v = (double)cgeo[nb].z;
-        double Unb[4];
-        ld(U, nb, Unb);
-        for (int k = 0; k < 4; k++) {
-            Usum[k] += Unb[k] * v;
-            vs += v;
-        }
-    }
-    if (nvalid == 0) ld(U, c, res);
-    else for (int k = 0; k < 4; k++) res[k] = Usum[k] / vs;
-}
-
 template <class G>
 struct TileArgs {
     const int *info;          // fvmr::kTileInfo ints per tile
@@ -1164,13 +1148,6 @@ struct TileArgs {
     const V2<G> *fnrm;        // per face: normal (FVMR_TILE_GEOM_LOADED)
     const CaseBC *cbc;        // per case of the batch (one case: the handle's)
     double *dtl;              // per-cell steps of a steady solve (k_tile_dt / k_tile with LDT), else NULL
-    // the deferred restriction of an SSP-RK3 step's flagged cells (k_tile_dt repairs the states it
-    // stages, k_dt_finish_cases writes them)
-    const int2 *cslot;        // [3][nc] face slots (Mesh::cslot)
-    const V4<G> *cgeo;        // per cell: centroid, volume (Mesh::cgeo)
-    int *badidx;              // per cell: its entry in the list of flagged cells, -1 none
-    S4 *fix;                  // per entry of that list: the repaired state
-    int nc;
 };
 
 __device__ __forceinline__ int lfield(int w, int k) { return (w >> (fvmr::kLocalBits * k)) & fvmr::kLocalMask; }
@@ -1256,7 +1233,7 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
                                                         const S4 *__restrict__ Un, S4 *__restrict__ Uout,
                                                         S4 *__restrict__ rhs_out, Scalars *__restrict__ sc,
                                                         int *__restrict__ bad_list, int nc, double a, double b,
-                                                        int first, int last, int reset_bad) {
+                                                        int first, int last) {
     if (SCHEME == kTileGgBarthRoe) {
         ph.gradient = FVMR_GRA_GREENGAUSS;
         ph.limiter = FVMR_LIM_BARTH;
@@ -1279,10 +1256,6 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
     Grad<G> *sGr = reinterpret_cast<Grad<G> *>(sGrFl);
     V4<C> *sFl = reinterpret_cast<V4<C> *>(sGrFl);
     const int tid = threadIdx.x;
-    // reset_bad (the first stage of an SSP-RK3 step): the previous step's flagged cells are
-    // repaired by now (k_tile_dt, k_dt_finish_cases); the list starts empty for this step's last
-    // stage
-    if (reset_bad && blockIdx.x == 0 && tid == 0) sc->bad_count = 0;
     const int *ti = ta.info + (size_t)xcd_block() * fvmr::kTileInfo;
     const int c0 = ti[fvmr::kTiC0], nT = ti[fvmr::kTiNT], nG = ti[fvmr::kTiNG], nL = ti[fvmr::kTiNL];
     const int nN = ti[fvmr::kTiNN], nF = ti[fvmr::kTiNF], nFF = ti[fvmr::kTiNFF];
@@ -1453,7 +1426,8 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
             Unew[k] = first ? s : a * Ub[k] + b * s;
         }
         if (rhs_out) st(rhs_out, c, r);
-        st(Uout, c, Unew);
+        if (std::is_same<C, double>::value) st_stream(Uout, c, Unew);
+        else st(Uout, c, Unew);
         if (!last) return;
         double rv[4];
         prim(Unew, rv, ph.gamma);
@@ -1465,7 +1439,6 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
             if (slot < nc) {
                 bad_list[slot] = c;
                 bad_list[nc + slot] = cs;
-                ta.badidx[c] = slot;
             }
         }
     }
@@ -1480,8 +1453,7 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
 template <class G, bool NS, bool LDT = false, int SHAPE = 0>
 __global__ __launch_bounds__((TileCfg<G, SHAPE>::kThreads)) void k_tile_dt(TileArgs<G> ta, Phys ph,
                                                                           const S4 *__restrict__ U,
-                                                                          double *__restrict__ partial,
-                                                                          const Scalars *__restrict__ sc) {
+                                                                          double *__restrict__ partial) {
     if (!NS) ph.equation = FVMR_EQ_EULER;   // lets the compiler drop the viscous terms
     using Cfg = TileCfg<G, SHAPE>;
     constexpr int kTileThreads = Cfg::kThreads, kTileMaxG = Cfg::kMaxG, kTileMaxN = Cfg::kMaxN;
@@ -1504,20 +1476,7 @@ __global__ __launch_bounds__((TileCfg<G, SHAPE>::kThreads)) void k_tile_dt(TileA
     const V2<double> nd = (tid < nN) ? ta.nxy[offN + tid] : V2<double>{0.0, 0.0};
     const int4 fr = (tid < nFF) ? ta.face[offF + tid] : make_int4(0, 0, 0, 0);
     const int src = (tid < nG) ? (tid < nT ? c0 + tid : ta.halo[offH + tid - nT]) : -1;
-    S4 u = (src >= 0) ? U[src] : S4{0.0, 0.0, 0.0, 0.0};
-    // cells the previous step's last stage flagged (rare): each work-group repairs the states it
-    // stages from the unrepaired field (nothing writes U during this kernel, so every copy of a
-    // repair has the same bits) and the tile that owns the cell leaves its repair for
-    // k_dt_finish_cases to write
-    if (!LDT && sc && sc->bad_count > 0 && src >= 0) {
-        const int bi = ta.badidx[src];
-        if (bi >= 0) {
-            double r[4];
-            repaired_state<G>(ta.cslot, ta.cgeo, U, ta.nc, src, r);
-            u = S4{r[0], r[1], r[2], r[3]};
-            if (tid < nT) ta.fix[bi] = u;
-        }
-    }
+    const S4 u = (src >= 0) ? U[src] : S4{0.0, 0.0, 0.0, 0.0};
     if (tid < nN) sN[tid] = nd;
     if (tid < nFF) sFace[tid] = fr;
     if (tid < nG) sU[tid] = u;
@@ -1819,30 +1778,39 @@ __global__ __launch_bounds__(kBlock) void k_from_caller(int nc, int base, const 
 }
 
 // modifyElementFieldU2d_device_kernel (RestrictGPU.cu:6-84) with Jacobi reads: one work-group,
-// phase 1 computes all repairs from the unrepaired field, phase 2 writes them and clears the
-// cells' entries in badidx.  reset_counts: every case's count of restricted cells starts from 0
-// (the restriction that ends an fvmr_advance call, whose steps deferred theirs).
+// phase 1 computes all repairs from the unrepaired field, phase 2 writes them.
 // The body of k_restrict (every thread of the work-group calls it).
 template <class G>
 __device__ __forceinline__ void restrict_cells(const Mesh<G> &m, S4 *__restrict__ U, Scalars *__restrict__ sc,
-                                               const int *__restrict__ bad_list, S4 *__restrict__ fix,
-                                               int *__restrict__ badidx, int reset_counts, int ncases) {
+                                               const int *__restrict__ bad_list, S4 *__restrict__ fix) {
     const int nc = m.nc;
-    if (reset_counts) {
-        for (int c = threadIdx.x; c < ncases; c += blockDim.x) sc[c].n_restricted = 0;
-        __syncthreads();
-    }
     const int count = min(sc->bad_count, nc);
     if (count == 0) return;   // the usual case: nothing flagged, nothing to write
     for (int i = threadIdx.x; i < count; i += blockDim.x) {
+        const int c = bad_list[i];
+        int nvalid = 0;
+        double Usum[4] = {0, 0, 0, 0}, vs = 0.0;
+        for (int j = 0; j < 3; j++) {
+            const int2 sl = m.cslot[j * nc + c];
+            const int nb = (sl.x & kSlotPeriodic) ? -1 : sl.y;   // periodic neighbours are not element neighbours
+            if (nb == -1) continue;
+            nvalid++;
+            const double v = (double)m.cgeo[nb].z;
+            double Unb[4];
+            ld(U, nb, Unb);
+            for (int k = 0; k < 4; k++) {
+                Usum[k] += Unb[k] * v;
+                vs += v;
+            }
+        }
         double res[4];
-        repaired_state<G>(m.cslot, m.cgeo, U, nc, bad_list[i], res);
+        if (nvalid == 0) ld(U, c, res);
+        else for (int k = 0; k < 4; k++) res[k] = Usum[k] / vs;
         st(fix, i, res);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < count; i += blockDim.x) {
         U[bad_list[i]] = fix[i];
-        badidx[bad_list[i]] = -1;
         atomicAdd(&sc[bad_list[nc + i]].n_restricted, 1);   // counted per case of a batch
     }
     __syncthreads();
@@ -1852,9 +1820,8 @@ __device__ __forceinline__ void restrict_cells(const Mesh<G> &m, S4 *__restrict_
 }
 template <class G>
 __global__ __launch_bounds__(1024) void k_restrict(Mesh<G> m, S4 *__restrict__ U, Scalars *__restrict__ sc,
-                                                   const int *__restrict__ bad_list, S4 *__restrict__ fix,
-                                                   int *__restrict__ badidx, int reset_counts, int ncases) {
-    restrict_cells<G>(m, U, sc, bad_list, fix, badidx, reset_counts, ncases);
+                                                   const int *__restrict__ bad_list, S4 *__restrict__ fix) {
+    restrict_cells<G>(m, U, sc, bad_list, fix);
 }
 
 // Minimum of the block partials of k_gradient<DT>; then the clamp to T - t and the t
@@ -1890,17 +1857,10 @@ __global__ __launch_bounds__(1024) void k_dt_finish(int nparts, const double *__
 }
 
 // k_dt_finish per case of a batch: block c takes the minimum over the partials of its tiles
-// tile_begin[c] .. tile_begin[c+1]-1 and clamps / advances the times of sc[c].  apply (SSP-RK3
-// steps of the tile kernels, whose restriction is deferred to the next step): then it writes the
-// repairs k_tile_dt left in fix for the cells of case c that the previous step's last stage
-// flagged, clears their badidx entries and counts them (the next step's first stage empties the
-// list).
+// tile_begin[c] .. tile_begin[c+1]-1 and clamps / advances the times of sc[c].
 __global__ __launch_bounds__(1024) void k_dt_finish_cases(const int *__restrict__ tile_begin,
                                                           const double *__restrict__ partial,
-                                                          Scalars *__restrict__ sc, int advance, int apply,
-                                                          const int *__restrict__ bad_list,
-                                                          const S4 *__restrict__ fix, int *__restrict__ badidx,
-                                                          S4 *__restrict__ U, int nc) {
+                                                          Scalars *__restrict__ sc, int advance) {
     __shared__ double s[16];
     const int c = blockIdx.x;
     const int beg = tile_begin[c], end = tile_begin[c + 1];
@@ -1929,18 +1889,6 @@ __global__ __launch_bounds__(1024) void k_dt_finish_cases(const int *__restrict_
             sc[c].n_restricted = 0;
         }
         sc[c].dt = d;
-    }
-    if (apply) {
-        const int count = min(sc[0].bad_count, nc);
-        if (count == 0) return;
-        __syncthreads();   // n_restricted cleared above
-        for (int i = threadIdx.x; i < count; i += blockDim.x) {
-            if (bad_list[nc + i] != c) continue;
-            const int cell = bad_list[i];
-            U[cell] = fix[i];
-            badidx[cell] = -1;
-            atomicAdd(&sc[c].n_restricted, 1);
-        }
     }
 }
 
@@ -2141,7 +2089,6 @@ struct fvmr_solver {
     std::vector<int> case_cell_begin;
     CaseBC *cbc = nullptr;
     int *tile_case_begin = nullptr;       // tiles of case c: [c] .. [c+1]-1
-    int *badidx = nullptr;                // per cell: its entry in bad_list, -1 none
     int *perm_dev = nullptr;              // perm on the device (state transfers, first use)
     double *io_soa = nullptr;             // 4 x nc doubles in the caller's order (state transfers)
     bool io_ready = false;                // perm_dev holds perm (set by ensure_io)
@@ -2197,11 +2144,6 @@ TileArgs<G> tile_args(const fvmr_solver *h) {
     t.lcxy = (const V2<G> *)h->t_lcxy; t.fnrm = (const V2<G> *)h->t_fnrm;
     t.cbc = h->cbc;
     t.dtl = nullptr;
-    t.cslot = h->cslot;
-    t.cgeo = (const V4<G> *)h->cgeo;
-    t.badidx = h->badidx;
-    t.fix = h->fix;
-    t.nc = h->nc;
     return t;
 }
 
@@ -2271,16 +2213,13 @@ int launch_gradient(fvmr_solver *h, const S4 *U, int dt_mode) {
 }
 
 // The restriction after the step's last stage (CIteration.cpp:79 and :152 restrict once per
-// iteration, after the evolve).  SSP-RK3 steps of the tile kernels defer it to the next step's
-// time-step kernels (defer_restriction) and the call that runs them restricts once after its last
-// step (reset_counts: the cases' counts then hold that step's repairs alone).
+// iteration, after the evolve).
 template <class G>
-int launch_restrict(fvmr_solver *h, S4 *U, int reset_counts = 0) {
+int launch_restrict(fvmr_solver *h, S4 *U) {
     hipEvent_t e0{};
     int rc = prof_begin(h, 3, &e0);
     if (rc) return rc;
-    k_restrict<G><<<1, 1024, 0, h->stream>>>(mesh_args<G>(h), U, h->sc, h->bad_list, h->fix, h->badidx,
-                                             reset_counts, h->ncases);
+    k_restrict<G><<<1, 1024, 0, h->stream>>>(mesh_args<G>(h), U, h->sc, h->bad_list, h->fix);
     HIP_OK(hipGetLastError());
     return prof_end(h, 3, e0);
 }
@@ -2302,12 +2241,9 @@ int launch_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, double a
     return last ? launch_restrict<G>(h, Uout) : FVMR_OK;
 }
 
-bool defer_restriction(const fvmr_solver *h) { return h->tiles && h->p.time_advance == FVMR_EVO_RK3; }
-
-// Global dt over the tiles (dt_mode as launch_gradient); apply: the finish writes the repairs of
-// the cells the previous step flagged (deferred restriction).
+// Global dt over the tiles (dt_mode as launch_gradient).
 template <class C, class G>
-int launch_tile_dt(fvmr_solver *h, const S4 *U, int dt_mode, int apply) {
+int launch_tile_dt(fvmr_solver *h, const S4 *U, int dt_mode) {
     hipEvent_t e0{};
     int rc = prof_begin(h, 1, &e0);
     if (rc) return rc;
@@ -2315,26 +2251,23 @@ int launch_tile_dt(fvmr_solver *h, const S4 *U, int dt_mode, int apply) {
         constexpr int S = decltype(sh)::value;
         if (h->ph.equation == FVMR_EQ_NS)
             k_tile_dt<G, true, false, S><<<h->ntiles, TileCfg<G, S>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, U,
-                                                                                               h->partial, h->sc);
+                                                                                               h->partial);
         else
             k_tile_dt<G, false, false, S><<<h->ntiles, TileCfg<G, S>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, U,
-                                                                                                h->partial, h->sc);
+                                                                                                h->partial);
         return FVMR_OK;
     });
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 1, e0))) return rc;
     if ((rc = prof_begin(h, 2, &e0))) return rc;
-    k_dt_finish_cases<<<h->ncases, 1024, 0, h->stream>>>(h->tile_case_begin, h->partial, h->sc, dt_mode, apply,
-                                                         h->bad_list, h->fix, h->badidx, const_cast<S4 *>(U), h->nc);
+    k_dt_finish_cases<<<h->ncases, 1024, 0, h->stream>>>(h->tile_case_begin, h->partial, h->sc, dt_mode);
     HIP_OK(hipGetLastError());
     return prof_end(h, 2, e0);
 }
 
-// restrict_after: the restriction follows a last stage (not when deferred); reset_bad: the
-// first stage of a step whose restriction was deferred empties the list of flagged cells.
 template <class C, class G>
 int launch_tile_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, double a, double b, int first, int last,
-                      S4 *rhs_out, int restrict_after, int reset_bad) {
+                      S4 *rhs_out) {
     hipEvent_t e0{};
     int rc = prof_begin(h, 0, &e0);
     if (rc) return rc;
@@ -2344,12 +2277,12 @@ int launch_tile_stage(fvmr_solver *h, const S4 *Uin, const S4 *Un, S4 *Uout, dou
         auto *kern = ns ? (fixed ? k_tile<C, G, true, false, kTileGgBarthRoe, S> : k_tile<C, G, true, false, kTileAny, S>)
                         : (fixed ? k_tile<C, G, false, false, kTileGgBarthRoe, S> : k_tile<C, G, false, false, kTileAny, S>);
         kern<<<h->ntiles, TileCfg<C, S>::kThreads, 0, h->stream>>>(tile_args<G>(h), h->ph, Uin, Un, Uout, rhs_out, h->sc,
-                                                                   h->bad_list, h->nc, a, b, first, last, reset_bad);
+                                                                   h->bad_list, h->nc, a, b, first, last);
         return FVMR_OK;
     });
     HIP_OK(hipGetLastError());
     if ((rc = prof_end(h, 0, e0))) return rc;
-    return (last && restrict_after) ? launch_restrict<G>(h, Uout) : FVMR_OK;
+    return last ? launch_restrict<G>(h, Uout) : FVMR_OK;
 }
 
 // One step on the device with the tile kernels.
@@ -2358,15 +2291,12 @@ int device_step_tiles(fvmr_solver *h, bool store_rhs) {
     int rc;
     const int a = h->cur, b = (a + 1) % 3, c = (a + 2) % 3;
     S4 *Ua = h->Ubuf[a], *Ub = h->Ubuf[b], *Uc = h->Ubuf[c];
-    // SSP-RK3: the restriction of this step's flagged cells happens in the next step's time-step
-    // kernels (or after the call's last step), which saves its launch per step
+    if ((rc = launch_tile_dt<C, G>(h, Ua, 1))) return rc;
     const int rk3 = h->p.time_advance == FVMR_EVO_RK3;
-    if ((rc = launch_tile_dt<C, G>(h, Ua, 1, rk3))) return rc;
-    if ((rc = launch_tile_stage<C, G>(h, Ua, Ua, Ub, 0.0, 1.0, 1, !rk3, store_rhs ? h->rhs : nullptr, !rk3, rk3)))
-        return rc;
+    if ((rc = launch_tile_stage<C, G>(h, Ua, Ua, Ub, 0.0, 1.0, 1, !rk3, store_rhs ? h->rhs : nullptr))) return rc;
     if (rk3) {
-        if ((rc = launch_tile_stage<C, G>(h, Ub, Ua, Uc, 0.75, 0.25, 0, 0, nullptr, 0, 0))) return rc;
-        if ((rc = launch_tile_stage<C, G>(h, Uc, Ua, Ub, 1.0 / 3.0, 2.0 / 3.0, 0, 1, nullptr, 0, 0))) return rc;
+        if ((rc = launch_tile_stage<C, G>(h, Ub, Ua, Uc, 0.75, 0.25, 0, 0, nullptr))) return rc;
+        if ((rc = launch_tile_stage<C, G>(h, Uc, Ua, Ub, 1.0 / 3.0, 2.0 / 3.0, 0, 1, nullptr))) return rc;
     }
     h->cur = b;
     if (store_rhs) h->rhs_valid = true;
@@ -2433,8 +2363,6 @@ int run_steps(fvmr_solver *h, int n, bool store_rhs_last) {
     }
     for (; done < n; done++)
         if ((rc = run_step(h, store_rhs_last && done == n - 1))) return rc;
-    if (n > 0 && defer_restriction(h))   // the last step's flagged cells
-        return h->mixed ? launch_restrict<float>(h, h->Ubuf[h->cur], 1) : launch_restrict<double>(h, h->Ubuf[h->cur], 1);
     return FVMR_OK;
 }
 
@@ -2480,7 +2408,7 @@ void free_all(fvmr_solver *h) {
     void *ptrs[] = {h->grad, h->rhs, h->fix, h->cgeo, h->nxy, h->egeo, h->emid, h->cslot, h->cnode, h->dtcode,
                     h->eint, h->partial, h->norm_out, h->bad_list, h->sc, h->t_info, h->t_halo, h->t_lnodes,
                     h->t_nxy, h->t_gslot, h->t_face, h->t_gcoef, h->t_dtrec, h->t_lcxy, h->t_fnrm,
-                    h->nc_off, h->nc_cells, h->node_buf, h->cbc, h->tile_case_begin, h->badidx, h->perm_dev, h->io_soa,
+                    h->nc_off, h->nc_cells, h->node_buf, h->cbc, h->tile_case_begin, h->perm_dev, h->io_soa,
                     h->bedge_ref_dev, h->bedge_nodes_dev, h->bedge_geo_dev};
     for (void *q : ptrs) if (q) (void)hipFree(q);
     for (int k = 0; k < kNumProf; k++)
@@ -2830,7 +2758,6 @@ static int create_impl(const fvmr_mesh_soa *m, const fvmr_params *p, int ncases,
         hipMalloc((void **)&h->rhs, (size_t)nc * sizeof(S4)) != hipSuccess ||
         hipMalloc((void **)&h->fix, (size_t)nc * sizeof(S4)) != hipSuccess ||
         hipMalloc((void **)&h->bad_list, 2 * (size_t)nc * sizeof(int)) != hipSuccess ||
-        hipMalloc((void **)&h->badidx, (size_t)nc * sizeof(int)) != hipSuccess ||
         hipMalloc((void **)&h->partial, (size_t)nparts * sizeof(double)) != hipSuccess ||
         hipMalloc((void **)&h->norm_out, 4 * sizeof(double)) != hipSuccess ||
         hipMalloc((void **)&h->sc, ncases * sizeof(Scalars)) != hipSuccess)
@@ -2842,7 +2769,6 @@ static int create_impl(const fvmr_mesh_soa *m, const fvmr_params *p, int ncases,
     if (rc) return bail(rc);
     std::vector<Scalars> s0(ncases);
     if (hipMemsetAsync(h->Ubuf[0], 0, (size_t)nc * sizeof(S4), h->stream) != hipSuccess ||
-        hipMemsetAsync(h->badidx, 0xff, (size_t)nc * sizeof(int), h->stream) != hipSuccess ||
         hipMemcpyAsync(h->sc, s0.data(), ncases * sizeof(Scalars), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
         return bail(set_error(FVMR_ERR_HIP, "initial copy failed"));
@@ -3141,8 +3067,8 @@ int fvmr_stage_rhs(fvmr_handle *h, double *const rhs[4]) {
     HIP_OK(hipMemcpyAsync(h->sc, &z, sizeof(Scalars), hipMemcpyHostToDevice, h->stream));
     const int a = h->cur, b = (a + 1) % 3;
     if (h->tiles) {
-        rc = h->mixed ? launch_tile_stage<float, float>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, 0, h->rhs, 0, 0)
-                      : launch_tile_stage<double, double>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, 0, h->rhs, 0, 0);
+        rc = h->mixed ? launch_tile_stage<float, float>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, 0, h->rhs)
+                      : launch_tile_stage<double, double>(h, h->Ubuf[a], h->Ubuf[a], h->Ubuf[b], 0, 1, 1, 0, h->rhs);
         if (rc) return rc;
     } else if (h->mixed) {
         if ((rc = launch_gradient<float, float>(h, h->Ubuf[a], -1)) ||
@@ -3188,8 +3114,8 @@ int fvmr_global_dt(fvmr_handle *h, double *dt) {
     if (!h || !dt) return set_error(FVMR_ERR_INVALID_ARG, "null argument");
     int rc;
     if (h->tiles)
-        rc = h->mixed ? launch_tile_dt<float, float>(h, h->Ubuf[h->cur], 0, 0)
-                      : launch_tile_dt<double, double>(h, h->Ubuf[h->cur], 0, 0);
+        rc = h->mixed ? launch_tile_dt<float, float>(h, h->Ubuf[h->cur], 0)
+                      : launch_tile_dt<double, double>(h, h->Ubuf[h->cur], 0);
     else
         rc = h->mixed ? launch_gradient<float, float>(h, h->Ubuf[h->cur], 0)
                       : launch_gradient<double, double>(h, h->Ubuf[h->cur], 0);
@@ -3387,8 +3313,9 @@ static std::vector<TileCand> tile_candidates(bool mixed) {
 }
 
 // Times `steps` steps of the uniform freestream of each case on a handle made by `make` for
-// every candidate (after 2 untimed steps) and returns the fastest; timings: the ms per step of
-// every candidate, in candidate order (may be NULL).
+// every candidate (all handles at once, after a warm-up; the faster of two timed runs) and
+// returns the fastest; timings: the ms per step of every candidate, in candidate order (may be
+// NULL).
 template <class Make>
 static int autotune_tiles(int ncases, const fvmr_mesh_soa *const *meshes, const fvmr_params *params, int steps,
                           Make make, int *best_tile_threads, int *best_tile_cells, double *best_ms, double *timings) {
@@ -3405,33 +3332,61 @@ static int autotune_tiles(int ncases, const fvmr_mesh_soa *const *meshes, const 
         int rc = fvmr_field_init(meshes[c], &params[c], FVMR_INIT_UNIFORM, nullptr, rows);
         if (rc) return rc;
     }
-    double best = 1e300;
-    TileCand best_c = cand[0];
-    for (size_t q = 0; q < cand.size(); q++) {
-        fvmr_handle *h = nullptr;
-        int rc = make(cand[q], &h);
-        if (rc) return rc;
+    // every candidate's handle first (their set-up is host work, during which the GPU's clocks
+    // drop), then a warm-up that keeps the GPU busy for at least 40 ms (the clocks rise over
+    // tens of milliseconds of load), then two timed passes, forward and backward, the faster of
+    // each candidate's two runs counting
+    const size_t nq = cand.size();
+    std::vector<fvmr_handle *> hs(nq, nullptr);
+    std::vector<std::vector<double>> tq(nq, std::vector<double>(ncases, 0.0));
+    const std::vector<double> T(ncases, 1e300);
+    auto release = [&]() {
+        for (auto *h : hs)
+            if (h) fvmr_destroy(h);
+    };
+    int rc = FVMR_OK;
+    for (size_t q = 0; q < nq && !rc; q++) {
+        rc = make(cand[q], &hs[q]);
         for (int c = 0; c < ncases && !rc; c++) {
             const double *rows[4] = {U0[4 * c].data(), U0[4 * c + 1].data(), U0[4 * c + 2].data(), U0[4 * c + 3].data()};
-            rc = (ncases == 1) ? fvmr_set_state(h, rows) : fvmr_batch_set_state(h, c, rows);
+            rc = (ncases == 1) ? fvmr_set_state(hs[q], rows) : fvmr_batch_set_state(hs[q], c, rows);
         }
-        std::vector<double> t(ncases, 0.0), T(ncases, 1e300);
-        float ms = 0.0f;
+    }
+    auto timed = [&](size_t q, float *ms) -> int {
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (!rc) rc = fvmr_batch_advance(h, 2, t.data(), T.data());
-        if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
-            rc = set_error(FVMR_ERR_HIP, "hipEventCreate failed");
-        if (!rc && hipEventRecord(e0, h->stream) != hipSuccess) rc = set_error(FVMR_ERR_HIP, "hipEventRecord failed");
-        if (!rc) rc = fvmr_batch_advance(h, steps, t.data(), T.data());
-        if (!rc && (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-            rc = set_error(FVMR_ERR_HIP, "event timing failed");
+        int r = FVMR_OK;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+            r = set_error(FVMR_ERR_HIP, "hipEventCreate failed");
+        if (!r && hipEventRecord(e0, hs[q]->stream) != hipSuccess) r = set_error(FVMR_ERR_HIP, "hipEventRecord failed");
+        if (!r) r = fvmr_batch_advance(hs[q], steps, tq[q].data(), T.data());
+        if (!r && (hipEventRecord(e1, hs[q]->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                   hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+            r = set_error(FVMR_ERR_HIP, "event timing failed");
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
-        fvmr_destroy(h);
-        if (rc) return rc;
-        if (timings) timings[q] = ms / steps;
-        if (ms < best) { best = ms; best_c = cand[q]; }
+        return r;
+    };
+    float warm = 0.0f;
+    for (int k = 0; k < 1000 && !rc && warm < 40.0f; k++) {
+        float ms = 0.0f;
+        rc = timed((size_t)k % nq, &ms);
+        warm += ms;
+    }
+    std::vector<double> tbest(nq, 1e300);
+    for (int pass = 0; pass < 2 && !rc; pass++)
+        for (size_t k = 0; k < nq && !rc; k++) {
+            const size_t q = pass == 0 ? k : nq - 1 - k;
+            float ms = 0.0f;
+            rc = timed(q, &ms);
+            tbest[q] = std::min(tbest[q], (double)ms);
+        }
+    release();
+    if (rc) return rc;
+    double best = 1e300;
+    TileCand best_c = cand[0];
+    for (size_t q = 0; q < nq; q++) {
+        if (timings) timings[q] = tbest[q] / steps;
+        if (tbest[q] < best) { best = tbest[q]; best_c = cand[q]; }
     }
     if (best_tile_threads) *best_tile_threads = best_c.threads;
     *best_tile_cells = best_c.cells;

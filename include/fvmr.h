@@ -386,7 +386,9 @@ int fvmr_use_graphs(fvmr_handle *h, int on);
 /* Autotuner of the tile kernels (no reference counterpart): times `steps` steps of the uniform
  * freestream of p on mesh m for every candidate work-group shape and tile size (fvmr_autotune_
  * candidates) and returns the fastest pair (set them as p->tile_threads, p->tile_cells) and its
- * ms per step (best_ms may be NULL). */
+ * ms per step (best_ms may be NULL).  It holds one handle per candidate at once, keeps the GPU
+ * busy for at least 40 ms before timing (clocks rise under load) and takes the faster of two
+ * timed runs per candidate, one pass forward and one backward. */
 int fvmr_autotune(const fvmr_mesh_soa *m, const fvmr_params *p, int steps, int *best_tile_threads,
                   int *best_tile_cells, double *best_ms);
 /* The same for a batch (fvmr_create_batch's arguments): one batch handle per candidate; timings
