@@ -1200,21 +1200,74 @@ __device__ __forceinline__ int tile_btype(int4 fr) {
 // takes each tile cell's FP64 state from the staging registers).
 template <class C> using TileState = typename std::conditional<std::is_same<C, float>::value, V4<float>, S4>::type;
 
+// Records in LDS as 16-byte columns: a table of n records of 32 or 64 bytes is 2 or 4 arrays
+// of n 16-byte parts, part k of record i at column k, entry i.  Entry i of a column starts at
+// bank 4i mod 32 (4-byte banks), so the random gathers of a wave spread over all bank groups; a
+// 32-byte record starts at bank 8i mod 32 and a 64-byte one at 16i mod 32, half and a quarter
+// of them.
+template <class T> __device__ __forceinline__ T *lds_col(void *base, int n, int k) {
+    return reinterpret_cast<T *>(base) + k * n;
+}
+template <class T> __device__ __forceinline__ const T *lds_col(const void *base, int n, int k) {
+    return reinterpret_cast<const T *>(base) + k * n;
+}
+__device__ __forceinline__ void lds_put(void *b, int n, int i, const V4<double> &v) {
+    lds_col<V2<double>>(b, n, 0)[i] = V2<double>{v.x, v.y};
+    lds_col<V2<double>>(b, n, 1)[i] = V2<double>{v.z, v.w};
+}
+__device__ __forceinline__ void lds_put(void *b, int n, int i, const V4<float> &v) { lds_col<V4<float>>(b, n, 0)[i] = v; }
+__device__ __forceinline__ void lds_put(void *b, int n, int i, const Grad<double> &g) {
+    lds_col<V2<double>>(b, n, 0)[i] = V2<double>{g.x[0], g.x[1]};
+    lds_col<V2<double>>(b, n, 1)[i] = V2<double>{g.x[2], g.x[3]};
+    lds_col<V2<double>>(b, n, 2)[i] = V2<double>{g.y[0], g.y[1]};
+    lds_col<V2<double>>(b, n, 3)[i] = V2<double>{g.y[2], g.y[3]};
+}
+__device__ __forceinline__ void lds_put(void *b, int n, int i, const Grad<float> &g) {
+    lds_col<V4<float>>(b, n, 0)[i] = V4<float>{g.x[0], g.x[1], g.x[2], g.x[3]};
+    lds_col<V4<float>>(b, n, 1)[i] = V4<float>{g.y[0], g.y[1], g.y[2], g.y[3]};
+}
+template <class T> __device__ __forceinline__ T lds_get(const void *b, int n, int i);
+template <> __device__ __forceinline__ V4<double> lds_get<V4<double>>(const void *b, int n, int i) {
+    const V2<double> p = lds_col<V2<double>>(b, n, 0)[i], q = lds_col<V2<double>>(b, n, 1)[i];
+    return V4<double>{p.x, p.y, q.x, q.y};
+}
+template <> __device__ __forceinline__ V4<float> lds_get<V4<float>>(const void *b, int n, int i) {
+    return lds_col<V4<float>>(b, n, 0)[i];
+}
+template <> __device__ __forceinline__ Grad<double> lds_get<Grad<double>>(const void *b, int n, int i) {
+    const V2<double> a = lds_col<V2<double>>(b, n, 0)[i], c = lds_col<V2<double>>(b, n, 1)[i];
+    const V2<double> d = lds_col<V2<double>>(b, n, 2)[i], e = lds_col<V2<double>>(b, n, 3)[i];
+    Grad<double> g;
+    g.x[0] = a.x; g.x[1] = a.y; g.x[2] = c.x; g.x[3] = c.y;
+    g.y[0] = d.x; g.y[1] = d.y; g.y[2] = e.x; g.y[3] = e.y;
+    return g;
+}
+template <> __device__ __forceinline__ Grad<float> lds_get<Grad<float>>(const void *b, int n, int i) {
+    const V4<float> a = lds_col<V4<float>>(b, n, 0)[i], c = lds_col<V4<float>>(b, n, 1)[i];
+    Grad<float> g;
+    g.x[0] = a.x; g.x[1] = a.y; g.x[2] = a.z; g.x[3] = a.w;
+    g.y[0] = c.x; g.y[1] = c.y; g.y[2] = c.z; g.y[3] = c.w;
+    return g;
+}
+
+// The cells of a tile in LDS (states and gradients in 16-byte columns of nU and nGr entries).
 template <class G, class SU = S4>
 struct TileCells {
-    const SU *sU;
+    const void *sU;
+    int nU;
     const V2<G> *sC;
-    const Grad<G> *sGr;
+    const void *sGr;
+    int nGr;
     __device__ __forceinline__ void state(int c, double u[4]) const {
-        const SU v = sU[c];
+        const SU v = lds_get<SU>(sU, nU, c);
         u[0] = v.x; u[1] = v.y; u[2] = v.z; u[3] = v.w;
     }
     template <class C> __device__ __forceinline__ void state_c(int c, C u[4]) const {
-        const SU v = sU[c];
+        const SU v = lds_get<SU>(sU, nU, c);
         u[0] = (C)v.x; u[1] = (C)v.y; u[2] = (C)v.z; u[3] = (C)v.w;
     }
     __device__ __forceinline__ V2<G> centroid(int c) const { return sC[c]; }
-    __device__ __forceinline__ Grad<G> gradient(int c) const { return sGr[c]; }
+    __device__ __forceinline__ Grad<G> gradient(int c) const { return lds_get<Grad<G>>(sGr, nGr, c); }
     __device__ __forceinline__ bool has(int c) const { return c >= 0; }
 };
 
@@ -1245,16 +1298,15 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
     constexpr int kTileMaxN = Cfg::kMaxN, kTileMaxF = Cfg::kMaxF, kTileMaxFF = Cfg::kMaxFF;
     static_assert(kTileMaxL <= kTileThreads && kTileMaxFF <= kTileThreads, "one local cell and one face per thread");
     using SU = TileState<C>;
-    __shared__ SU sU[kTileMaxL];
+    __shared__ __align__(16) char sU[kTileMaxL * sizeof(SU)];   // states, 16-byte columns of kTileMaxL
     __shared__ V2<double> sN[kTileMaxN];
     __shared__ V2<G> sC[kTileMaxL];
     __shared__ int4 sFace[kTileMaxF];
     __shared__ V2<G> sNrm[kTileMaxF];   // face normals
-    // the face fluxes replace the gradients once every flux is formed
+    // the face fluxes (16-byte columns of kTileMaxFF) replace the gradients (columns of
+    // kTileMaxG) once every flux is formed
     static_assert(kTileMaxFF * sizeof(V4<C>) <= kTileMaxG * sizeof(Grad<G>), "flux buffer overlays the gradients");
     __shared__ __align__(16) char sGrFl[kTileMaxG * sizeof(Grad<G>)];
-    Grad<G> *sGr = reinterpret_cast<Grad<G> *>(sGrFl);
-    V4<C> *sFl = reinterpret_cast<V4<C> *>(sGrFl);
     const int tid = threadIdx.x;
     const int *ti = ta.info + (size_t)xcd_block() * fvmr::kTileInfo;
     const int c0 = ti[fvmr::kTiC0], nT = ti[fvmr::kTiNT], nG = ti[fvmr::kTiNG], nL = ti[fvmr::kTiNL];
@@ -1286,7 +1338,7 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
     if (tid < nN) sN[tid] = nd;
     if (tid < nF) sFace[tid] = fr0;
     if (tid + kTileThreads < nF) sFace[tid + kTileThreads] = fr1;
-    if (i < nL) sU[i] = SU{(C)uin.x, (C)uin.y, (C)uin.z, (C)uin.w};
+    if (i < nL) lds_put(sU, kTileMaxL, i, SU{(C)uin.x, (C)uin.y, (C)uin.z, (C)uin.w});
 #if FVMR_TILE_GEOM_LOADED
     if (i < nL) sC[i] = cc;
     if (tid < nF) sNrm[tid] = nr0;
@@ -1313,7 +1365,7 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
         const C vol = (C)gvol;
         C Uc[4];
         {
-            const SU v = sU[i];
+            const SU v = lds_get<SU>(sU, kTileMaxL, i);
             Uc[0] = (C)v.x; Uc[1] = (C)v.y; Uc[2] = (C)v.z; Uc[3] = (C)v.w;
         }
         const C xc = (C)sC[i].x, yc = (C)sC[i].y;
@@ -1336,7 +1388,7 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
             V2<G> em;
             tile_face_geom_n<G>(sN, sNrm, f, fr, eg, em);
             {
-                const SU v = sU[has_out ? out : i];
+                const SU v = lds_get<SU>(sU, kTileMaxL, has_out ? out : i);
                 sl.u[j][0] = (C)v.x; sl.u[j][1] = (C)v.y; sl.u[j][2] = (C)v.z; sl.u[j][3] = (C)v.w;
             }
             const V2<double> nd = sN[lfield(nw, j)];
@@ -1368,13 +1420,13 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
             g.x[k] = (G)gx[k];
             g.y[k] = (G)gy[k];
         }
-        sGr[i] = g;
+        lds_put(sGrFl, kTileMaxG, i, g);
     }
     __syncthreads();
     // 4. one canonical flux per face of the tile (nFF <= threads), written over the gradients
     const double t_rk = sc[cs].t_rk;
     const CaseBC &bc = ta.cbc[cs];
-    const TileCells<G, SU> cells{sU, sC, sGr};
+    const TileCells<G, SU> cells{sU, kTileMaxL, sC, sGrFl, kTileMaxG};
     C F[4] = {C(0), C(0), C(0), C(0)};
     const int f = tid;
     if (f < nFF) {
@@ -1389,7 +1441,7 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
         face_flux<C, G, NS>(cells, ph, bc, t_rk, L, R == fvmr::kLocalNone ? -1 : R, bType, has_pair, eg, em, pm, F);
     }
     __syncthreads();
-    if (f < nFF) sFl[f] = V4<C>{F[0], F[1], F[2], F[3]};
+    if (f < nFF) lds_put(sGrFl, kTileMaxFF, f, V4<C>{F[0], F[1], F[2], F[3]});
     __syncthreads();
     // 5. cell-owned sum in slot order, /(-V), RK update (as k_stage); LDT: the cell's own dt
     double dt = sc[cs].dt;
@@ -1400,7 +1452,7 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            const V4<C> fv = sFl[sw[j] & fvmr::kLocalMask];
+            const V4<C> fv = lds_get<V4<C>>(sGrFl, kTileMaxFF, sw[j] & fvmr::kLocalMask);
             const double s = (sw[j] & fvmr::kGsIsR) ? -1.0 : 1.0;
             acc[0] += s * (double)fv.x;
             acc[1] += s * (double)fv.y;
@@ -1412,7 +1464,7 @@ __global__ __launch_bounds__((TileCfg<C, SHAPE>::kThreads)) void k_tile(TileArgs
         // registers in mixed precision (its LDS copy is FP32)
         double Ui[4];
         if (std::is_same<SU, S4>::value) {
-            const SU v = sU[i];
+            const SU v = lds_get<SU>(sU, kTileMaxL, i);
             Ui[0] = v.x; Ui[1] = v.y; Ui[2] = v.z; Ui[3] = v.w;
         } else {
             Ui[0] = uin.x; Ui[1] = uin.y; Ui[2] = uin.z; Ui[3] = uin.w;
