@@ -1250,6 +1250,11 @@ template <> __device__ __forceinline__ Grad<float> lds_get<Grad<float>>(const vo
     return g;
 }
 
+__device__ __forceinline__ void lds_state(const void *b, int n, int i, double u[4]) {
+    const V4<double> v = lds_get<V4<double>>(b, n, i);
+    u[0] = v.x; u[1] = v.y; u[2] = v.z; u[3] = v.w;
+}
+
 // The cells of a tile in LDS (states and gradients in 16-byte columns of nU and nGr entries).
 template <class G, class SU = S4>
 struct TileCells {
@@ -1510,7 +1515,7 @@ __global__ __launch_bounds__((TileCfg<G, SHAPE>::kThreads)) void k_tile_dt(TileA
     using Cfg = TileCfg<G, SHAPE>;
     constexpr int kTileThreads = Cfg::kThreads, kTileMaxG = Cfg::kMaxG, kTileMaxN = Cfg::kMaxN;
     constexpr int kTileMaxFF = Cfg::kMaxFF;
-    __shared__ S4 sU[kTileMaxG];
+    __shared__ __align__(16) char sU[kTileMaxG * sizeof(S4)];   // states, 16-byte columns (lds_put)
     __shared__ V2<double> sN[kTileMaxN];
     __shared__ int4 sFace[kTileMaxFF];   // faces of the tile cells
     __shared__ double sAi[kTileMaxFF], sAv[kTileMaxFF];
@@ -1531,7 +1536,7 @@ __global__ __launch_bounds__((TileCfg<G, SHAPE>::kThreads)) void k_tile_dt(TileA
     const S4 u = (src >= 0) ? U[src] : S4{0.0, 0.0, 0.0, 0.0};
     if (tid < nN) sN[tid] = nd;
     if (tid < nFF) sFace[tid] = fr;
-    if (tid < nG) sU[tid] = u;
+    if (tid < nG) lds_put(sU, kTileMaxG, tid, u);
     __syncthreads();
     // faces of the tile cells (one per thread): L and R (when present) are tile or first-ring cells
     static_assert(kTileMaxFF <= kTileThreads, "one face per thread");
@@ -1542,9 +1547,9 @@ __global__ __launch_bounds__((TileCfg<G, SHAPE>::kThreads)) void k_tile_dt(TileA
         V2<G> em;
         tile_face_geom<G>(sN, fr, eg, em);
         double uL[4], rL[4], uR[4], rR[4];
-        ld(sU, L, uL);
+        lds_state(sU, kTileMaxG, L, uL);
         const bool has_R = R != fvmr::kLocalNone;
-        ld(sU, has_R ? R : L, uR);
+        lds_state(sU, kTileMaxG, has_R ? R : L, uR);
         double av, vv;
         if (!LDT && !std::is_same<G, double>::value) {
             // mixed precision: the face's primitives and spectral radius in FP32 (the per-cell
@@ -1618,9 +1623,9 @@ __global__ __launch_bounds__((TileCfg<G, SHAPE>::kThreads)) void k_tile_dt(TileA
                 V2<G> em;
                 tile_face_geom<G>(sN, fr, eg, em);
                 double Uc[4], rc[4], Uo[4], ro[4];
-                ld(sU, i, Uc);
+                lds_state(sU, kTileMaxG, i, Uc);
                 prim(Uc, rc, ph.gamma);
-                ld(sU, out, Uo);
+                lds_state(sU, kTileMaxG, out, Uo);
                 prim(Uo, ro, ph.gamma);
                 edge_alpha(ph, rc, ro, true, (double)eg.x, (double)eg.y, (double)eg.z, vol_c, &av, &vv);
             }
